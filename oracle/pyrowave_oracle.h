@@ -1,0 +1,183 @@
+/*
+ * pyrowave_oracle.h -- CPU restatement of the PyroWave encode/decode hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP product
+ * path is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The shipped library
+ * (pyrowave_amd/libpyrowave-shared.so) never links, loads or calls it.
+ *
+ * Every function cites the reference file:line it restates.  Reference files
+ * are named relative to the Themaister/pyrowave source tree.
+ *
+ * Pinning status of each part (what checks it besides the restated formula):
+ *   layout / block index order ... golden fixtures from the reference's own
+ *                                  metal/pyrowave_bitstream.cpp (oracle/_ref)
+ *   packetize / num_packets ...... same fixtures
+ *   decoder packet parser ........ same fixtures (push/ready/clear sequences)
+ *   FP64 DWT / iDWT .............. perfect reconstruction, constant input ->
+ *                                  zero high bands, cubic vanishing moments,
+ *                                  band orientation, brute-force convolution
+ *                                  with the textbook 9/7 analysis filters
+ *   FP32 DWT / iDWT .............. error bound against the FP64 transform
+ *   fp16 conversion .............. exhaustive comparison with numpy float16
+ *   quant scale / magnitudes ..... closed-form dequant error bound
+ *                                  |c - c'| <= step/2, M in [1,1.25) -> zero
+ *   stats / analyze / buckets .... brute-force per-block recomputation in the
+ *                                  tests, monotone word counts, bucket
+ *                                  strictly increasing in q
+ *   finalize (quirk Q1) .......... literal simulation of the reference's
+ *                                  512-thread Hillis-Steele loop
+ *   resolve ...................... literal simulation of the reference's
+ *                                  per-(bucket,subdivision) greedy loop
+ *   pack ......................... structural validator (restates
+ *                                  validate_bitstream), decode round trip,
+ *                                  analyzed words == packed words
+ *   dequant / iDWT output ........ round trip |decoded - input| <= 1 at a
+ *                                  64 KiB budget (pyrowave_c_test.cpp:347-379)
+ *   parity unpinned (byte level vs the reference GPU codec): GLSL log2
+ *   rounding at bucket boundaries, GLSL division rounding in the scale code,
+ *   float summation order of subgroup reductions, UNORM rounding on store.
+ */
+#ifndef PYROWAVE_ORACLE_H
+#define PYROWAVE_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OR_LEVELS = 5, OR_COMPONENTS = 3, OR_BANDS = 4 };
+
+typedef struct or_band
+{
+	int present;       /* transmitted band (has block indices) */
+	int bw, bh;        /* band size in coefficients */
+	int bx8, by8;      /* 8x8 block grid */
+	int bx32, by32;    /* 32x32 block grid */
+	int off8, off32;   /* first 8x8 / 32x32 block index */
+	long coeff_off;    /* offset of this band in a flat coefficient array */
+} or_band;
+
+typedef struct or_layout
+{
+	int width, height, chroma; /* chroma: 0 = 4:2:0, 1 = 4:4:4 */
+	int aligned_w, aligned_h;
+	int count8, count32;
+	long coeff_count;          /* total floats of the flat coefficient array */
+	int per_subdivision;       /* next_pow2(ceil(count32 / 16)) */
+	or_band band[OR_COMPONENTS][OR_LEVELS][OR_BANDS];
+} or_layout;
+
+/* 32x32 block -> band lookup, in block-index order. */
+typedef struct or_block_desc
+{
+	int16_t component, level, band, pad;
+	int32_t bx, by;            /* 32x32 position within the band */
+} or_block_desc;
+
+int or_layout_init(or_layout *L, int width, int height, int chroma);
+long or_layout_size(void);
+int or_block_descs(const or_layout *L, or_block_desc *out);
+
+/* Encoder-side quantizer constants (pyrowave_encoder.cpp:107-172, 458-459). */
+float or_quant_resolution(int level, int component, int band);
+float or_rdo_distortion_scale(const or_layout *L, int level, int component, int band);
+uint8_t or_encode_quant(float decoder_q_scale);
+float or_decode_quant(uint8_t code);
+
+/* fp16 helpers (round to nearest even, subnormals kept). */
+uint16_t or_f32_to_f16(float f);
+float or_f16_to_f32(uint16_t h);
+void or_round_to_f16_array(float *v, long n);
+
+/* Forward extension index for the encoder's sampler trick (dwt.comp:27-36). */
+int or_fwd_ext(int p, int R, int A);
+
+/* One 2-D DWT level, FP64 and FP32.  `in` has R_y rows of R_x valid samples
+ * (pitch in elements); the transform domain is A_x x A_y; the four outputs are
+ * (A_x/2) x (A_y/2) with pitch out_pitch.  dc_shift subtracts 0.5 from inputs
+ * in the u8 variants. */
+void or_dwt_level_f64(const double *in, int in_pitch, int R_x, int R_y, int A_x, int A_y,
+                      double *ll, double *hl, double *lh, double *hh, int out_pitch);
+void or_dwt_level_f32(const float *in, int in_pitch, int R_x, int R_y, int A_x, int A_y,
+                      float *ll, float *hl, float *lh, float *hh, int out_pitch);
+void or_idwt_level_f64(const double *ll, const double *hl, const double *lh, const double *hh,
+                       int in_pitch, int bw, int bh, double *out, int out_pitch);
+void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const float *hh,
+                       int in_pitch, int bw, int bh, float *out, int out_pitch);
+
+/* Whole-frame forward transform into the flat coefficient array.
+ * precision 2: FP32 everywhere.  precision 1: FP32 math, bands of levels 0-1
+ * (and the LL fed forward from them) rounded to fp16 storage.
+ * use_f64 != 0: FP64 math; result written to coeffs64 (may be NULL otherwise). */
+int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
+                 int precision, float *coeffs, double *coeffs64);
+
+/* Quantizer + analyze (wavelet_quant.comp, analyze_rate_control.comp).
+ * Outputs, per 32x32 block b: words[b*16+q], bucket[b*16+q] (inclusive bucket,
+ * 0 for q == 0); per 8x8: code8 (code word at q=0), np8 (num_planes).
+ * totals[2048] receives the per-(bucket,subdivision) savings, *consumed the
+ * sum of words at q=0. */
+int or_quant_analyze(const or_layout *L, const float *coeffs,
+                     uint16_t *words, uint8_t *bucket, uint32_t *code8, uint8_t *np8,
+                     uint32_t *totals, uint32_t *consumed);
+
+/* analyze_rate_control_finalize.comp.  q1 != 0 reproduces the reference's
+ * 8-step scan (windowed sums for flat index >= 1024); q1 == 0 is a true
+ * inclusive prefix sum. */
+void or_finalize(const uint32_t *totals, uint32_t *prefix, int q1);
+
+/* resolve_rate_control.comp: greedy truncation; writes quant[count32]. */
+int or_resolve(const or_layout *L, const uint16_t *words, const uint8_t *bucket,
+               const uint32_t *prefix, uint32_t consumed, size_t budget, int32_t *quant);
+
+/* block_packing.comp: writes blocks in block-index order.  meta[2*b] = offset
+ * in words (0 for empty blocks), meta[2*b+1] = words.  Returns total words or
+ * -1 if bitstream_capacity_words is too small. */
+long or_pack(const or_layout *L, const float *coeffs, const int32_t *quant, int sequence,
+             uint32_t *bitstream, long bitstream_capacity_words, uint32_t *meta);
+
+/* Full encode = dwt + quant_analyze + finalize + resolve + pack. */
+long or_encode(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
+               int precision, size_t budget, int sequence, int q1,
+               uint32_t *bitstream, long capacity_words, uint32_t *meta, int32_t *quant_out);
+
+/* Encoder::Impl::compute_num_packets / packetize (pyrowave_encoder.cpp:596-623, 1048-1115). */
+size_t or_compute_num_packets(const or_layout *L, const uint32_t *meta, size_t boundary);
+size_t or_packetize(const or_layout *L, const uint32_t *meta, const uint32_t *bitstream, int sequence,
+                    size_t boundary, uint8_t *out, size_t out_size, size_t *packets /* 2 per packet */);
+
+/* Structural check of one coded block (Encoder::Impl::validate_bitstream,
+ * pyrowave_encoder.cpp:960-1046).  Returns 1 if valid. */
+int or_validate_block(const or_layout *L, const uint32_t *block_words, uint32_t block_index, uint32_t meta_words);
+
+/* Decoder packet parser (pyrowave_decoder.cpp:126-262, 780-794, 832-840). */
+typedef struct or_decoder or_decoder;
+or_decoder *or_decoder_create(const or_layout *L);
+void or_decoder_destroy(or_decoder *d);
+void or_decoder_clear(or_decoder *d);
+int or_decoder_push_packet(or_decoder *d, const void *data, size_t size);
+int or_decoder_is_ready(const or_decoder *d, int allow_partial);
+void or_decoder_mark_decoded(or_decoder *d);
+long or_decoder_payload_words(const or_decoder *d);
+const uint32_t *or_decoder_payload(const or_decoder *d);
+const uint32_t *or_decoder_offsets(const or_decoder *d);
+int or_decoder_decoded_blocks(const or_decoder *d);
+
+/* wavelet_dequant.comp: writes every transmitted band of `coeffs`. */
+int or_dequant(const or_layout *L, const uint32_t *payload, long payload_words,
+               const uint32_t *offsets, float *coeffs);
+
+/* idwt.comp chain: levels 4 -> 0, u8 output planes with DC shift and clamp.
+ * precision 1 rounds the LL outputs of levels 1 and 2 (stored at levels 0/1)
+ * to fp16 like the encoder's storage policy. */
+int or_idwt_frame(const or_layout *L, const float *coeffs, int precision,
+                  uint8_t *const planes[3], const int pitches[3]);
+int or_idwt_frame_f64(const or_layout *L, const double *coeffs, double *const planes[3], const int pitches[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
