@@ -1,0 +1,185 @@
+/*
+ * pyrowave_hip.h -- HIP-native entry points of the MI355X PyroWave codec.
+ *
+ * These replace the reference's Vulkan-image paths
+ * (pyrowave_encoder_encode_gpu_synchronous / pyrowave_decoder_decode_gpu_buffer,
+ * reference pyrowave.h:457-462 and 528-532) with device pointers and a
+ * hipStream_t, in the shape of the reference's Vulkan-free Metal API
+ * (reference metal/pyrowave_metal.h:175-198, 266-291): result codes keep the
+ * meanings of pyrowave_result.  They add batching: one call encodes or decodes
+ * frames of different sizes and chroma formats with one launch per kernel
+ * stage.
+ *
+ * Streams are passed as `void *` holding a hipStream_t (NULL = the device's
+ * own stream).  All calls are asynchronous with respect to the host unless
+ * stated otherwise.
+ */
+#ifndef PYROWAVE_HIP_H_
+#define PYROWAVE_HIP_H_
+
+#include "pyrowave.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum pyrowave_hip_precision
+{
+	/* Packed FP16 lifting for levels 0-1 with FP16 band storage there; FP32
+	 * for levels 2-4.  Quantizer policy as the reference's precision >= 1. */
+	PYROWAVE_HIP_PRECISION_FP16 = 1,
+	/* FP32 lifting and storage everywhere (reference precision 2). */
+	PYROWAVE_HIP_PRECISION_FP32 = 2,
+	PYROWAVE_HIP_PRECISION_INT_MAX = 0x7fffffff
+} pyrowave_hip_precision;
+
+typedef enum pyrowave_hip_encoder_flag_bits
+{
+	/* Rate-control finalize computes a true prefix sum over all 2048
+	 * (bucket, subdivision) entries.  Without the flag the reference's scan is
+	 * reproduced, whose 8 Hillis-Steele steps leave entries at flat index
+	 * >= 1024 holding 256-thread windowed sums
+	 * (reference shaders/analyze_rate_control_finalize.comp:244). */
+	PYROWAVE_HIP_ENCODER_EXACT_PREFIX_BIT = 1u << 0
+} pyrowave_hip_encoder_flag_bits;
+
+typedef struct pyrowave_hip_encoder_create_info
+{
+	pyrowave_device device;
+	int width;
+	int height;
+	pyrowave_chroma_subsampling chroma;
+	pyrowave_hip_precision precision;
+	uint32_t flags; /* pyrowave_hip_encoder_flag_bits */
+} pyrowave_hip_encoder_create_info;
+
+typedef struct pyrowave_hip_decoder_create_info
+{
+	pyrowave_device device;
+	int width;
+	int height;
+	pyrowave_chroma_subsampling chroma;
+	pyrowave_hip_precision precision;
+} pyrowave_hip_decoder_create_info;
+
+/* One 8-bit plane in device memory.  pixel_stride_in_bytes is 1 for planar
+ * data and 2 for an interleaved NV12 CbCr plane (pass data + 1 for Cr). */
+typedef struct pyrowave_hip_plane
+{
+	const void *data;
+	size_t row_stride_in_bytes;
+	size_t pixel_stride_in_bytes;
+} pyrowave_hip_plane;
+
+typedef struct pyrowave_hip_frame
+{
+	pyrowave_hip_plane planes[3];
+} pyrowave_hip_frame;
+
+typedef struct pyrowave_hip_output_plane
+{
+	void *data;
+	size_t row_stride_in_bytes;
+} pyrowave_hip_output_plane;
+
+typedef struct pyrowave_hip_decode_target
+{
+	pyrowave_hip_output_plane planes[3];
+} pyrowave_hip_decode_target;
+
+/* Device-resident result of an encoder's last encode. */
+typedef struct pyrowave_hip_encoded_frame
+{
+	/* Coded 32x32 blocks, contiguous, in block-index order. */
+	const uint32_t *bitstream;
+	/* Two words per 32x32 block: offset into bitstream (words), word count
+	 * (0 = block not coded). */
+	const uint32_t *meta;
+	uint32_t num_blocks;
+	uint32_t sequence;
+} pyrowave_hip_encoded_frame;
+
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_create(const pyrowave_hip_encoder_create_info *info, pyrowave_encoder *encoder);
+
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decoder_create(const pyrowave_hip_decoder_create_info *info, pyrowave_decoder *decoder);
+
+/* Encodes frames[i] with encoders[i] for i < count (encoders must be distinct
+ * and belong to one device; sizes and chroma formats may differ).  Each frame
+ * gets its own rate control with rate_controls[i].  Results stay on the device
+ * (pyrowave_hip_encoder_get_result); pyrowave_encoder_compute_num_packets and
+ * pyrowave_encoder_packetize work on them as after a CPU-buffer encode. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encode_batch(pyrowave_encoder *encoders, const pyrowave_hip_frame *frames,
+                          const pyrowave_rate_control *rate_controls, uint32_t count, void *hip_stream);
+
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_get_result(pyrowave_encoder encoder, pyrowave_hip_encoded_frame *result);
+
+/* Decodes the packets queued in decoders[i] (pyrowave_decoder_push_packet)
+ * into targets[i].  The payload is uploaded on the given stream. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decode_batch(pyrowave_decoder *decoders, const pyrowave_hip_decode_target *targets,
+                          uint32_t count, void *hip_stream);
+
+/* Decodes encoders[i]'s device-resident result with decoders[i] into
+ * targets[i] without a host round trip (the serving path).  Frame sizes and
+ * chroma formats of each pair must match. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decode_encoded_batch(pyrowave_decoder *decoders, pyrowave_encoder *encoders,
+                                  const pyrowave_hip_decode_target *targets, uint32_t count, void *hip_stream);
+
+/* ---- Introspection (synchronous; for parity tests and tools) ---- */
+
+/* Number of floats of the flat coefficient layout for a frame geometry:
+ * for component 0..2, level 0..4 (chroma level 0 absent under 4:2:0),
+ * band 0..3 (LL, HL, LH, HH), a dense row-major band of
+ * (aligned_width/2 >> level) x (aligned_height/2 >> level) values, where
+ * aligned = max(align(size, 32), 128).  LL bands below level 4 are the
+ * transform's intermediates. */
+PYROWAVE_PUBLIC_API size_t
+pyrowave_hip_coefficient_count(int width, int height, pyrowave_chroma_subsampling chroma);
+
+/* Copies the encoder's wavelet bands of the last encode, as float. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_read_coefficients(pyrowave_encoder encoder, float *coefficients, size_t count);
+
+/* Quantizes, rate-controls and packs caller-supplied coefficients (flat
+ * layout, converted to the encoder's storage precision) instead of running
+ * the forward transform: the reference's Encoder::encode_pre_transformed
+ * (reference pyrowave_encoder.cpp:1138-1151) with the encoder's own quantizer
+ * resolutions. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_encode_coefficients(pyrowave_encoder encoder, const float *coefficients, size_t count,
+                                         const pyrowave_rate_control *rate_control);
+
+/* Rate-control state of the last encode.  Any pointer may be NULL.
+ * words/buckets: 16 entries per 32x32 block (quant level 0..15);
+ * totals/prefix: 2048 entries; quant: final truncation per block. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_read_rate_control(pyrowave_encoder encoder, uint16_t *words, uint8_t *buckets,
+                                       uint32_t *totals, uint32_t *prefix, uint32_t *consumed,
+                                       int32_t *quant);
+
+/* Copies the decoder's dequantized bands of the last decode, as float. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decoder_read_coefficients(pyrowave_decoder decoder, float *coefficients, size_t count);
+
+/* Per-stage GPU time in milliseconds, summed since the last reset, and the
+ * number of frames it covers.  Stage order: DWT, Quant, Analyze, Resolve,
+ * Packing, Dequant, iDWT (Quant and Analyze run as one fused kernel; its time
+ * is reported under Quant and Analyze reads 0). */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_device_query_stage_times(pyrowave_device device, double ms[7], uint64_t frames[7], bool reset);
+
+/* Enables or disables per-stage event timing (on by default; off removes the
+ * event records from the launch sequence). */
+PYROWAVE_PUBLIC_API void
+pyrowave_hip_device_set_timing(pyrowave_device device, bool enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

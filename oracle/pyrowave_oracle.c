@@ -639,7 +639,7 @@ int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int p
 				                 coeffs + B[0].coeff_off, coeffs + B[1].coeff_off,
 				                 coeffs + B[2].coeff_off, coeffs + B[3].coeff_off, B[0].bw);
 				if (precision == 1 && l < 2)
-					for (int b = 0; b < 4; b++)
+					for (int b = 1; b < 4; b++)
 						or_round_to_f16_array(coeffs + B[b].coeff_off, (long)B[b].bw * B[b].bh);
 			}
 			if (l == first_level)
@@ -694,7 +694,10 @@ static void quantize_8x8(const float *band, int pitch, int bw, int bh, int x0, i
 		{
 			int x = sx + (p >> 1), y = sy + (p & 1);
 			float v = (x < bw && y < bh) ? band[(size_t)y * pitch + x] : 0.0f;
-			float t = v * quant_resolution;
+			/* Magnitudes beyond 16383.75 would need more than the 14 planes a
+			 * block can describe; they are clamped (only reachable far outside
+			 * the nominal coefficient range; the reference has no guard). */
+			float t = fminf(fmaxf(v * quant_resolution, -16383.75f), 16383.75f);
 			Q->t[s][p] = t;
 			float a = fabsf(t);
 			if (a > max_sub[s])
@@ -754,10 +757,9 @@ static void quantize_8x8(const float *band, int pitch, int bw, int bh, int x0, i
 		return;
 	}
 
+	/* With |t| <= 16383.75 the scale code keeps maxabs <= 16383, so
+	 * num_planes = msb + 1 <= 14 fits the 15 BlockStats entries. */
 	int msb = find_msb((uint32_t)Q->maxabs);
-	/* BlockStats holds 15 entries, so num_planes = msb + 1 must stay <= 14. */
-	if (msb > 13)
-		msb = 13;
 	Q->msb = msb;
 
 	uint16_t cost_tab[15];
@@ -1702,6 +1704,11 @@ int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
 				continue;
 			const or_band *B = L->band[c][l];
 			int bw = B[0].bw, bh = B[0].bh;
+			/* precision 1: the decoder holds the high bands of levels 0-1 in
+			 * fp16 storage; LL intermediates stay FP32. */
+			if (precision == 1 && l < 2)
+				for (int b = 1; b < 4; b++)
+					or_round_to_f16_array(coeffs + B[b].coeff_off, (long)B[b].bw * B[b].bh);
 			if (l == first_level)
 			{
 				int pw, ph;
@@ -1719,8 +1726,6 @@ int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
 				const or_band *P = &L->band[c][l - 1][0];
 				or_idwt_level_f32(coeffs + B[0].coeff_off, coeffs + B[1].coeff_off, coeffs + B[2].coeff_off,
 				                  coeffs + B[3].coeff_off, bw, bw, bh, coeffs + P->coeff_off, P->bw);
-				if (precision == 1 && l - 1 < 2)
-					or_round_to_f16_array(coeffs + P->coeff_off, (long)P->bw * P->bh);
 			}
 		}
 	}

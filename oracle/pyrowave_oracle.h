@@ -109,8 +109,8 @@ void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const 
                        int in_pitch, int bw, int bh, float *out, int out_pitch);
 
 /* Whole-frame forward transform into the flat coefficient array.
- * precision 2: FP32 everywhere.  precision 1: FP32 math, bands of levels 0-1
- * (and the LL fed forward from them) rounded to fp16 storage.
+ * precision 2: FP32 everywhere.  precision 1: FP32 math; the high bands
+ * (HL, LH, HH) of levels 0-1 rounded to fp16 storage, LL kept FP32.
  * use_f64 != 0: FP64 math; result written to coeffs64 (may be NULL otherwise). */
 int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                  int precision, float *coeffs, double *coeffs64);
@@ -171,8 +171,8 @@ int or_dequant(const or_layout *L, const uint32_t *payload, long payload_words,
                const uint32_t *offsets, float *coeffs);
 
 /* idwt.comp chain: levels 4 -> 0, u8 output planes with DC shift and clamp.
- * precision 1 rounds the LL outputs of levels 1 and 2 (stored at levels 0/1)
- * to fp16 like the encoder's storage policy. */
+ * precision 1 rounds the input high bands of levels 0-1 to fp16 first (the
+ * decoder's fp16 band storage). */
 int or_idwt_frame(const or_layout *L, const float *coeffs, int precision,
                   uint8_t *const planes[3], const int pitches[3]);
 int or_idwt_frame_f64(const or_layout *L, const double *coeffs, double *const planes[3], const int pitches[3]);

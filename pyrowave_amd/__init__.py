@@ -1,0 +1,433 @@
+"""pyrowave_amd -- Python front end of the MI355X PyroWave codec.
+
+Thin ctypes bindings of libpyrowave-shared.so, the HIP implementation behind
+the reference's pyrowave.h C API.  The classes mirror the C API one to one
+(Device -> pyrowave_device, Encoder -> pyrowave_encoder, Decoder ->
+pyrowave_decoder) with the same argument meaning; a non-success
+pyrowave_result raises PyroWaveError carrying the code.
+
+The compute path is the HIP library only: importing this module does not
+fall back to anything else, and using it without the built library raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpyrowave-shared.so")
+
+# pyrowave_result (pyrowave.h)
+SUCCESS = 0
+TIMEOUT = 1
+ERROR_GENERIC = -1
+ERROR_INVALID_ARGUMENT = -2
+ERROR_OUT_OF_HOST_MEMORY = -3
+ERROR_OUT_OF_DEVICE_MEMORY = -4
+ERROR_NO_VULKAN = -5
+ERROR_NOT_IMPLEMENTED = -6
+
+CHROMA_420 = 0
+CHROMA_444 = 1
+
+FORMAT_NV12 = 0
+FORMAT_YUV420P = 1
+FORMAT_YUV444P = 2
+
+PRECISION_FP16 = 1
+PRECISION_FP32 = 2
+
+EXACT_PREFIX = 1
+
+VK_QUEUE_GRAPHICS_BIT = 1
+VK_QUEUE_COMPUTE_BIT = 2
+
+STAGES = ("DWT", "Quant", "Analyze", "Resolve", "Packing", "Dequant", "iDWT")
+
+
+class PyroWaveError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__(f"{what} failed with pyrowave_result {code}")
+        self.code = code
+
+
+class EncoderCreateInfo(C.Structure):
+    _fields_ = [("device", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("chroma", C.c_int)]
+
+
+class DecoderCreateInfo(C.Structure):
+    _fields_ = [("device", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("chroma", C.c_int),
+                ("fragment_path", C.c_bool)]
+
+
+class HipEncoderCreateInfo(C.Structure):
+    _fields_ = [("device", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("chroma", C.c_int),
+                ("precision", C.c_int), ("flags", C.c_uint32)]
+
+
+class HipDecoderCreateInfo(C.Structure):
+    _fields_ = [("device", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("chroma", C.c_int),
+                ("precision", C.c_int)]
+
+
+class CpuBuffer(C.Structure):
+    _fields_ = [("data", C.c_void_p * 3), ("row_stride_in_bytes", C.c_size_t * 3),
+                ("plane_size_in_bytes", C.c_size_t * 3), ("width", C.c_int), ("height", C.c_int),
+                ("format", C.c_int)]
+
+
+class RateControl(C.Structure):
+    _fields_ = [("maximum_bitstream_size", C.c_size_t)]
+
+
+class Packet(C.Structure):
+    _fields_ = [("offset", C.c_size_t), ("size", C.c_size_t)]
+
+
+class HipPlane(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("row_stride_in_bytes", C.c_size_t), ("pixel_stride_in_bytes", C.c_size_t)]
+
+
+class HipFrame(C.Structure):
+    _fields_ = [("planes", HipPlane * 3)]
+
+
+class HipOutputPlane(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("row_stride_in_bytes", C.c_size_t)]
+
+
+class HipDecodeTarget(C.Structure):
+    _fields_ = [("planes", HipOutputPlane * 3)]
+
+
+class HipEncodedFrame(C.Structure):
+    _fields_ = [("bitstream", C.c_void_p), ("meta", C.c_void_p), ("num_blocks", C.c_uint32),
+                ("sequence", C.c_uint32)]
+
+
+MESSAGE_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
+
+_lib = None
+
+# Every symbol pyrowave.h and pyrowave_hip.h declare.
+EXPORTS = (
+    "pyrowave_get_api_version", "pyrowave_create_default_device", "pyrowave_create_device_by_compat",
+    "pyrowave_create_device", "pyrowave_device_set_command_buffer", "pyrowave_device_report_performance_stats",
+    "pyrowave_device_get_vk_device_handles", "pyrowave_device_confirm_interop_support",
+    "pyrowave_device_set_queue_type", "pyrowave_device_destroy", "pyrowave_sync_object_create",
+    "pyrowave_sync_object_get_semaphore", "pyrowave_sync_object_export_handle", "pyrowave_sync_object_cpu_wait",
+    "pyrowave_sync_object_cpu_signal", "pyrowave_sync_object_destroy", "pyrowave_image_create",
+    "pyrowave_image_get_handle", "pyrowave_image_get_image_view", "pyrowave_image_destroy",
+    "pyrowave_encoder_create", "pyrowave_encoder_encode_gpu_synchronous", "pyrowave_encoder_encode_cpu_synchronous",
+    "pyrowave_encoder_compute_num_packets", "pyrowave_encoder_packetize", "pyrowave_encoder_destroy",
+    "pyrowave_decoder_device_prefers_fragment_path", "pyrowave_decoder_create", "pyrowave_decoder_clear",
+    "pyrowave_decoder_push_packet", "pyrowave_decoder_decode_is_ready", "pyrowave_decoder_decode_gpu_buffer",
+    "pyrowave_decoder_decode_cpu_buffer_synchronous", "pyrowave_decoder_destroy",
+    "pyrowave_hip_encoder_create", "pyrowave_hip_decoder_create", "pyrowave_hip_encode_batch",
+    "pyrowave_hip_encoder_get_result", "pyrowave_hip_decode_batch", "pyrowave_hip_decode_encoded_batch",
+    "pyrowave_hip_coefficient_count", "pyrowave_hip_encoder_read_coefficients",
+    "pyrowave_hip_encoder_encode_coefficients", "pyrowave_hip_encoder_read_rate_control",
+    "pyrowave_hip_decoder_read_coefficients", "pyrowave_hip_device_query_stage_times",
+    "pyrowave_hip_device_set_timing",
+)
+
+
+def load():
+    """Loads the HIP library; raises if it has not been built.
+
+    One process holds one HIP runtime: PyTorch ships its own libamdhip64, and
+    a second copy loaded beside it leaves torch without a device.  So torch,
+    when installed, is imported first and the library binds to its runtime
+    (set PYROWAVE_NO_TORCH=1 to skip that in processes that never use torch).
+    """
+    global _lib
+    if _lib is not None:
+        return _lib
+    if os.environ.get("PYROWAVE_NO_TORCH") != "1":
+        try:
+            import torch  # noqa: F401
+        except Exception:
+            pass
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pyrowave_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    P, I, U32, SZ = C.c_void_p, C.c_int, C.c_uint32, C.c_size_t
+    sig = {
+        "pyrowave_get_api_version": (None, [C.POINTER(U32)] * 3),
+        "pyrowave_create_default_device": (I, [C.POINTER(P)]),
+        "pyrowave_create_device_by_compat": (I, [U32, U32, P, P, P, C.POINTER(P)]),
+        "pyrowave_create_device": (I, [P, C.POINTER(P)]),
+        "pyrowave_device_report_performance_stats": (None, [P, MESSAGE_CB, P, C.c_bool]),
+        "pyrowave_device_set_queue_type": (I, [P, I]),
+        "pyrowave_device_confirm_interop_support": (C.c_bool, [P]),
+        "pyrowave_device_destroy": (None, [P]),
+        "pyrowave_encoder_create": (I, [C.POINTER(EncoderCreateInfo), C.POINTER(P)]),
+        "pyrowave_encoder_encode_gpu_synchronous": (I, [P, P, P, P, P]),
+        "pyrowave_encoder_encode_cpu_synchronous": (I, [P, C.POINTER(CpuBuffer), C.POINTER(RateControl)]),
+        "pyrowave_encoder_compute_num_packets": (I, [P, SZ, C.POINTER(SZ)]),
+        "pyrowave_encoder_packetize": (I, [P, P, SZ, C.POINTER(SZ), P, SZ]),
+        "pyrowave_encoder_destroy": (None, [P]),
+        "pyrowave_decoder_device_prefers_fragment_path": (C.c_bool, [P]),
+        "pyrowave_decoder_create": (I, [C.POINTER(DecoderCreateInfo), C.POINTER(P)]),
+        "pyrowave_decoder_clear": (None, [P]),
+        "pyrowave_decoder_push_packet": (I, [P, P, SZ]),
+        "pyrowave_decoder_decode_is_ready": (C.c_bool, [P, C.c_bool]),
+        "pyrowave_decoder_decode_gpu_buffer": (I, [P, P, P, P]),
+        "pyrowave_decoder_decode_cpu_buffer_synchronous": (I, [P, C.POINTER(CpuBuffer)]),
+        "pyrowave_decoder_destroy": (None, [P]),
+        "pyrowave_hip_encoder_create": (I, [C.POINTER(HipEncoderCreateInfo), C.POINTER(P)]),
+        "pyrowave_hip_decoder_create": (I, [C.POINTER(HipDecoderCreateInfo), C.POINTER(P)]),
+        "pyrowave_hip_encode_batch": (I, [P, P, P, U32, P]),
+        "pyrowave_hip_encoder_get_result": (I, [P, C.POINTER(HipEncodedFrame)]),
+        "pyrowave_hip_decode_batch": (I, [P, P, U32, P]),
+        "pyrowave_hip_decode_encoded_batch": (I, [P, P, P, U32, P]),
+        "pyrowave_hip_coefficient_count": (SZ, [I, I, I]),
+        "pyrowave_hip_encoder_read_coefficients": (I, [P, P, SZ]),
+        "pyrowave_hip_encoder_encode_coefficients": (I, [P, P, SZ, C.POINTER(RateControl)]),
+        "pyrowave_hip_encoder_read_rate_control": (I, [P, P, P, P, P, P, P]),
+        "pyrowave_hip_decoder_read_coefficients": (I, [P, P, SZ]),
+        "pyrowave_hip_device_query_stage_times": (I, [P, P, P, C.c_bool]),
+        "pyrowave_hip_device_set_timing": (None, [P, C.c_bool]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code, what):
+    if code != SUCCESS:
+        raise PyroWaveError(code, what)
+
+
+def api_version():
+    a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+    load().pyrowave_get_api_version(C.byref(a), C.byref(b), C.byref(c))
+    return a.value, b.value, c.value
+
+
+def coefficient_count(width, height, chroma):
+    return int(load().pyrowave_hip_coefficient_count(width, height, chroma))
+
+
+def plane_shape(width, height, chroma, c):
+    if c and chroma == CHROMA_420:
+        return height // 2, width // 2
+    return height, width
+
+
+class Device:
+    def __init__(self):
+        h = C.c_void_p()
+        _check(load().pyrowave_create_default_device(C.byref(h)), "pyrowave_create_default_device")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().pyrowave_device_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_queue_type(self, flags):
+        return load().pyrowave_device_set_queue_type(self.handle, flags)
+
+    def report_performance_stats(self, reset=False):
+        msgs = []
+        cb = MESSAGE_CB(lambda ud, m: msgs.append(m.decode()))
+        load().pyrowave_device_report_performance_stats(self.handle, cb, None, reset)
+        return msgs
+
+    def stage_times(self, reset=False):
+        ms = (C.c_double * 7)()
+        fr = (C.c_uint64 * 7)()
+        _check(load().pyrowave_hip_device_query_stage_times(self.handle, ms, fr, reset), "query_stage_times")
+        return {STAGES[i]: (ms[i], fr[i]) for i in range(7)}
+
+    def set_timing(self, enable):
+        load().pyrowave_hip_device_set_timing(self.handle, enable)
+
+
+def _cpu_buffer(planes, fmt, width, height, strides=None, sizes=None):
+    buf = CpuBuffer()
+    buf.width, buf.height, buf.format = width, height, fmt
+    for i, p in enumerate(planes):
+        if p is None:
+            continue
+        buf.data[i] = p.ctypes.data
+        buf.row_stride_in_bytes[i] = p.strides[0] if strides is None else strides[i]
+        buf.plane_size_in_bytes[i] = p.nbytes if sizes is None else sizes[i]
+    return buf
+
+
+class Encoder:
+    def __init__(self, device, width, height, chroma=CHROMA_420, precision=None, flags=0):
+        self.device, self.width, self.height, self.chroma = device, width, height, chroma
+        h = C.c_void_p()
+        if precision is None:
+            info = EncoderCreateInfo(device.handle if device else None, width, height, chroma)
+            code = load().pyrowave_encoder_create(C.byref(info), C.byref(h))
+        else:
+            info = HipEncoderCreateInfo(device.handle if device else None, width, height, chroma, precision, flags)
+            code = load().pyrowave_hip_encoder_create(C.byref(info), C.byref(h))
+        _check(code, "pyrowave_encoder_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().pyrowave_encoder_destroy(self.handle)
+            self.handle = None
+
+    def encode_cpu(self, planes, budget, fmt=None, width=None, height=None):
+        """planes: [y, cb, cr] (or [y, cbcr] for NV12) uint8 numpy arrays."""
+        if fmt is None:
+            fmt = FORMAT_YUV444P if self.chroma == CHROMA_444 else FORMAT_YUV420P
+        planes = list(planes) + [None] * (3 - len(planes))
+        buf = _cpu_buffer(planes, fmt, width or self.width, height or self.height)
+        rc = RateControl(budget)
+        return load().pyrowave_encoder_encode_cpu_synchronous(self.handle, C.byref(buf), C.byref(rc))
+
+    def encode_cpu_buffer(self, buf, budget):
+        rc = RateControl(budget)
+        return load().pyrowave_encoder_encode_cpu_synchronous(self.handle, C.byref(buf), C.byref(rc))
+
+    def compute_num_packets(self, boundary):
+        n = C.c_size_t()
+        _check(load().pyrowave_encoder_compute_num_packets(self.handle, boundary, C.byref(n)), "compute_num_packets")
+        return n.value
+
+    def packetize(self, boundary, size):
+        n = self.compute_num_packets(boundary)
+        packets = (Packet * max(n, 1))()
+        out = np.zeros(size, dtype=np.uint8)
+        got = C.c_size_t()
+        _check(load().pyrowave_encoder_packetize(self.handle, packets, boundary, C.byref(got), out.ctypes.data, size),
+               "packetize")
+        return out, [(packets[i].offset, packets[i].size) for i in range(got.value)]
+
+    def packet_bytes(self, boundary, size):
+        out, pk = self.packetize(boundary, size)
+        return [out[o:o + s].tobytes() for o, s in pk]
+
+    def result(self):
+        r = HipEncodedFrame()
+        _check(load().pyrowave_hip_encoder_get_result(self.handle, C.byref(r)), "get_result")
+        return r
+
+    def read_coefficients(self):
+        n = coefficient_count(self.width, self.height, self.chroma)
+        out = np.zeros(n, dtype=np.float32)
+        _check(load().pyrowave_hip_encoder_read_coefficients(self.handle, out.ctypes.data, n), "read_coefficients")
+        return out
+
+    def encode_coefficients(self, coeffs, budget):
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.float32)
+        rc = RateControl(budget)
+        _check(load().pyrowave_hip_encoder_encode_coefficients(self.handle, coeffs.ctypes.data, coeffs.size,
+                                                               C.byref(rc)), "encode_coefficients")
+
+    def read_rate_control(self, count32):
+        words = np.zeros((count32, 16), np.uint16)
+        bucket = np.zeros((count32, 16), np.uint8)
+        totals = np.zeros(2048, np.uint32)
+        prefix = np.zeros(2048, np.uint32)
+        consumed = np.zeros(1, np.uint32)
+        quant = np.zeros(count32, np.int32)
+        _check(load().pyrowave_hip_encoder_read_rate_control(
+            self.handle, words.ctypes.data, bucket.ctypes.data, totals.ctypes.data, prefix.ctypes.data,
+            consumed.ctypes.data, quant.ctypes.data), "read_rate_control")
+        return dict(words=words, bucket=bucket, totals=totals, prefix=prefix, consumed=int(consumed[0]), quant=quant)
+
+
+class Decoder:
+    def __init__(self, device, width, height, chroma=CHROMA_420, precision=None, fragment_path=False):
+        self.device, self.width, self.height, self.chroma = device, width, height, chroma
+        h = C.c_void_p()
+        if precision is None:
+            info = DecoderCreateInfo(device.handle if device else None, width, height, chroma, fragment_path)
+            code = load().pyrowave_decoder_create(C.byref(info), C.byref(h))
+        else:
+            info = HipDecoderCreateInfo(device.handle if device else None, width, height, chroma, precision)
+            code = load().pyrowave_hip_decoder_create(C.byref(info), C.byref(h))
+        _check(code, "pyrowave_decoder_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().pyrowave_decoder_destroy(self.handle)
+            self.handle = None
+
+    def clear(self):
+        load().pyrowave_decoder_clear(self.handle)
+
+    def push_packet(self, data):
+        data = bytes(data)
+        return load().pyrowave_decoder_push_packet(self.handle, data, len(data))
+
+    def is_ready(self, allow_partial=False):
+        return bool(load().pyrowave_decoder_decode_is_ready(self.handle, allow_partial))
+
+    def decode_cpu(self, fmt=None):
+        if fmt is None:
+            fmt = FORMAT_YUV444P if self.chroma == CHROMA_444 else FORMAT_YUV420P
+        planes = [np.zeros(plane_shape(self.width, self.height, self.chroma, c), np.uint8) for c in range(3)]
+        buf = _cpu_buffer(planes, fmt, self.width, self.height)
+        _check(load().pyrowave_decoder_decode_cpu_buffer_synchronous(self.handle, C.byref(buf)), "decode_cpu")
+        return planes
+
+    def decode_cpu_buffer(self, buf):
+        return load().pyrowave_decoder_decode_cpu_buffer_synchronous(self.handle, C.byref(buf))
+
+    def read_coefficients(self):
+        n = coefficient_count(self.width, self.height, self.chroma)
+        out = np.zeros(n, dtype=np.float32)
+        _check(load().pyrowave_hip_decoder_read_coefficients(self.handle, out.ctypes.data, n), "read_coefficients")
+        return out
+
+
+def _handles(objs):
+    arr = (C.c_void_p * len(objs))()
+    for i, o in enumerate(objs):
+        arr[i] = o.handle.value if isinstance(o.handle, C.c_void_p) else o.handle
+    return arr
+
+
+def hip_frame(planes_ptrs):
+    """planes_ptrs: [(device_ptr, row_stride, pixel_stride)] * 3."""
+    f = HipFrame()
+    for i, (ptr, stride, step) in enumerate(planes_ptrs):
+        f.planes[i] = HipPlane(ptr, stride, step)
+    return f
+
+
+def hip_target(planes_ptrs):
+    t = HipDecodeTarget()
+    for i, (ptr, stride) in enumerate(planes_ptrs):
+        t.planes[i] = HipOutputPlane(ptr, stride)
+    return t
+
+
+def encode_batch(encoders, frames, budgets, stream=None):
+    n = len(encoders)
+    fa = (HipFrame * n)(*frames)
+    ra = (RateControl * n)(*[RateControl(b) for b in budgets])
+    _check(load().pyrowave_hip_encode_batch(_handles(encoders), fa, ra, n, stream), "encode_batch")
+
+
+def decode_batch(decoders, targets, stream=None):
+    n = len(decoders)
+    ta = (HipDecodeTarget * n)(*targets)
+    _check(load().pyrowave_hip_decode_batch(_handles(decoders), ta, n, stream), "decode_batch")
+
+
+def decode_encoded_batch(decoders, encoders, targets, stream=None):
+    n = len(decoders)
+    ta = (HipDecodeTarget * n)(*targets)
+    _check(load().pyrowave_hip_decode_encoded_batch(_handles(decoders), _handles(encoders), ta, n, stream),
+           "decode_encoded_batch")

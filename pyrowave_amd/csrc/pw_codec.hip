@@ -1,0 +1,972 @@
+// pw_codec.hip -- per-32x32-block kernels of the MI355X PyroWave codec.
+//
+//   k_quant_analyze : quantizer + rate analysis fused; one 128-lane workgroup
+//                     per coded 32x32 block (reference shaders/wavelet_quant.comp
+//                     and shaders/analyze_rate_control.comp).  The per-8x8
+//                     statistics stay in LDS; only 48 B per block leave.
+//   k_rc_reset      : clears one frame's rate-control accumulators.
+//   k_rc_finalize   : savings scan + per-(bucket,subdivision) classification
+//                     (shaders/analyze_rate_control_finalize.comp).
+//   k_rc_partial    : ordered greedy walk of the partially applied entries
+//                     (shaders/resolve_rate_control.comp).
+//   k_rc_prepare    : final truncation per block and the block-order word
+//                     scan that places blocks contiguously.
+//   k_pack          : block bitstream assembled in LDS, written with
+//                     coalesced stores (shaders/block_packing.comp).
+//   k_dequant       : block parse + reconstruction (shaders/wavelet_dequant.comp).
+//
+// Lane layout inside a 32x32 block (all kernels): lane = 8 * k8 + s, where
+// k8 = 4 * y8 + x8 is the 8x8 block in raster order and s the 4x2 subblock;
+// subblock s covers x = 4*(s>>2) .. +3, y = 2*(s&3) .. +1 and its pixel p sits
+// at (p >> 1, p & 1) (reference bitstream/bitstream.md:350-383).
+#include "pw_common.hpp"
+
+#include <hip/hip_fp16.h>
+
+#pragma clang fp contract(off)
+
+namespace pw
+{
+namespace
+{
+constexpr int kBlockThreads = 128;
+
+// Smallest floats >= 2^0.25 and 2^0.75: floor(60.5 + 2*log2(r)) for
+// r = m * 2^E, m in [1,2), is 60 + 2E + [m >= 2^0.25] + [m >= 2^0.75].
+constexpr float kBucketT1 = 0x1.306fe2p+0f;
+constexpr float kBucketT2 = 0x1.ae89fap+0f;
+
+__device__ __forceinline__ int find_msb(uint32_t v)
+{
+	return v ? 31 - __clz(v) : -1;
+}
+
+// float -> fp16 bits, round to nearest even, subnormals kept.
+__device__ __forceinline__ uint32_t f32_to_f16_bits(float f)
+{
+	uint32_t x = __float_as_uint(f);
+	uint32_t sign = (x >> 16) & 0x8000u;
+	uint32_t ax = x & 0x7fffffffu;
+	if (ax >= 0x7f800000u)
+		return sign | (ax > 0x7f800000u ? 0x7e00u : 0x7c00u);
+	if (ax == 0)
+		return sign;
+	int e = (int)(ax >> 23) - 127;
+	uint32_t mant = (ax & 0x7fffffu) | 0x800000u;
+	if (e >= -14)
+	{
+		uint32_t hm = mant >> 13;
+		uint32_t rem = mant & 0x1fffu;
+		if (rem > 0x1000u || (rem == 0x1000u && (hm & 1u)))
+			hm++;
+		int he = e + 15;
+		if (hm & 0x800u)
+		{
+			hm >>= 1;
+			he++;
+		}
+		if (he >= 31)
+			return sign | 0x7c00u;
+		return sign | ((uint32_t)he << 10) | (hm & 0x3ffu);
+	}
+	int shift = -e - 1;
+	if (shift > 24)
+		return sign;
+	uint32_t hm = mant >> shift;
+	uint32_t rem = mant & ((1u << shift) - 1u);
+	uint32_t halfway = 1u << (shift - 1);
+	if (rem > halfway || (rem == halfway && (hm & 1u)))
+		hm++;
+	return sign | hm;
+}
+
+__device__ __forceinline__ float f16_bits_to_f32(uint32_t h)
+{
+	return __half2float(__ushort_as_half((unsigned short)h));
+}
+
+__device__ __forceinline__ float load_band(const BandJob &J, long idx)
+{
+	if (J.kind == kStoreF16)
+		return __half2float(static_cast<const __half *>(J.data)[idx]);
+	return static_cast<const float *>(J.data)[idx];
+}
+
+__device__ __forceinline__ void store_band(const BandJob &J, long idx, float v)
+{
+	if (J.kind == kStoreF16)
+		static_cast<__half *>(J.data)[idx] = __float2half_rn(v);
+	else
+		static_cast<float *>(J.data)[idx] = v;
+}
+
+// Butterfly sums / maxima over the 8 lanes of one 8x8 block.
+__device__ __forceinline__ float cluster_sum_f(float v)
+{
+	v = v + __shfl_xor(v, 1, 8);
+	v = v + __shfl_xor(v, 2, 8);
+	v = v + __shfl_xor(v, 4, 8);
+	return v;
+}
+
+__device__ __forceinline__ int cluster_sum_i(int v)
+{
+	v += __shfl_xor(v, 1, 8);
+	v += __shfl_xor(v, 2, 8);
+	v += __shfl_xor(v, 4, 8);
+	return v;
+}
+
+__device__ __forceinline__ float cluster_max_f(float v)
+{
+	v = fmaxf(v, __shfl_xor(v, 1, 8));
+	v = fmaxf(v, __shfl_xor(v, 2, 8));
+	v = fmaxf(v, __shfl_xor(v, 4, 8));
+	return v;
+}
+
+__device__ __forceinline__ uint32_t cluster_or(uint32_t v)
+{
+	v |= __shfl_xor(v, 1, 8);
+	v |= __shfl_xor(v, 2, 8);
+	v |= __shfl_xor(v, 4, 8);
+	return v;
+}
+
+// Exclusive scan over the 128 lanes of a workgroup (two waves).
+__device__ __forceinline__ int block_excl_scan128(int v, int *s_tmp, int &total)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int incl = v;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1)
+	{
+		int up = __shfl_up(incl, d, 64);
+		if (lane >= d)
+			incl += up;
+	}
+	if (lane == 63)
+		s_tmp[wave] = incl;
+	__syncthreads();
+	int base = wave ? s_tmp[0] : 0;
+	total = s_tmp[0] + s_tmp[1];
+	__syncthreads();
+	return base + incl - v;
+}
+
+// Quantizer state of one lane (one 4x2 subblock).
+struct LaneQuant
+{
+	float t[8];           // scaled texels
+	int n[8];             // magnitudes
+	int b4max;            // subblock max magnitude
+	int msb;              // 8x8 msb, -1 when the 8x8 quantizes to zero
+	uint32_t scale_code;
+	float iq2;            // (1 / (resolution * scale))^2
+};
+
+// reference shaders/wavelet_quant.comp:172-205, 292-316.
+__device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, int k8, int s, LaneQuant &Q)
+{
+	const int x = bx * 32 + 8 * (k8 & 3) + 4 * (s >> 2);
+	const int y = by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
+	float ms = 0.0f;
+#pragma unroll
+	for (int p = 0; p < 8; p++)
+	{
+		int px = x + (p >> 1), py = y + (p & 1);
+		float v = (px < J.bw && py < J.bh) ? load_band(J, (long)py * J.pitch + px) : 0.0f;
+		// Magnitudes beyond 16383.75 would overflow the 14 planes a block can
+		// describe; they are clamped (reachable only far outside the nominal
+		// coefficient range).
+		float t = fminf(fmaxf(v * J.qres, -16383.75f), 16383.75f);
+		Q.t[p] = t;
+		ms = fmaxf(ms, fabsf(t));
+	}
+	float M = cluster_max_f(ms);
+
+	float quant_scale;
+	if (M < 1.0f)
+	{
+		Q.scale_code = 6;
+		quant_scale = 1.0f;
+	}
+	else
+	{
+		int e;
+		frexpf(M - 0.25f, &e);
+		float target_max = (float)(1 << e) - 0.25f;
+		float inv_scale = M / target_max;
+		Q.scale_code = (uint32_t)ceilf((inv_scale - 0.25f) * 8.0f);
+		quant_scale = 1.0f / ((float)Q.scale_code / 8.0f + 0.25f);
+	}
+#pragma unroll
+	for (int p = 0; p < 8; p++)
+		Q.t[p] = Q.t[p] * quant_scale;
+	M = M * quant_scale;
+	ms = ms * quant_scale;
+	float inv_quant = 1.0f / (J.qres * quant_scale);
+	Q.iq2 = inv_quant * inv_quant;
+#pragma unroll
+	for (int p = 0; p < 8; p++)
+		Q.n[p] = abs((int)Q.t[p]);
+	Q.b4max = (int)ms;
+	int maxabs = (int)M;
+	Q.msb = find_msb((uint32_t)maxabs);
+}
+
+__device__ __forceinline__ float dot4_sq(float a, float b, float c, float d)
+{
+	return ((a * a + b * b) + c * c) + d * d;
+}
+
+// Code word of an 8x8 block at quant 0 (reference shaders/wavelet_quant.comp:219-229).
+__device__ __forceinline__ uint32_t code_word_q0(const LaneQuant &Q, int s)
+{
+	if (Q.msb < 0)
+		return 0;
+	int qp0 = Q.msb >= 3 ? Q.msb - 2 : 0;
+	uint32_t plane_code = (uint32_t)(find_msb((uint32_t)(Q.b4max >> qp0)) + 1);
+	uint32_t cw = cluster_or(plane_code << (2 * s));
+	return cw | ((uint32_t)qp0 << 16) | ((Q.scale_code & 0xfu) << 20);
+}
+
+__device__ __forceinline__ uint32_t bucket_of_ratio(float r)
+{
+	if (!(r > 0.0f))
+		return 0;
+	int e;
+	float f = frexpf(r, &e);
+	float m = 2.0f * f;
+	int k = 60 + 2 * (e - 1) + (m >= kBucketT1 ? 1 : 0) + (m >= kBucketT2 ? 1 : 0);
+	return k < 0 ? 0u : (uint32_t)k;
+}
+
+struct BlockCoord
+{
+	BandJob J;
+	int b;   // frame-local block index
+	int bx, by;
+};
+
+__device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const uint32_t *blk_band,
+                                                   const int32_t *frame_block_base)
+{
+	BlockCoord C;
+	C.J = bands[blk_band[blockIdx.x]];
+	C.b = (int)blockIdx.x - frame_block_base[C.J.frame];
+	int r = C.b - C.J.off32;
+	C.by = r / C.J.bx32;
+	C.bx = r - C.by * C.J.bx32;
+	return C;
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
+                                                                 const uint32_t *__restrict__ blk_band,
+                                                                 const FrameState *__restrict__ frames,
+                                                                 const int32_t *__restrict__ frame_block_base)
+{
+	__shared__ int s_cost[16][16];
+	__shared__ float s_dist[16][16];
+	__shared__ uint32_t s_words[16];
+	__shared__ float s_D[16];
+	__shared__ uint32_t s_v[16];
+
+	const BlockCoord C = locate_block(bands, blk_band, frame_block_base);
+	const BandJob &J = C.J;
+	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
+	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
+	const bool in_range = x8 < J.bx8 && y8 < J.by8;
+
+	LaneQuant Q;
+	quantize_lane(J, C.bx, C.by, k8, s, Q);
+	const int msb = Q.msb;
+
+	// Distortion of discarding the whole 8x8 block; the reference leaves out
+	// rdo_distortion_scale here (reference shaders/wavelet_quant.comp:254).
+	float dd = cluster_sum_f((dot4_sq(Q.t[0], Q.t[1], Q.t[2], Q.t[3]) + dot4_sq(Q.t[4], Q.t[5], Q.t[6], Q.t[7])) * Q.iq2);
+	float dist_discard = msb >= 0 ? f16_bits_to_f32(f32_to_f16_bits(fminf(dd, 60000.0f))) : 0.0f;
+
+	for (int q = 0; q < 16; q++)
+	{
+		// compute_quant_stats (reference shaders/wavelet_quant.comp:113-165).
+		float se_lane = 0.0f;
+		int late_lane = 0, nsig_lane = 0;
+		if (q <= msb)
+		{
+			float err[8];
+#pragma unroll
+			for (int p = 0; p < 8; p++)
+			{
+				int iv = Q.n[p] >> q;
+				float a = fabsf(Q.t[p]);
+				float recon = 0.0f;
+				if (iv != 0)
+				{
+					nsig_lane++;
+					recon = q == 0 ? (float)iv : ((float)iv + 0.5f) * (float)(1 << q);
+				}
+				err[p] = a - recon;
+			}
+			se_lane = ((dot4_sq(err[0], err[1], err[2], err[3]) + dot4_sq(err[4], err[5], err[6], err[7])) *
+			           J.rdo_scale) * Q.iq2;
+			int b = Q.b4max >> q;
+			int early = b > 0 ? 1 : 0;
+			int m = msb - q;
+			if (m >= 3)
+			{
+				int qp = m - 2;
+				early = qp + 1;
+				b >>= qp;
+			}
+			early += find_msb((uint32_t)b) + 1;
+			late_lane = early - 1 > 0 ? early - 1 : 0;
+		}
+		float se = cluster_sum_f(se_lane);
+		int late = cluster_sum_i(late_lane);
+		int nsig = cluster_sum_i(nsig_lane);
+		if (s == 0)
+		{
+			int cost = 0;
+			float dist = 0.0f;
+			if (in_range)
+			{
+				if (q <= msb)
+				{
+					cost = 8 * late + nsig;
+					dist = q == 0 ? 0.0f : f16_bits_to_f32(f32_to_f16_bits(fminf(se, 60000.0f)));
+				}
+				else
+					dist = dist_discard;
+			}
+			s_cost[q][k8] = cost;
+			s_dist[q][k8] = dist;
+		}
+	}
+	__syncthreads();
+
+	const FrameState &F = frames[J.frame];
+	if (tid < 16)
+	{
+		// reference shaders/analyze_rate_control.comp:169-215.
+		const int q = tid;
+		uint32_t total = 0;
+#pragma unroll
+		for (int k = 0; k < 16; k++)
+			total += s_cost[q][k] ? (uint32_t)s_cost[q][k] + 24u : 0u;
+		if (total)
+			total += 64;
+		const float *d = s_dist[q];
+		float D = (((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7]))) +
+		          (((d[8] + d[9]) + (d[10] + d[11])) + ((d[12] + d[13]) + (d[14] + d[15])));
+		s_words[q] = (total + 31) >> 5;
+		s_D[q] = D;
+	}
+	__syncthreads();
+	if (tid < 16)
+	{
+		// emit_rdo_operations (reference shaders/analyze_rate_control.comp:101-147).
+		const int q = tid;
+		uint32_t bi = 0;
+		if (q != 0)
+		{
+			float cost_f = (float)s_words[q], base_f = (float)s_words[0];
+			if (cost_f != base_f)
+			{
+				float dd2 = s_D[q] - s_D[0];
+				dd2 = dd2 > 0.0f ? dd2 : 0.0f;
+				bi = bucket_of_ratio(dd2 / (base_f - cost_f));
+			}
+		}
+		s_v[q] = bi < 112u + (uint32_t)q ? bi : 112u + (uint32_t)q;
+	}
+	__syncthreads();
+	if (tid < 16)
+	{
+		const int q = tid;
+		uint32_t Bq = s_v[q];
+		for (int k = 0; k < q; k++)
+		{
+			uint32_t cand = s_v[k] + (uint32_t)(q - k);
+			Bq = cand > Bq ? cand : Bq;
+		}
+		F.words[(long)C.b * 16 + q] = (uint16_t)s_words[q];
+		F.bucket[(long)C.b * 16 + q] = (uint8_t)Bq;
+		if (q == 0)
+			atomicAdd(&F.rc[kRcConsumed], s_words[0]);
+		else
+		{
+			uint32_t saving = s_words[q - 1] - s_words[q];
+			if (saving)
+				atomicAdd(&F.totals[Bq * kSubdiv + ((uint32_t)C.b >> F.shamt)], saving);
+		}
+	}
+}
+
+__global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, int first, int count)
+{
+	const int i = threadIdx.x;
+	if (i >= count)
+		return;
+	FrameState &F = frames[first + i];
+	const CallParams &P = block.p[i];
+	for (int c = 0; c < 3; c++)
+	{
+		F.src[c] = P.src[c];
+		F.src_pitch[c] = P.src_pitch[c];
+		F.src_step[c] = P.src_step[c];
+		F.dst[c] = P.dst[c];
+		F.dst_pitch[c] = P.dst_pitch[c];
+	}
+	F.sequence = P.sequence;
+	F.target_words = P.target_words;
+	F.flags = P.flags;
+	F.dec_offsets = P.dec_offsets;
+	F.dec_payload = P.dec_payload;
+	F.dec_offset_stride = P.dec_offset_stride;
+	F.dec_payload_bytes = P.dec_payload_bytes;
+}
+
+__global__ void k_rc_reset(FrameState *__restrict__ frames)
+{
+	const FrameState &F = frames[blockIdx.y];
+	const int n = F.n32 > kFlat ? F.n32 : kFlat;
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+	{
+		if (i < kFlat)
+			F.totals[i] = 0;
+		if (i < F.n32)
+			F.partial_quant[i] = 0;
+		if (i < kRcCount)
+			F.rc[i] = 0;
+	}
+}
+
+// reference shaders/analyze_rate_control_finalize.comp:14-38 (512 threads x 4
+// entries) and the skip / required logic of shaders/resolve_rate_control.comp:38-56.
+__global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ frames)
+{
+	__shared__ uint32_t s_scan[512];
+	const FrameState &F = frames[blockIdx.x];
+	const int t = threadIdx.x;
+	uint32_t v[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+		v[i] = F.totals[4 * t + i];
+	v[1] += v[0];
+	v[2] += v[1];
+	v[3] += v[2];
+	s_scan[t] = v[3];
+	__syncthreads();
+	// The reference stops at step < 256; the exact variant runs one more step.
+	const uint32_t limit = (F.flags & 1u) ? 512u : 256u;
+	for (uint32_t step = 1; step < limit; step *= 2)
+	{
+		__syncthreads();
+		uint32_t up = t >= (int)step ? s_scan[t - step] : 0u;
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			v[i] += up;
+		s_scan[t] = v[3];
+	}
+	__syncthreads();
+	const int required_base = (int)F.rc[kRcConsumed] - F.target_words;
+	uint32_t prev = t ? s_scan[t - 1] : 0u;
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+	{
+		const int x = 4 * t + i;
+		F.prefix[x] = v[i];
+		bool empty;
+		int required;
+		if (x == 0)
+		{
+			empty = v[0] == 0;
+			required = required_base;
+		}
+		else
+		{
+			empty = v[i] == prev;
+			required = required_base - (int)prev;
+		}
+		uint8_t cls = kRcSkip;
+		if (!empty && required > 0)
+			cls = (int)F.totals[x] <= required ? kRcFull : kRcPartial;
+		F.cls[x] = cls;
+		F.required[x] = required;
+		prev = v[i];
+	}
+}
+
+__device__ __forceinline__ int block_incl_scan256(int v, int *s_tmp, int &total)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	int incl = v;
+#pragma unroll
+	for (int d = 1; d < 64; d <<= 1)
+	{
+		int up = __shfl_up(incl, d, 64);
+		if (lane >= d)
+			incl += up;
+	}
+	if (lane == 63)
+		s_tmp[wave] = incl;
+	__syncthreads();
+	int base = 0;
+	for (int w = 0; w < wave; w++)
+		base += s_tmp[w];
+	total = s_tmp[0] + s_tmp[1] + s_tmp[2] + s_tmp[3];
+	__syncthreads();
+	return base + incl;
+}
+
+// One workgroup per (bucket, subdivision) entry; only partially applied
+// entries walk their operations (reference shaders/resolve_rate_control.comp:57-79).
+__global__ __launch_bounds__(256) void k_rc_partial(FrameState *__restrict__ frames)
+{
+	__shared__ int s_tmp[4];
+	const FrameState &F = frames[blockIdx.y];
+	const int x = blockIdx.x;
+	if (F.cls[x] != kRcPartial)
+		return;
+	const int required = F.required[x];
+	const uint32_t beta = (uint32_t)x >> 4;
+	const int sd = x & 15;
+	const int base = sd * F.per_sub;
+	int end = base + F.per_sub;
+	end = end < F.n32 ? end : F.n32;
+	int total_saved = 0;
+	for (int chunk = base; chunk < end && total_saved < required; chunk += 256)
+	{
+		const int b = chunk + (int)threadIdx.x;
+		int saving = 0, q = 0;
+		if (b < end)
+		{
+			for (int qq = 1; qq < 16; qq++)
+			{
+				if (F.bucket[(long)b * 16 + qq] == beta)
+				{
+					saving = (int)F.words[(long)b * 16 + qq - 1] - (int)F.words[(long)b * 16 + qq];
+					q = qq;
+					break;
+				}
+			}
+		}
+		int sum;
+		int incl = block_incl_scan256(saving, s_tmp, sum);
+		if (saving != 0 && total_saved + incl - saving < required)
+			atomicMax(&F.partial_quant[b], q);
+		total_saved += sum;
+	}
+}
+
+// Final truncation per block and word offsets within 256-block groups.
+__global__ __launch_bounds__(256) void k_rc_prepare(FrameState *__restrict__ frames, const uint32_t *__restrict__ grp_frame,
+                                                    const int32_t *__restrict__ frame_group_base)
+{
+	__shared__ int s_tmp[4];
+	const int f = (int)grp_frame[blockIdx.x];
+	const FrameState &F = frames[f];
+	const int g = (int)blockIdx.x - frame_group_base[f];
+	const int b = g * 256 + (int)threadIdx.x;
+	int wf = 0;
+	if (b < F.n32)
+	{
+		int qf = F.partial_quant[b];
+		const uint32_t sd = (uint32_t)b >> F.shamt;
+		for (int q = 1; q < 16; q++)
+		{
+			if (F.words[(long)b * 16 + q - 1] != F.words[(long)b * 16 + q])
+			{
+				uint32_t x = (uint32_t)F.bucket[(long)b * 16 + q] * kSubdiv + sd;
+				if (F.cls[x] == kRcFull && q > qf)
+					qf = q;
+			}
+		}
+		F.quant[b] = (uint8_t)qf;
+		wf = F.words[(long)b * 16 + qf];
+	}
+	int total;
+	int incl = block_incl_scan256(wf, s_tmp, total);
+	if (b < F.n32)
+		F.excl[b] = (uint32_t)(incl - wf);
+	if (threadIdx.x == 0)
+		F.group_sum[g] = (uint32_t)total;
+}
+
+// reference shaders/block_packing.comp:107-142.
+__device__ __forceinline__ uint32_t quantize_code_word(uint32_t cw, int quant)
+{
+	if (quant != 0 && cw != 0)
+	{
+		int q_bits = (int)((cw >> 16) & 0xf);
+		int sub = q_bits < quant ? q_bits : quant;
+		q_bits -= sub;
+		quant -= sub;
+		if (quant != 0)
+		{
+			quant = quant < 3 ? quant : 3;
+			uint32_t plane0 = cw & 0x5555u;
+			uint32_t plane1 = (cw & 0xaaaau) >> 1;
+			uint32_t plane2 = plane0 & plane1;
+			do
+			{
+				plane0 = plane1;
+				plane1 = plane2;
+				plane2 = 0;
+				quant--;
+			} while (quant != 0);
+			plane0 &= ~plane1;
+			cw = (cw & ~0xffffu) | ((plane0 | (plane1 << 1)) & 0xffffu);
+		}
+		cw = (cw & ~(0xfu << 16)) | ((uint32_t)q_bits << 16);
+	}
+	return cw;
+}
+
+__device__ __forceinline__ void lds_or_byte(uint32_t *buf, uint32_t byte_pos, uint32_t value)
+{
+	atomicOr(&buf[byte_pos >> 2], (value & 0xffu) << (8 * (byte_pos & 3)));
+}
+
+__global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restrict__ bands,
+                                                        const uint32_t *__restrict__ blk_band,
+                                                        FrameState *__restrict__ frames,
+                                                        const int32_t *__restrict__ frame_block_base)
+{
+	__shared__ uint32_t s_buf[kMaxBlockWords];
+	__shared__ int s_tmp[2];
+	__shared__ uint32_t s_ballot;
+	__shared__ uint32_t s_bits;
+	__shared__ uint32_t s_group_base;
+
+	const BlockCoord C = locate_block(bands, blk_band, frame_block_base);
+	const BandJob &J = C.J;
+	FrameState &F = frames[J.frame];
+	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
+	const int quant = F.quant[C.b];
+	const uint32_t words_expected = F.words[(long)C.b * 16 + quant];
+
+	if (words_expected == 0)
+	{
+		if (tid == 0)
+		{
+			F.meta[2 * C.b] = 0;
+			F.meta[2 * C.b + 1] = 0;
+		}
+		return;
+	}
+
+	for (int i = tid; i < kMaxBlockWords; i += kBlockThreads)
+		s_buf[i] = 0;
+	if (tid == 0)
+	{
+		s_ballot = 0;
+		s_bits = 0;
+	}
+	if (tid < 64)
+	{
+		// Start of this block's 256-block group: sum of the earlier groups.
+		const int groups = C.b >> 8;
+		uint32_t acc = 0;
+		for (int i = tid; i < groups; i += 64)
+			acc += F.group_sum[i];
+#pragma unroll
+		for (int d = 32; d >= 1; d >>= 1)
+			acc += __shfl_xor(acc, d, 64);
+		if (tid == 0)
+			s_group_base = acc;
+	}
+
+	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
+	const bool in_range = x8 < J.bx8 && y8 < J.by8;
+	LaneQuant Q;
+	quantize_lane(J, C.bx, C.by, k8, s, Q);
+	uint32_t cw0 = in_range ? code_word_q0(Q, s) : 0u;
+	uint32_t cw = quantize_code_word(cw0, quant);
+	const bool active = (cw & 0xffffu) != 0;
+
+	const uint32_t in_q = (cw0 >> 16) & 0xf, out_q = (cw >> 16) & 0xf;
+	const uint32_t in_planes = ((cw0 >> (2 * s)) & 3u) + in_q;
+	const uint32_t out_planes = active ? ((cw >> (2 * s)) & 3u) + out_q : 0u;
+	uint32_t sig = 0;
+	uint32_t plane_bytes_lane[14];
+#pragma unroll
+	for (int pi = 0; pi < 14; pi++)
+	{
+		uint32_t byte = 0;
+		if ((uint32_t)pi < out_planes)
+		{
+			int bit = (int)in_planes - 1 - pi;
+#pragma unroll
+			for (int p = 0; p < 8; p++)
+				byte |= (uint32_t)((Q.n[p] >> bit) & 1) << p;
+		}
+		plane_bytes_lane[pi] = byte;
+		sig |= byte;
+	}
+	const int nsig = __popc(sig);
+	const int cost8 = cluster_sum_i(8 * (int)out_planes + nsig);
+	__syncthreads();
+
+	if (s == 0)
+	{
+		if (active)
+			atomicOr(&s_ballot, 1u << k8);
+		if (cost8)
+			atomicAdd(&s_bits, (uint32_t)cost8 + 24u);
+	}
+	__syncthreads();
+	const uint32_t ballot = s_ballot;
+	const uint32_t N = __popc(ballot);
+	uint32_t wcount = (s_bits + 31) / 32;
+	if (wcount)
+		wcount += 2;
+
+	int planes_total;
+	const int plane_excl = block_excl_scan128((int)out_planes, s_tmp, planes_total);
+	int sig_total;
+	const int sig_excl = block_excl_scan128(nsig, s_tmp, sig_total);
+
+	const uint32_t plane_base = 8 + 3 * N;
+	if (active && s == 0)
+	{
+		uint32_t j = __popc(ballot & ((1u << k8) - 1u));
+		lds_or_byte(s_buf, 8 + 2 * j, cw & 0xffu);
+		lds_or_byte(s_buf, 9 + 2 * j, (cw >> 8) & 0xffu);
+		lds_or_byte(s_buf, 8 + 2 * N + j, (cw >> 16) & 0xffu);
+	}
+	for (uint32_t pi = 0; pi < out_planes; pi++)
+		lds_or_byte(s_buf, plane_base + (uint32_t)plane_excl + pi, plane_bytes_lane[pi]);
+	{
+		uint32_t sbit = 8u * (plane_base + (uint32_t)planes_total) + (uint32_t)sig_excl;
+#pragma unroll
+		for (int p = 0; p < 8; p++)
+		{
+			if ((sig >> p) & 1u)
+			{
+				if (Q.t[p] < 0.0f)
+					atomicOr(&s_buf[sbit >> 5], 1u << (sbit & 31));
+				sbit++;
+			}
+		}
+	}
+	if (tid == 0)
+	{
+		uint32_t w0 = ballot | (wcount << 16) | ((F.sequence & 7u) << 28);
+		int e = (int)((J.qcode >> 3) & 31) - quant;
+		e = e > 0 ? e : 0;
+		uint32_t qc = (J.qcode & ~(31u << 3)) | ((uint32_t)e << 3);
+		uint32_t w1 = qc | ((uint32_t)C.b << 8);
+		atomicOr(&s_buf[0], w0);
+		atomicOr(&s_buf[1], w1);
+	}
+	__syncthreads();
+
+	const uint32_t offset = s_group_base + F.excl[C.b];
+	const bool fits = offset + wcount <= F.capacity_words && wcount <= (uint32_t)kMaxBlockWords;
+	if (fits)
+		for (uint32_t i = tid; i < wcount; i += kBlockThreads)
+			F.bitstream[offset + i] = s_buf[i];
+	if (tid == 0)
+	{
+		F.meta[2 * C.b] = offset;
+		F.meta[2 * C.b + 1] = wcount;
+		if (wcount != words_expected)
+			atomicOr(&F.rc[kRcError], kErrWordMismatch);
+		if (!fits)
+			atomicOr(&F.rc[kRcError], kErrCapacity);
+	}
+}
+
+__device__ __forceinline__ uint32_t payload_byte(const uint32_t *payload, uint32_t payload_bytes, uint32_t pos)
+{
+	if (pos >= payload_bytes)
+		return 0;
+	return (payload[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
+}
+
+// reference shaders/wavelet_dequant.comp:100-160 and 220-391.
+__global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
+                                                           const uint32_t *__restrict__ blk_band,
+                                                           const FrameState *__restrict__ frames,
+                                                           const int32_t *__restrict__ frame_block_base)
+{
+	__shared__ uint32_t s_poff[16];
+	__shared__ uint32_t s_ctrl[16];
+	__shared__ uint32_t s_qb[16];
+	__shared__ uint32_t s_sign_base;
+	__shared__ int s_tmp[2];
+
+	const BlockCoord C = locate_block(bands, blk_band, frame_block_base);
+	const BandJob &J = C.J;
+	const FrameState &F = frames[J.frame];
+	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
+	const int x = C.bx * 32 + 8 * (k8 & 3) + 4 * (s >> 2);
+	const int y = C.by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
+
+	uint32_t off = F.dec_offsets[(long)C.b * F.dec_offset_stride];
+	bool missing = off == 0xffffffffu;
+	if (F.dec_offset_stride == 2 && F.dec_offsets[2L * C.b + 1] == 0)
+		missing = true;
+
+	float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	if (!missing)
+	{
+		const uint32_t *payload = F.dec_payload;
+		const uint32_t pbytes = F.dec_payload_bytes;
+		const uint32_t base = off * 4u;
+		const uint32_t ballot = payload_byte(payload, pbytes, base) | (payload_byte(payload, pbytes, base + 1) << 8);
+		const uint32_t q_code = payload_byte(payload, pbytes, base + 4);
+		const uint32_t N = __popc(ballot);
+		if (tid < 16)
+		{
+			uint32_t ctrl = 0, qb = 0;
+			if ((ballot >> tid) & 1u)
+			{
+				uint32_t j = __popc(ballot & ((1u << tid) - 1u));
+				ctrl = payload_byte(payload, pbytes, base + 8 + 2 * j) |
+				       (payload_byte(payload, pbytes, base + 9 + 2 * j) << 8);
+				qb = payload_byte(payload, pbytes, base + 8 + 2 * N + j);
+			}
+			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
+			msbs |= msbs >> 1;
+			uint32_t cost = (uint32_t)__popc(lsbs) + (uint32_t)__popc(msbs) + (qb & 0xfu) * 8;
+			uint32_t incl = cost;
+#pragma unroll
+			for (int d = 1; d < 16; d <<= 1)
+			{
+				uint32_t up = __shfl_up(incl, d, 16);
+				if (tid >= d)
+					incl += up;
+			}
+			s_poff[tid] = base + 8 + 3 * N + incl - cost;
+			s_ctrl[tid] = ctrl;
+			s_qb[tid] = qb;
+			if (tid == 15)
+				s_sign_base = 8u * (base + 8 + 3 * N + incl);
+		}
+		__syncthreads();
+
+		int sig_count = 0;
+		if ((ballot >> k8) & 1u)
+		{
+			const uint32_t ctrl = s_ctrl[k8], qb = s_qb[k8];
+			const uint32_t q_bits = qb & 0xfu;
+			const int bo = 2 * s;
+			const uint32_t lmask = (1u << bo) - 1u;
+			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
+			msbs |= msbs >> 1;
+			uint32_t byte_off = s_poff[k8] + (uint32_t)__popc(lsbs & lmask) + (uint32_t)__popc(msbs & lmask) + q_bits * s;
+			const int planes = (int)(q_bits + ((ctrl >> bo) & 3u));
+			int mag[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+			for (int pl = planes - 1; pl >= 0; pl--)
+			{
+				uint32_t byte = payload_byte(payload, pbytes, byte_off++);
+#pragma unroll
+				for (int p = 0; p < 8; p++)
+					mag[p] |= (int)((byte >> p) & 1u) << pl;
+			}
+			// reference shaders/wavelet_dequant.comp:168-175, 312-315.
+			int ef = (int)(q_code >> 3);
+			ef = ef > 24 ? 24 : ef;
+			const int e = 4 - ef;
+			const int m = (int)(q_code & 7u);
+			const float qstep = (1.0f / (8.0f * 1024.0f * 1024.0f)) * (float)((8 + m) * (1 << (20 + e)));
+			const float inv_scale = qstep * ((float)(qb >> 4) / 8.0f + 0.25f);
+#pragma unroll
+			for (int p = 0; p < 8; p++)
+			{
+				float f = (float)mag[p];
+				if (f != 0.0f)
+				{
+					f += 0.5f;
+					sig_count++;
+				}
+				v[p] = f * inv_scale;
+			}
+		}
+		int sig_total;
+		const int sig_excl = block_excl_scan128(sig_count, s_tmp, sig_total);
+		uint32_t sbit = s_sign_base + (uint32_t)sig_excl;
+#pragma unroll
+		for (int p = 0; p < 8; p++)
+		{
+			if (v[p] != 0.0f)
+			{
+				uint32_t byte = payload_byte(payload, pbytes, sbit >> 3);
+				if ((byte >> (sbit & 7)) & 1u)
+					v[p] = -v[p];
+				sbit++;
+			}
+		}
+	}
+#pragma unroll
+	for (int p = 0; p < 8; p++)
+		store_band(J, (long)(y + (p & 1)) * J.pitch + x + (p >> 1), v[p]);
+}
+} // namespace
+
+void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream)
+{
+	for (int first = 0; first < nframes; first += kParamsPerLaunch)
+	{
+		ParamBlock block;
+		const int count = nframes - first < kParamsPerLaunch ? nframes - first : kParamsPerLaunch;
+		for (int i = 0; i < count; i++)
+			block.p[i] = params[first + i];
+		hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, stream, frames, block, first, count);
+	}
+}
+
+void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream)
+{
+	int n = max_n32 > kFlat ? max_n32 : kFlat;
+	int blocks = (n + 255) / 256;
+	blocks = blocks > 64 ? 64 : blocks;
+	hipLaunchKernelGGL(k_rc_reset, dim3(blocks, nframes), dim3(256), 0, stream, frames);
+}
+
+void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
+                          const int32_t *frame_block_base, hipStream_t stream)
+{
+	if (nblocks > 0)
+		hipLaunchKernelGGL(k_quant_analyze, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
+		                   frame_block_base);
+}
+
+void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_rc_finalize, dim3(nframes), dim3(512), 0, stream, frames);
+}
+
+void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_rc_partial, dim3(kFlat, nframes), dim3(256), 0, stream, frames);
+}
+
+void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroups_total,
+                       const int32_t *frame_group_base, hipStream_t stream)
+{
+	if (ngroups_total > 0)
+		hipLaunchKernelGGL(k_rc_prepare, dim3(ngroups_total), dim3(256), 0, stream, frames, grp_frame,
+		                   frame_group_base);
+}
+
+void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
+                 const int32_t *frame_block_base, hipStream_t stream)
+{
+	if (nblocks > 0)
+		hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
+		                   frame_block_base);
+}
+
+void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
+                    const int32_t *frame_block_base, hipStream_t stream)
+{
+	if (nblocks > 0)
+		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
+		                   frame_block_base);
+}
+} // namespace pw

@@ -1,0 +1,189 @@
+// pw_common.hpp -- definitions shared by the host code and the HIP kernels.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace pw
+{
+// Geometry (reference pyrowave_common.hpp:112-117, bitstream/bitstream.md:103-109).
+constexpr int kLevels = 5;
+constexpr int kComponents = 3;
+constexpr int kBands = 4;
+constexpr int kAlignment = 1 << kLevels;
+constexpr int kMinimumImageSize = 4 << kLevels;
+
+// Rate control (reference pyrowave_encoder.cpp:17-19).
+constexpr int kSubdiv = 16;
+constexpr int kBuckets = 128;
+constexpr int kFlat = kBuckets * kSubdiv;
+
+// Forward / inverse DWT tile: 64x64 samples of the level domain, 4-sample
+// apron, so 72x72 staged in LDS.  One 256-thread workgroup per tile.
+constexpr int kTile = 64;
+constexpr int kApron = 4;
+constexpr int kTileIn = kTile + 2 * kApron;
+constexpr int kDwtThreads = 256;
+
+// Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
+// sign bits, 8 B header and 3 B per 8x8 block, rounded up to words.
+constexpr int kMaxBlockWords = 512;
+
+enum StorageKind : int32_t
+{
+	kStoreF32 = 0,
+	kStoreF16 = 1,
+	kStoreU8 = 2
+};
+
+// Per-(bucket, subdivision) resolve class.
+enum RcClass : uint8_t
+{
+	kRcSkip = 0,
+	kRcFull = 1,
+	kRcPartial = 2
+};
+
+// Error bits reported by the device in FrameState::rc[kRcError].
+enum DeviceError : uint32_t
+{
+	kErrWordMismatch = 1u << 0,   // packed words != analyzed words
+	kErrCapacity = 1u << 1        // bitstream buffer too small
+};
+
+// Indices into FrameState::rc.
+enum RcWord
+{
+	kRcConsumed = 0,
+	kRcError = 1,
+	kRcCount = 4
+};
+
+// One frame slot of a batch: every device array the encode or decode of
+// that frame touches.  Lives in device memory; kernels index it by slot.
+struct FrameState
+{
+	// Encoder rate-control arrays.
+	uint16_t *words;        // [n32][16] words at each quant level
+	uint8_t *bucket;        // [n32][16] inclusive R-D bucket
+	int32_t *partial_quant; // [n32] truncation chosen by partially applied buckets
+	uint8_t *quant;         // [n32] final truncation
+	uint32_t *excl;         // [n32] exclusive word prefix within a 256-block group
+	uint32_t *group_sum;    // [ngroups] words per 256-block group
+	uint32_t *meta;         // [n32][2] offset (words), words
+	uint32_t *bitstream;    // coded blocks in block-index order
+	uint32_t *totals;       // [2048] savings per (bucket, subdivision)
+	uint32_t *prefix;       // [2048] finalized scan
+	uint8_t *cls;           // [2048] RcClass
+	int32_t *required;      // [2048] savings still required at each entry
+	uint32_t *rc;           // [kRcCount] RcWord
+	// Per-call encoder inputs: u8 planes (device pointers), sequence counter
+	// and word budget for the coded blocks.
+	const uint8_t *src[3];
+	int32_t src_pitch[3];
+	int32_t src_step[3];
+	uint32_t sequence;
+	int32_t target_words;
+	// Per-call decoder outputs: u8 planes (device pointers).
+	uint8_t *dst[3];
+	int32_t dst_pitch[3];
+	// Decoder input: offsets (u32 per block, or meta pairs) and payload.
+	const uint32_t *dec_offsets;
+	const uint32_t *dec_payload;
+	int32_t dec_offset_stride;   // 1: offset table (UINT32_MAX = missing); 2: meta pairs
+	uint32_t dec_payload_bytes;
+	int32_t n32;
+	int32_t per_sub;
+	int32_t shamt;
+	int32_t ngroups;
+	uint32_t capacity_words;
+	int32_t width, height;
+	uint32_t flags;   // bit 0: exact prefix in rate-control finalize
+};
+
+// Per-call fields of one frame, passed to k_set_params as kernel arguments
+// (no host-to-device copy on the launch stream).
+struct CallParams
+{
+	const uint8_t *src[3];
+	uint8_t *dst[3];
+	const uint32_t *dec_offsets;
+	const uint32_t *dec_payload;
+	int32_t src_pitch[3];
+	int32_t src_step[3];
+	int32_t dst_pitch[3];
+	uint32_t sequence;
+	int32_t target_words;
+	uint32_t flags;
+	int32_t dec_offset_stride;
+	uint32_t dec_payload_bytes;
+};
+
+constexpr int kParamsPerLaunch = 24;
+
+struct ParamBlock
+{
+	CallParams p[kParamsPerLaunch];
+};
+
+// One component of one frame at one DWT level (forward or inverse).
+struct DwtJob
+{
+	const void *src;    // forward: LL of the previous level (null: u8 plane from FrameState)
+	void *dst;          // inverse: LL of the next level up, or u8 plane (null: from the decode target)
+	void *band[4];      // forward: outputs, inverse: inputs
+	int32_t src_kind;   // forward input storage (kStoreU8 for planes)
+	int32_t dst_kind;   // inverse output storage
+	int32_t band_kind;  // storage of the high bands (HL, LH, HH)
+	int32_t ll_kind;    // storage of the LL band
+	int32_t src_pitch;  // elements
+	int32_t dst_pitch;  // elements (bytes for u8)
+	int32_t band_pitch; // elements
+	int32_t R_x, R_y;   // forward: valid input size; inverse: valid output size
+	int32_t A_x, A_y;   // forward: transform domain; inverse: 2 * band size
+	int32_t bw, bh;     // band size
+	int32_t tiles_x;
+	int32_t tile_base;
+	int32_t frame;      // frame slot
+	int32_t component;
+	int32_t fp16_math;  // packed FP16 lifting
+};
+
+// One transmitted band of one frame, for the per-32x32-block kernels.
+struct BandJob
+{
+	void *data;         // padded band storage: pitch >= bx32 * 32, rows >= by32 * 32
+	int32_t kind;       // kStoreF32 / kStoreF16
+	int32_t pitch;      // elements
+	int32_t bw, bh;
+	int32_t bx8, by8;
+	int32_t bx32, by32;
+	int32_t off32;      // first frame-local 32x32 block index
+	int32_t frame;      // frame slot
+	float qres;         // 1 / decode_quant(encode_quant(1 / resolution))
+	float rdo_scale;
+	uint32_t qcode;     // encode_quant(1 / resolution)
+	int32_t pad;
+};
+
+// Launch wrappers (defined in the .hip files).
+void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                        bool fp16, hipStream_t stream);
+void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                        bool fp16, hipStream_t stream);
+void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream);
+void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
+                          const int32_t *frame_block_base, hipStream_t stream);
+void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream);
+void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream);
+void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroups_total,
+                       const int32_t *frame_group_base, hipStream_t stream);
+void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
+                 const int32_t *frame_block_base, hipStream_t stream);
+void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
+                    const int32_t *frame_block_base, hipStream_t stream);
+void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream);
+void upload_constants();
+} // namespace pw

@@ -1,0 +1,349 @@
+// pw_dwt.hip -- forward and inverse CDF 9/7 DWT levels on gfx950.
+//
+// One 256-thread workgroup transforms a 64x64 tile of a level's domain.  The
+// tile plus a 4-sample apron (72x72) is staged in LDS once; the first 1-D pass
+// runs over LDS rows (forward: horizontal; inverse: vertical columns), writes a
+// second LDS buffer, and the second pass writes coalesced rows of the output
+// bands straight to HBM.  Every lifting step is fmaf(c, l + r, v), the order
+// of the reference shaders (reference shaders/dwt.comp:108-115,
+// shaders/idwt.comp:95-108), so a tile reproduces the whole-line transform
+// bit for bit.
+#include "pw_common.hpp"
+
+#include <hip/hip_fp16.h>
+
+#pragma clang fp contract(off)
+
+namespace pw
+{
+namespace
+{
+// reference shaders/dwt_common.h:37-42
+constexpr float kAlpha = float(-1.586134342059924);
+constexpr float kBeta = float(-0.052980118572961);
+constexpr float kGamma = float(0.882911075530934);
+constexpr float kDelta = float(0.443506852043971);
+constexpr float kK = float(1.230174104914001);
+constexpr float kInvK = float(1.0 / 1.230174104914001);
+
+constexpr int kPitchA = kTileIn + 1; // odd pitch: row-per-lane reads are conflict-free
+constexpr int kPitchB = kTileIn + 1;
+
+__constant__ float c_u8_dc[256]; // u8 / 255 - 0.5 (UNORM8 texel with the level-0 DC shift)
+
+__device__ __forceinline__ int mirror_repeat(int t, int R)
+{
+	if ((unsigned)t < (unsigned)R)
+		return t;
+	int period = 2 * R;
+	int m = t % period;
+	if (m < 0)
+		m += period;
+	return m < R ? m : period - 1 - m;
+}
+
+// Encoder extension: clamp-to-edge from R to A folded with whole-sample
+// mirroring about 0 and A-1 (reference shaders/dwt.comp:27-36 with a 2x2
+// gather footprint and a MIRRORED_REPEAT sampler).
+__device__ __forceinline__ int fwd_ext(int p, int R, int A)
+{
+	int c = p & ~1;
+	int coord = c < 0 ? c : c + 1;
+	if (coord >= 2 * A - R)
+		coord = coord + 2 * (R - A) + 1;
+	else if (coord > R)
+		coord = R;
+	return mirror_repeat(coord - 1 + (p & 1), R);
+}
+
+// Decoder extension: whole-sample symmetric about 0 and N-1 of the
+// interleaved signal (reference shaders/idwt.comp:25-32).
+__device__ __forceinline__ int inv_ext(int p, int N)
+{
+	while (p < 0 || p >= N)
+	{
+		if (p < 0)
+			p = -p;
+		if (p >= N)
+			p = 2 * (N - 1) - p;
+	}
+	return p;
+}
+
+__device__ __forceinline__ float load_kind(const void *base, int kind, long idx)
+{
+	if (kind == kStoreF16)
+		return __half2float(static_cast<const __half *>(base)[idx]);
+	return static_cast<const float *>(base)[idx];
+}
+
+__device__ __forceinline__ void store_kind(void *base, int kind, long idx, float v)
+{
+	if (kind == kStoreF16)
+		static_cast<__half *>(base)[idx] = __float2half_rn(v);
+	else
+		static_cast<float *>(base)[idx] = v;
+}
+
+// Forward lifting on a 16-sample window whose first sample is at an even
+// position; outputs are windows 4..11 (low at even, high at odd).
+__device__ __forceinline__ void lift_fwd16(float (&v)[16])
+{
+#pragma unroll
+	for (int i = 1; i <= 13; i += 2)
+		v[i] = fmaf(kAlpha, v[i - 1] + v[i + 1], v[i]);
+#pragma unroll
+	for (int i = 2; i <= 12; i += 2)
+		v[i] = fmaf(kBeta, v[i - 1] + v[i + 1], v[i]);
+#pragma unroll
+	for (int i = 3; i <= 11; i += 2)
+		v[i] = fmaf(kGamma, v[i - 1] + v[i + 1], v[i]);
+#pragma unroll
+	for (int i = 4; i <= 10; i += 2)
+		v[i] = fmaf(kDelta, v[i - 1] + v[i + 1], v[i]);
+}
+
+// Inverse lifting on a 16-sample window (already scaled by K / 1/K).
+__device__ __forceinline__ void lift_inv16(float (&v)[16])
+{
+#pragma unroll
+	for (int i = 2; i <= 14; i += 2)
+		v[i] = fmaf(-kDelta, v[i - 1] + v[i + 1], v[i]);
+#pragma unroll
+	for (int i = 3; i <= 13; i += 2)
+		v[i] = fmaf(-kGamma, v[i - 1] + v[i + 1], v[i]);
+#pragma unroll
+	for (int i = 4; i <= 12; i += 2)
+		v[i] = fmaf(-kBeta, v[i - 1] + v[i + 1], v[i]);
+#pragma unroll
+	for (int i = 5; i <= 11; i += 2)
+		v[i] = fmaf(-kAlpha, v[i - 1] + v[i + 1], v[i]);
+}
+
+__global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__restrict__ jobs,
+                                                             const uint16_t *__restrict__ tile_job,
+                                                             const FrameState *__restrict__ frames)
+{
+	__shared__ float sA[kTileIn * kPitchA];
+	__shared__ float sB[kTileIn * kPitchB];
+	__shared__ int s_xmap[kTileIn];
+	__shared__ int s_ymap[kTileIn];
+
+	const int tid = threadIdx.x;
+	const DwtJob J = jobs[tile_job[blockIdx.x]];
+	const int local = blockIdx.x - J.tile_base;
+	const int x0 = (local % J.tiles_x) * kTile;
+	const int y0 = (local / J.tiles_x) * kTile;
+
+	const void *src = J.src;
+	long pitch = J.src_pitch;
+	long step = 1;
+	if (J.src_kind == kStoreU8)
+	{
+		const FrameState &F = frames[J.frame];
+		src = F.src[J.component];
+		pitch = F.src_pitch[J.component];
+		step = F.src_step[J.component];
+	}
+
+	if (tid < kTileIn)
+		s_xmap[tid] = fwd_ext(x0 - kApron + tid, J.R_x, J.A_x);
+	else if (tid < 2 * kTileIn)
+		s_ymap[tid - kTileIn] = fwd_ext(y0 - kApron + tid - kTileIn, J.R_y, J.A_y);
+	__syncthreads();
+
+	if (J.src_kind == kStoreU8)
+	{
+		const uint8_t *s8 = static_cast<const uint8_t *>(src);
+		for (int i = tid; i < kTileIn * kTileIn; i += kDwtThreads)
+		{
+			int r = i / kTileIn, c = i - r * kTileIn;
+			sA[r * kPitchA + c] = c_u8_dc[s8[(long)s_ymap[r] * pitch + (long)s_xmap[c] * step]];
+		}
+	}
+	else
+	{
+		for (int i = tid; i < kTileIn * kTileIn; i += kDwtThreads)
+		{
+			int r = i / kTileIn, c = i - r * kTileIn;
+			sA[r * kPitchA + c] = load_kind(src, J.src_kind, (long)s_ymap[r] * pitch + s_xmap[c]);
+		}
+	}
+	__syncthreads();
+
+	// Horizontal pass: 72 rows x 8 segments of 8 outputs.
+	for (int task = tid; task < kTileIn * 8; task += kDwtThreads)
+	{
+		int r = task % kTileIn, seg = task / kTileIn;
+		float v[16];
+#pragma unroll
+		for (int j = 0; j < 16; j++)
+			v[j] = sA[r * kPitchA + seg * 8 + j];
+		lift_fwd16(v);
+#pragma unroll
+		for (int k = 0; k < 4; k++)
+		{
+			sB[r * kPitchB + seg * 4 + k] = v[4 + 2 * k] * kInvK;
+			sB[r * kPitchB + 32 + seg * 4 + k] = v[5 + 2 * k] * kK;
+		}
+	}
+	__syncthreads();
+
+	// Vertical pass: 64 columns x 8 segments; lanes are consecutive columns.
+	for (int task = tid; task < kTile * 8; task += kDwtThreads)
+	{
+		int c = task & 63, seg = task >> 6;
+		float v[16];
+#pragma unroll
+		for (int j = 0; j < 16; j++)
+			v[j] = sB[(seg * 8 + j) * kPitchB + c];
+		lift_fwd16(v);
+		int ox = (x0 >> 1) + (c & 31);
+		if (ox < J.bw)
+		{
+			void *lo = c < 32 ? J.band[0] : J.band[1];
+			void *hi = c < 32 ? J.band[2] : J.band[3];
+			const int lo_kind = c < 32 ? J.ll_kind : J.band_kind;
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+			{
+				int oy = (y0 >> 1) + seg * 4 + k;
+				if (oy < J.bh)
+				{
+					long idx = (long)oy * J.band_pitch + ox;
+					store_kind(lo, lo_kind, idx, v[4 + 2 * k] * kInvK);
+					store_kind(hi, J.band_kind, idx, v[5 + 2 * k] * kK);
+				}
+			}
+		}
+	}
+}
+
+__device__ __forceinline__ uint8_t unorm8_store(float v)
+{
+	// +0.5 DC shift, clamp to [0,1], round to nearest (reference
+	// shaders/idwt.comp:207-211 with an R8_UNORM store).
+	v = v + 0.5f;
+	v = v > 0.0f ? v : 0.0f;
+	v = v < 1.0f ? v : 1.0f;
+	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
+}
+
+__global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
+                                                             const uint16_t *__restrict__ tile_job,
+                                                             const FrameState *__restrict__ frames)
+{
+	__shared__ float sA[kTileIn * kPitchA];
+	__shared__ float sB[kTile * kPitchB];
+	__shared__ int s_xmap[kTileIn];
+	__shared__ int s_ymap[kTileIn];
+
+	const int tid = threadIdx.x;
+	const DwtJob J = jobs[tile_job[blockIdx.x]];
+	const int local = blockIdx.x - J.tile_base;
+	const int ox0 = (local % J.tiles_x) * kTile;
+	const int oy0 = (local / J.tiles_x) * kTile;
+
+	if (tid < kTileIn)
+		s_xmap[tid] = inv_ext(ox0 - kApron + tid, J.A_x);
+	else if (tid < 2 * kTileIn)
+		s_ymap[tid - kTileIn] = inv_ext(oy0 - kApron + tid - kTileIn, J.A_y);
+	__syncthreads();
+
+	for (int i = tid; i < kTileIn * kTileIn; i += kDwtThreads)
+	{
+		int r = i / kTileIn, c = i - r * kTileIn;
+		int py = s_ymap[r], px = s_xmap[c];
+		// Interleaved position parity selects the band: LL, HL (x odd),
+		// LH (y odd), HH.  Selects keep the job's pointers in registers.
+		const void *even_row = (px & 1) ? J.band[1] : J.band[0];
+		const void *odd_row = (px & 1) ? J.band[3] : J.band[2];
+		const void *bp = (py & 1) ? odd_row : even_row;
+		const int kind = ((py | px) & 1) ? J.band_kind : J.ll_kind;
+		sA[r * kPitchA + c] = load_kind(bp, kind, (long)(py >> 1) * J.band_pitch + (px >> 1));
+	}
+	__syncthreads();
+
+	// Vertical pass over all 72 columns; rows of the tile are windows 4..11.
+	for (int task = tid; task < kTileIn * 8; task += kDwtThreads)
+	{
+		int c = task % kTileIn, seg = task / kTileIn;
+		float v[16];
+#pragma unroll
+		for (int j = 0; j < 16; j++)
+			v[j] = sA[(seg * 8 + j) * kPitchA + c] * ((j & 1) ? kInvK : kK);
+		lift_inv16(v);
+#pragma unroll
+		for (int j = 4; j < 12; j++)
+			sB[(seg * 8 + j - 4) * kPitchB + c] = v[j];
+	}
+	__syncthreads();
+
+	// Horizontal pass: 8 lanes per row write 64 consecutive outputs.
+	uint8_t *plane = nullptr;
+	long dpitch = J.dst_pitch;
+	if (J.dst_kind == kStoreU8)
+	{
+		plane = frames[J.frame].dst[J.component];
+		dpitch = frames[J.frame].dst_pitch[J.component];
+	}
+	for (int task = tid; task < kTile * 8; task += kDwtThreads)
+	{
+		int seg = task & 7, row = task >> 3;
+		float v[16];
+#pragma unroll
+		for (int j = 0; j < 16; j++)
+			v[j] = sB[row * kPitchB + seg * 8 + j] * ((j & 1) ? kInvK : kK);
+		lift_inv16(v);
+		int py = oy0 + row;
+		if (J.dst_kind == kStoreU8)
+		{
+			if (py < J.R_y)
+			{
+#pragma unroll
+				for (int j = 4; j < 12; j++)
+				{
+					int px = ox0 + seg * 8 + j - 4;
+					if (px < J.R_x)
+						plane[(long)py * dpitch + px] = unorm8_store(v[j]);
+				}
+			}
+		}
+		else if (py < J.A_y)
+		{
+#pragma unroll
+			for (int j = 4; j < 12; j++)
+			{
+				int px = ox0 + seg * 8 + j - 4;
+				if (px < J.A_x)
+					store_kind(J.dst, J.dst_kind, (long)py * dpitch + px, v[j]);
+			}
+		}
+	}
+}
+} // namespace
+
+void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                        bool fp16, hipStream_t stream)
+{
+	(void)fp16;
+	if (tiles > 0)
+		hipLaunchKernelGGL(k_dwt_forward, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames);
+}
+
+void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                        bool fp16, hipStream_t stream)
+{
+	(void)fp16;
+	if (tiles > 0)
+		hipLaunchKernelGGL(k_dwt_inverse, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames);
+}
+
+void upload_constants()
+{
+	float lut[256];
+	for (int u = 0; u < 256; u++)
+		lut[u] = (float)u / 255.0f - 0.5f;
+	(void)hipMemcpyToSymbol(HIP_SYMBOL(c_u8_dc), lut, sizeof(lut));
+}
+} // namespace pw

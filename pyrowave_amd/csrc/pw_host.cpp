@@ -1,0 +1,725 @@
+// pw_host.cpp -- geometry, device memory, launch plans and the packet parser.
+#include "pw_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+namespace pw
+{
+namespace
+{
+int align_int(int v, int a) { return (v + a - 1) & ~(a - 1); }
+size_t align_size(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
+
+int next_pow2(int v)
+{
+	int p = 1;
+	while (p < v)
+		p <<= 1;
+	return p;
+}
+
+int floor_log2(int v)
+{
+	int r = 0;
+	while ((1 << (r + 1)) <= v)
+		r++;
+	return r;
+}
+} // namespace
+
+// ---------------------------------------------------------------------------
+// Quantizer constants (reference pyrowave_encoder.cpp:107-172,
+// pyrowave_common.hpp:124-145).
+// ---------------------------------------------------------------------------
+
+static float noise_power_normalized_quant_resolution(int level, int component, int band)
+{
+	// Precision >= 1 policy: 8 base bits.
+	int bits = 8;
+	if (band == 0)
+		bits += 2;
+	else if (band < 3)
+		bits += 1;
+	bits += level;
+	if (component != 0)
+		bits--;
+	return float(1 << bits);
+}
+
+float quant_resolution(int level, int component, int band)
+{
+	return std::min<float>(4096.0f, noise_power_normalized_quant_resolution(level, component, band));
+}
+
+float rdo_distortion_scale(int chroma, int level, int component, int band)
+{
+	float horiz_midpoint = (band & 1) ? 0.75f : 0.25f;
+	float vert_midpoint = (band & 2) ? 0.75f : 0.25f;
+	constexpr float dpi = 96.0f;
+	constexpr float viewing_distance = 1.0f;
+	constexpr float cpd_nyquist = 0.34f * viewing_distance * dpi;
+	float cpd = std::sqrt(horiz_midpoint * horiz_midpoint + vert_midpoint * vert_midpoint) * cpd_nyquist *
+	            std::exp2(-float(level));
+	cpd = std::max(cpd, 8.0f);
+	float csf = 2.6f * (0.0192f + 0.114f * cpd) * std::exp(-std::pow(0.114f * cpd, 1.1f));
+	if (component != 0 && level != kLevels - 1 && chroma == PYROWAVE_CHROMA_SUBSAMPLING_420)
+		csf *= 0.6f;
+	float resolution = noise_power_normalized_quant_resolution(level, component, band);
+	float weighted = csf * resolution;
+	return (weighted * weighted) * (1.0f / 256.0f);
+}
+
+uint8_t encode_quant(float decoder_q_scale)
+{
+	uint32_t v;
+	memcpy(&v, &decoder_q_scale, sizeof(v));
+	int e = int((v >> 23) & 0xff) - 127 - 4;
+	int m = int((v >> 20) & 0x7);
+	e = -e;
+	return uint8_t((e << 3) | m);
+}
+
+float decode_quant(uint8_t code)
+{
+	int e = 4 - (code >> 3);
+	int m = code & 0x7;
+	return (1.0f / (8.0f * 1024.0f * 1024.0f)) * float((8 + m) * (1 << (20 + e)));
+}
+
+// ---------------------------------------------------------------------------
+// Geometry
+// ---------------------------------------------------------------------------
+
+bool Geometry::init(int w, int h, int chroma_, int precision_)
+{
+	if (w <= 0 || h <= 0 || w > 16384 || h > 16384)
+		return false;
+	*this = Geometry();
+	width = w;
+	height = h;
+	chroma = chroma_;
+	precision = precision_;
+	aligned_w = std::max(align_int(w, kAlignment), kMinimumImageSize);
+	aligned_h = std::max(align_int(h, kAlignment), kMinimumImageSize);
+
+	long flat = 0;
+	size_t store = 0;
+	for (int c = 0; c < kComponents; c++)
+		for (int l = 0; l < kLevels; l++)
+		{
+			if (l == 0 && c != 0 && chroma == PYROWAVE_CHROMA_SUBSAMPLING_420)
+				continue;
+			for (int b = 0; b < kBands; b++)
+			{
+				BandGeom &B = band[c][l][b];
+				B.exists = true;
+				B.bw = (aligned_w >> 1) >> l;
+				B.bh = (aligned_h >> 1) >> l;
+				B.bx8 = (B.bw + 7) / 8;
+				B.by8 = (B.bh + 7) / 8;
+				B.bx32 = (B.bw + 31) / 32;
+				B.by32 = (B.bh + 31) / 32;
+				B.flat_off = flat;
+				flat += long(B.bw) * B.bh;
+				// FP16 path: the transmitted high bands of levels 0-1 are stored
+				// as fp16; LL intermediates stay FP32 (fp16 LL storage costs
+				// ~0.07 dB PSNR per side at high rates, the high bands ~0).
+				B.kind = (precision == PYROWAVE_HIP_PRECISION_FP16 && l < 2 && b != 0) ? kStoreF16 : kStoreF32;
+				B.pitch = B.bx32 * 32;
+				size_t bytes = size_t(B.pitch) * size_t(B.by32 * 32) * (B.kind == kStoreF16 ? 2 : 4);
+				B.store_off = store;
+				store += align_size(bytes, 256);
+			}
+		}
+	flat_count = flat;
+	store_bytes = store;
+
+	for (int l = kLevels - 1; l >= 0; l--)
+		for (int c = 0; c < kComponents; c++)
+		{
+			if (l == 0 && c != 0 && chroma == PYROWAVE_CHROMA_SUBSAMPLING_420)
+				continue;
+			for (int b = (l == kLevels - 1 ? 0 : 1); b < kBands; b++)
+			{
+				BandGeom &B = band[c][l][b];
+				B.present = true;
+				B.off8 = count8;
+				B.off32 = count32;
+				count8 += B.bx8 * B.by8;
+				count32 += B.bx32 * B.by32;
+			}
+		}
+	per_sub = next_pow2(align_int(count32, kSubdiv) / kSubdiv);
+	shamt = floor_log2(per_sub);
+	ngroups = (count32 + 255) / 256;
+	return true;
+}
+
+uint32_t Geometry::capacity_words() const
+{
+	return uint32_t(count32) * uint32_t(kMaxBlockWords);
+}
+
+// ---------------------------------------------------------------------------
+// Buffers
+// ---------------------------------------------------------------------------
+
+DeviceBuffer::~DeviceBuffer() { release(); }
+
+bool DeviceBuffer::alloc(size_t bytes)
+{
+	release();
+	if (!bytes)
+		return true;
+	if (hipMalloc(&ptr, bytes) != hipSuccess)
+	{
+		ptr = nullptr;
+		return false;
+	}
+	size = bytes;
+	return true;
+}
+
+void DeviceBuffer::release()
+{
+	if (ptr)
+		(void)hipFree(ptr);
+	ptr = nullptr;
+	size = 0;
+}
+
+HostBuffer::~HostBuffer() { release(); }
+
+bool HostBuffer::alloc(size_t bytes)
+{
+	release();
+	if (!bytes)
+		return true;
+	if (hipHostMalloc(&ptr, bytes, hipHostMallocDefault) != hipSuccess)
+	{
+		ptr = nullptr;
+		return false;
+	}
+	size = bytes;
+	return true;
+}
+
+void HostBuffer::release()
+{
+	if (ptr)
+		(void)hipHostFree(ptr);
+	ptr = nullptr;
+	size = 0;
+}
+
+// ---------------------------------------------------------------------------
+// Device: stage timing with HIP events
+// ---------------------------------------------------------------------------
+
+hipEvent_t Device::get_event()
+{
+	if (!event_pool.empty())
+	{
+		hipEvent_t e = event_pool.back();
+		event_pool.pop_back();
+		return e;
+	}
+	hipEvent_t e = nullptr;
+	(void)hipEventCreate(&e);
+	return e;
+}
+
+void Device::begin_stage(hipStream_t s, int stage, int frames, hipEvent_t *start)
+{
+	(void)stage;
+	(void)frames;
+	*start = nullptr;
+	if (!timing)
+		return;
+	*start = get_event();
+	(void)hipEventRecord(*start, s);
+}
+
+void Device::end_stage(hipStream_t s, int stage, int frames, hipEvent_t start)
+{
+	if (!timing || !start)
+		return;
+	hipEvent_t end = get_event();
+	(void)hipEventRecord(end, s);
+	pending.push_back({start, end, stage, frames});
+	if (pending.size() > 4096)
+		resolve_timings(true);
+}
+
+void Device::resolve_timings(bool wait)
+{
+	size_t keep = 0;
+	for (size_t i = 0; i < pending.size(); i++)
+	{
+		TimingRecord &r = pending[i];
+		if (!wait && hipEventQuery(r.end) != hipSuccess)
+		{
+			pending[keep++] = r;
+			continue;
+		}
+		(void)hipEventSynchronize(r.end);
+		float ms = 0.0f;
+		(void)hipEventElapsedTime(&ms, r.start, r.end);
+		stage_ms[r.stage] += ms;
+		stage_frames[r.stage] += uint64_t(r.frames);
+		event_pool.push_back(r.start);
+		event_pool.push_back(r.end);
+	}
+	pending.resize(keep);
+}
+
+Device::~Device()
+{
+	resolve_timings(true);
+	for (hipEvent_t e : event_pool)
+		(void)hipEventDestroy(e);
+	if (stream)
+		(void)hipStreamDestroy(stream);
+}
+
+// ---------------------------------------------------------------------------
+// Slot
+// ---------------------------------------------------------------------------
+
+bool Slot::init(Device *dev_, int w, int h, int chroma, int precision, bool encoder_)
+{
+	dev = dev_;
+	encoder = encoder_;
+	if (!geom.init(w, h, chroma, precision))
+		return false;
+	const Geometry &G = geom;
+	const size_t n32 = size_t(G.count32);
+
+	struct Part
+	{
+		size_t off, bytes;
+	};
+	size_t total = 0;
+	auto take = [&](size_t bytes) {
+		Part p{total, bytes};
+		total += align_size(bytes, 256);
+		return p;
+	};
+	Part p_bands = take(G.store_bytes);
+	Part p_words{}, p_bucket{}, p_pq{}, p_quant{}, p_excl{}, p_gsum{}, p_meta{}, p_totals{}, p_prefix{}, p_cls{},
+	    p_req{}, p_rc{}, p_bits{};
+	if (encoder)
+	{
+		p_words = take(n32 * 16 * sizeof(uint16_t));
+		p_bucket = take(n32 * 16);
+		p_pq = take(n32 * sizeof(int32_t));
+		p_quant = take(n32);
+		p_excl = take(n32 * sizeof(uint32_t));
+		p_gsum = take(size_t(G.ngroups) * sizeof(uint32_t));
+		p_meta = take(n32 * 2 * sizeof(uint32_t));
+		p_totals = take(kFlat * sizeof(uint32_t));
+		p_prefix = take(kFlat * sizeof(uint32_t));
+		p_cls = take(kFlat);
+		p_req = take(kFlat * sizeof(int32_t));
+		p_rc = take(kRcCount * sizeof(uint32_t));
+		p_bits = take(size_t(G.capacity_words()) * sizeof(uint32_t));
+	}
+	if (!mem.alloc(total))
+		return false;
+	if (hipMemset(mem.ptr, 0, total) != hipSuccess)
+		return false;
+	uint8_t *base = static_cast<uint8_t *>(mem.ptr);
+	bands = base + p_bands.off;
+
+	memset(&fs, 0, sizeof(fs));
+	if (encoder)
+	{
+		fs.words = reinterpret_cast<uint16_t *>(base + p_words.off);
+		fs.bucket = base + p_bucket.off;
+		fs.partial_quant = reinterpret_cast<int32_t *>(base + p_pq.off);
+		fs.quant = base + p_quant.off;
+		fs.excl = reinterpret_cast<uint32_t *>(base + p_excl.off);
+		fs.group_sum = reinterpret_cast<uint32_t *>(base + p_gsum.off);
+		fs.meta = reinterpret_cast<uint32_t *>(base + p_meta.off);
+		fs.totals = reinterpret_cast<uint32_t *>(base + p_totals.off);
+		fs.prefix = reinterpret_cast<uint32_t *>(base + p_prefix.off);
+		fs.cls = base + p_cls.off;
+		fs.required = reinterpret_cast<int32_t *>(base + p_req.off);
+		fs.rc = reinterpret_cast<uint32_t *>(base + p_rc.off);
+		fs.bitstream = reinterpret_cast<uint32_t *>(base + p_bits.off);
+		fs.capacity_words = G.capacity_words();
+	}
+	fs.n32 = G.count32;
+	fs.per_sub = G.per_sub;
+	fs.shamt = G.shamt;
+	fs.ngroups = G.ngroups;
+	fs.width = G.width;
+	fs.height = G.height;
+	fs.flags = exact_prefix ? 1u : 0u;
+
+	if (hipEventCreateWithFlags(&staging_done, hipEventDisableTiming) != hipSuccess)
+		return false;
+	if (hipEventCreateWithFlags(&result_ready, hipEventDisableTiming) != hipSuccess)
+		return false;
+
+	offsets.assign(n32, UINT32_MAX);
+	clear_packets();
+	return true;
+}
+
+Slot::~Slot()
+{
+	if (dev && dev->stream)
+		(void)hipStreamSynchronize(dev->stream);
+	single_plan.reset();
+	if (staging_done)
+		(void)hipEventDestroy(staging_done);
+	if (result_ready)
+		(void)hipEventDestroy(result_ready);
+}
+
+// Packet parser: reference pyrowave_decoder.cpp:126-262, 780-794, 832-840.
+void Slot::clear_packets()
+{
+	std::fill(offsets.begin(), offsets.end(), UINT32_MAX);
+	decoded_blocks = 0;
+	last_seq = UINT32_MAX;
+	decoded_frame = false;
+	total_blocks = geom.count32;
+	payload.clear();
+}
+
+bool Slot::push_packet(const void *data_, size_t size)
+{
+	const uint8_t *data = static_cast<const uint8_t *>(data_);
+	while (size >= 8)
+	{
+		uint32_t w[2];
+		memcpy(w, data, 8);
+		const uint32_t seq = (w[0] >> 28) & 7u;
+		if (w[0] >> 31)
+		{
+			// Sequence header (bitstream/bitstream.md:204-315).
+			if (int((w[1] >> 26) & 1u) != geom.chroma)
+				return false;
+			uint32_t diff = (seq - last_seq) & 7u;
+			if (last_seq != UINT32_MAX && diff > 3)
+				return true;
+			if (last_seq == UINT32_MAX || diff != 0)
+			{
+				clear_packets();
+				last_seq = seq;
+			}
+			if (((w[1] >> 24) & 3u) != 0)
+				return false;
+			int wd = int(w[0] & 0x3fffu) + 1, ht = int((w[0] >> 14) & 0x3fffu) + 1;
+			if (wd != geom.width || ht != geom.height)
+				return false;
+			total_blocks = int(w[1] & 0xffffffu);
+			data += 8;
+			size -= 8;
+			continue;
+		}
+
+		size_t packet_size = size_t((w[0] >> 16) & 0xfffu) * 4;
+		if (packet_size > size)
+			return false;
+		// A block must cover its own 8-byte header; a zero-length duplicate
+		// would otherwise never advance the parse.
+		if (packet_size < 8)
+			return false;
+		bool restart;
+		if (last_seq == UINT32_MAX)
+			restart = true;
+		else
+		{
+			uint32_t diff = (seq - last_seq) & 7u;
+			if (diff > 3)
+				return true;
+			restart = diff != 0;
+		}
+		if (restart)
+		{
+			clear_packets();
+			last_seq = seq;
+		}
+		uint32_t block = w[1] >> 8;
+		if (block >= uint32_t(geom.count32))
+			return false;
+		if (offsets[block] == UINT32_MAX)
+		{
+			decoded_blocks++;
+			offsets[block] = uint32_t(payload.size());
+			size_t n = payload.size();
+			payload.resize(n + packet_size / 4);
+			memcpy(payload.data() + n, data, packet_size);
+		}
+		data += packet_size;
+		size -= packet_size;
+	}
+	return size == 0;
+}
+
+bool Slot::decode_is_ready(bool allow_partial) const
+{
+	if (decoded_frame)
+		return false;
+	if (last_seq == UINT32_MAX)
+		return false;
+	if (decoded_blocks < total_blocks)
+		if (!allow_partial || decoded_blocks <= total_blocks / 2)
+			return false;
+	return true;
+}
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+
+bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
+{
+	slots = slots_;
+	encoder = encoder_;
+	const int n = int(slots.size());
+	std::vector<FrameState> host_frames(n);
+	params.assign(n, CallParams{});
+	std::vector<DwtJob> jobs[kLevels];
+	std::vector<uint16_t> tile_map[kLevels];
+	std::vector<BandJob> band_jobs;
+	std::vector<uint32_t> blk_band;
+	std::vector<int32_t> frame_block_base(n), frame_group_base(n);
+	std::vector<uint32_t> grp_frame;
+	max_n32 = 0;
+	any_fp16 = false;
+
+	for (int f = 0; f < n; f++)
+	{
+		Slot *S = slots[f];
+		const Geometry &G = S->geom;
+		host_frames[f] = S->fs;
+		max_n32 = std::max(max_n32, G.count32);
+		any_fp16 |= G.precision == PYROWAVE_HIP_PRECISION_FP16;
+		auto band_ptr = [&](int c, int l, int b) { return static_cast<void *>(S->bands + G.band[c][l][b].store_off); };
+
+		for (int l = 0; l < kLevels; l++)
+		{
+			for (int c = 0; c < kComponents; c++)
+			{
+				const int first = G.first_level(c);
+				if (l < first)
+					continue;
+				const BandGeom &B0 = G.band[c][l][0];
+				DwtJob J = {};
+				J.frame = f;
+				J.component = c;
+				J.ll_kind = B0.kind;
+				J.band_kind = G.band[c][l][1].kind;
+				J.band_pitch = B0.pitch;
+				J.bw = B0.bw;
+				J.bh = B0.bh;
+				J.fp16_math = J.band_kind == kStoreF16;
+				for (int b = 0; b < kBands; b++)
+					J.band[b] = band_ptr(c, l, b);
+				if (encoder)
+				{
+					J.A_x = G.aligned_w >> l;
+					J.A_y = G.aligned_h >> l;
+					if (l == first)
+					{
+						J.src = nullptr;
+						J.src_kind = kStoreU8;
+						J.R_x = G.plane_w(c);
+						J.R_y = G.plane_h(c);
+					}
+					else
+					{
+						const BandGeom &P = G.band[c][l - 1][0];
+						J.src = band_ptr(c, l - 1, 0);
+						J.src_kind = P.kind;
+						J.src_pitch = P.pitch;
+						J.R_x = J.A_x;
+						J.R_y = J.A_y;
+					}
+				}
+				else
+				{
+					J.A_x = 2 * B0.bw;
+					J.A_y = 2 * B0.bh;
+					if (l == first)
+					{
+						J.dst = nullptr;
+						J.dst_kind = kStoreU8;
+						J.R_x = G.plane_w(c);
+						J.R_y = G.plane_h(c);
+					}
+					else
+					{
+						const BandGeom &P = G.band[c][l - 1][0];
+						J.dst = band_ptr(c, l - 1, 0);
+						J.dst_kind = P.kind;
+						J.dst_pitch = P.pitch;
+						J.R_x = J.A_x;
+						J.R_y = J.A_y;
+					}
+				}
+				J.tiles_x = (J.A_x + kTile - 1) / kTile;
+				int tiles_y = (J.A_y + kTile - 1) / kTile;
+				J.tile_base = int(tile_map[l].size());
+				uint16_t idx = uint16_t(jobs[l].size());
+				jobs[l].push_back(J);
+				tile_map[l].insert(tile_map[l].end(), size_t(J.tiles_x) * tiles_y, idx);
+			}
+		}
+
+		frame_block_base[f] = int32_t(blk_band.size());
+		for (int l = kLevels - 1; l >= 0; l--)
+			for (int c = 0; c < kComponents; c++)
+				for (int b = 0; b < kBands; b++)
+				{
+					const BandGeom &B = G.band[c][l][b];
+					if (!B.present)
+						continue;
+					BandJob J = {};
+					J.data = band_ptr(c, l, b);
+					J.kind = B.kind;
+					J.pitch = B.pitch;
+					J.bw = B.bw;
+					J.bh = B.bh;
+					J.bx8 = B.bx8;
+					J.by8 = B.by8;
+					J.bx32 = B.bx32;
+					J.by32 = B.by32;
+					J.off32 = B.off32;
+					J.frame = f;
+					float res = quant_resolution(l, c, b);
+					J.qres = 1.0f / decode_quant(encode_quant(1.0f / res));
+					J.rdo_scale = rdo_distortion_scale(G.chroma, l, c, b);
+					J.qcode = encode_quant(1.0f / res);
+					uint32_t idx = uint32_t(band_jobs.size());
+					band_jobs.push_back(J);
+					blk_band.insert(blk_band.end(), size_t(B.bx32) * B.by32, idx);
+				}
+		frame_group_base[f] = int32_t(grp_frame.size());
+		grp_frame.insert(grp_frame.end(), size_t(G.ngroups), uint32_t(f));
+	}
+	nblocks = int(blk_band.size());
+	ngroups = int(grp_frame.size());
+	for (int l = 0; l < kLevels; l++)
+	{
+		tiles[l] = int(tile_map[l].size());
+		if (jobs[l].size() > 65535)
+			return false;
+	}
+
+	// One device allocation for every table.
+	size_t total = 0;
+	auto take = [&](size_t bytes) {
+		size_t off = total;
+		total += align_size(bytes, 256);
+		return off;
+	};
+	size_t o_frames = take(sizeof(FrameState) * n);
+	size_t o_jobs[kLevels], o_tiles[kLevels];
+	for (int l = 0; l < kLevels; l++)
+	{
+		o_jobs[l] = take(sizeof(DwtJob) * jobs[l].size());
+		o_tiles[l] = take(sizeof(uint16_t) * tile_map[l].size());
+	}
+	size_t o_bands = take(sizeof(BandJob) * band_jobs.size());
+	size_t o_blk = take(sizeof(uint32_t) * blk_band.size());
+	size_t o_fbb = take(sizeof(int32_t) * n);
+	size_t o_grp = take(sizeof(uint32_t) * grp_frame.size());
+	size_t o_fgb = take(sizeof(int32_t) * n);
+	if (!mem.alloc(total))
+		return false;
+	uint8_t *base = static_cast<uint8_t *>(mem.ptr);
+	std::vector<uint8_t> host(total, 0);
+	auto put = [&](size_t off, const void *src, size_t bytes) {
+		if (bytes)
+			memcpy(host.data() + off, src, bytes);
+	};
+	for (int l = 0; l < kLevels; l++)
+	{
+		put(o_jobs[l], jobs[l].data(), sizeof(DwtJob) * jobs[l].size());
+		put(o_tiles[l], tile_map[l].data(), sizeof(uint16_t) * tile_map[l].size());
+		d_dwt_jobs[l] = reinterpret_cast<DwtJob *>(base + o_jobs[l]);
+		d_tile_job[l] = reinterpret_cast<uint16_t *>(base + o_tiles[l]);
+	}
+	put(o_bands, band_jobs.data(), sizeof(BandJob) * band_jobs.size());
+	put(o_blk, blk_band.data(), sizeof(uint32_t) * blk_band.size());
+	put(o_fbb, frame_block_base.data(), sizeof(int32_t) * n);
+	put(o_grp, grp_frame.data(), sizeof(uint32_t) * grp_frame.size());
+	put(o_fgb, frame_group_base.data(), sizeof(int32_t) * n);
+	put(o_frames, host_frames.data(), sizeof(FrameState) * n);
+	if (hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice) != hipSuccess)
+		return false;
+	d_frames = reinterpret_cast<FrameState *>(base + o_frames);
+	d_bands = reinterpret_cast<BandJob *>(base + o_bands);
+	d_blk_band = reinterpret_cast<uint32_t *>(base + o_blk);
+	d_frame_block_base = reinterpret_cast<int32_t *>(base + o_fbb);
+	d_grp_frame = reinterpret_cast<uint32_t *>(base + o_grp);
+	d_frame_group_base = reinterpret_cast<int32_t *>(base + o_fgb);
+
+	return true;
+}
+
+FrameState *Plan::stage_frames(hipStream_t stream)
+{
+	launch_set_params(d_frames, params.data(), int(params.size()), stream);
+	return d_frames;
+}
+
+// ---------------------------------------------------------------------------
+// Launch sequences
+// ---------------------------------------------------------------------------
+
+void run_encode(Plan &P, hipStream_t stream)
+{
+	Device &D = *P.slots[0]->dev;
+	const int n = int(P.slots.size());
+	FrameState *frames = P.stage_frames(stream);
+	hipEvent_t ev;
+
+	launch_rc_reset(frames, n, P.max_n32, stream);
+
+	D.begin_stage(stream, kStageDWT, n, &ev);
+	for (int l = 0; l < kLevels; l++)
+		launch_dwt_forward(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.any_fp16, stream);
+	D.end_stage(stream, kStageDWT, n, ev);
+
+	D.begin_stage(stream, kStageQuant, n, &ev);
+	launch_quant_analyze(P.d_bands, P.d_blk_band, P.nblocks, frames, P.d_frame_block_base, stream);
+	D.end_stage(stream, kStageQuant, n, ev);
+
+	D.begin_stage(stream, kStageResolve, n, &ev);
+	launch_rc_finalize(frames, n, stream);
+	launch_rc_partial(frames, n, stream);
+	launch_rc_prepare(frames, P.d_grp_frame, P.ngroups, P.d_frame_group_base, stream);
+	D.end_stage(stream, kStageResolve, n, ev);
+
+	D.begin_stage(stream, kStagePacking, n, &ev);
+	launch_pack(P.d_bands, P.d_blk_band, P.nblocks, frames, P.d_frame_block_base, stream);
+	D.end_stage(stream, kStagePacking, n, ev);
+}
+
+void run_decode(Plan &P, hipStream_t stream)
+{
+	Device &D = *P.slots[0]->dev;
+	const int n = int(P.slots.size());
+	FrameState *frames = P.stage_frames(stream);
+	hipEvent_t ev;
+
+	D.begin_stage(stream, kStageDequant, n, &ev);
+	launch_dequant(P.d_bands, P.d_blk_band, P.nblocks, frames, P.d_frame_block_base, stream);
+	D.end_stage(stream, kStageDequant, n, ev);
+
+	D.begin_stage(stream, kStageIDWT, n, &ev);
+	for (int l = kLevels - 1; l >= 0; l--)
+		launch_dwt_inverse(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.any_fp16, stream);
+	D.end_stage(stream, kStageIDWT, n, ev);
+}
+} // namespace pw

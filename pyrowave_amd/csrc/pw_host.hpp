@@ -1,0 +1,210 @@
+// pw_host.hpp -- host-side objects of the MI355X PyroWave codec.
+#pragma once
+
+#include "pw_common.hpp"
+
+#include <atomic>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "pyrowave.h"
+#include "pyrowave_hip.h"
+
+namespace pw
+{
+enum Stage
+{
+	kStageDWT = 0,
+	kStageQuant,
+	kStageAnalyze,
+	kStageResolve,
+	kStagePacking,
+	kStageDequant,
+	kStageIDWT,
+	kStageCount
+};
+
+// Band geometry of one frame (reference pyrowave_common.cpp:191-254,
+// bitstream/bitstream.md:99-152, 487-511).
+struct BandGeom
+{
+	bool present = false; // transmitted (has block indices)
+	bool exists = false;  // produced by the transform (incl. intermediate LL)
+	int bw = 0, bh = 0, bx8 = 0, by8 = 0, bx32 = 0, by32 = 0;
+	int off8 = 0, off32 = 0;
+	long flat_off = 0;      // offset in the flat float coefficient layout
+	size_t store_off = 0;   // byte offset in the band storage allocation
+	int pitch = 0;          // storage pitch (elements), multiple of 32
+	int kind = kStoreF32;
+};
+
+struct Geometry
+{
+	int width = 0, height = 0, chroma = 0;
+	int aligned_w = 0, aligned_h = 0;
+	int count8 = 0, count32 = 0;
+	int per_sub = 0, shamt = 0, ngroups = 0;
+	long flat_count = 0;
+	size_t store_bytes = 0;
+	int precision = PYROWAVE_HIP_PRECISION_FP32;
+	BandGeom band[kComponents][kLevels][kBands];
+
+	bool init(int w, int h, int chroma, int precision);
+	int plane_w(int c) const { return (c && chroma == 0) ? width / 2 : width; }
+	int plane_h(int c) const { return (c && chroma == 0) ? height / 2 : height; }
+	int first_level(int c) const { return (c && chroma == 0) ? 1 : 0; }
+	uint32_t capacity_words() const;
+};
+
+struct DeviceBuffer
+{
+	void *ptr = nullptr;
+	size_t size = 0;
+	~DeviceBuffer();
+	bool alloc(size_t bytes);
+	void release();
+};
+
+struct HostBuffer
+{
+	void *ptr = nullptr;
+	size_t size = 0;
+	~HostBuffer();
+	bool alloc(size_t bytes);
+	void release();
+};
+
+struct TimingRecord
+{
+	hipEvent_t start, end;
+	int stage;
+	int frames;
+};
+
+struct Device
+{
+	int hip_device = 0;
+	hipStream_t stream = nullptr;
+	std::recursive_mutex lock;
+	bool timing = true;
+	double stage_ms[kStageCount] = {};
+	uint64_t stage_frames[kStageCount] = {};
+	std::vector<TimingRecord> pending;
+	std::vector<hipEvent_t> event_pool;
+	std::atomic<int> live_objects{0};
+
+	hipEvent_t get_event();
+	void begin_stage(hipStream_t s, int stage, int frames, hipEvent_t *start);
+	void end_stage(hipStream_t s, int stage, int frames, hipEvent_t start);
+	void resolve_timings(bool wait);
+	~Device();
+};
+
+struct Plan;
+
+// One frame slot: everything one encode or decode of a frame needs on the
+// device, plus the host mirrors of the CPU-buffer paths.
+struct Slot
+{
+	Device *dev = nullptr;
+	Geometry geom;
+	bool encoder = true;
+	bool exact_prefix = false;
+
+	DeviceBuffer mem;        // bands + rate-control arrays + bitstream/payload
+	FrameState fs = {};      // static fields (device pointers into mem)
+	uint8_t *bands = nullptr;
+
+	// CPU-buffer path staging.
+	DeviceBuffer dplanes;    // u8 input (encoder) or output (decoder) planes
+	HostBuffer hplanes;      // pinned staging of the planes
+	hipEvent_t staging_done = nullptr;
+
+	// Encoder results mirrored to the host.
+	HostBuffer hmeta;        // n32 * 2 words + rc words
+	HostBuffer hbits;        // bitstream words
+	uint32_t hbits_words = 0;
+	hipEvent_t result_ready = nullptr;
+	bool have_result = false;
+	bool result_on_host = false;
+	uint32_t sequence = 0;   // 3-bit frame counter (reference pyrowave_encoder.cpp:1155)
+	size_t last_budget = 0;
+
+	// Decoder packet state (reference pyrowave_decoder.cpp:34-39, 126-262).
+	std::vector<uint32_t> offsets;
+	std::vector<uint32_t> payload;
+	int decoded_blocks = 0;
+	int total_blocks = 0;
+	uint32_t last_seq = UINT32_MAX;
+	bool decoded_frame = false;
+	DeviceBuffer dpayload;   // device payload (+16 B padding)
+	DeviceBuffer doffsets;
+	HostBuffer hpayload;     // pinned staging for payload + offsets
+
+	std::unique_ptr<Plan> single_plan;
+
+	bool init(Device *dev, int w, int h, int chroma, int precision, bool encoder);
+	~Slot();
+
+	// decoder parser
+	void clear_packets();
+	bool push_packet(const void *data, size_t size);
+	bool decode_is_ready(bool allow_partial) const;
+};
+
+// Launch tables for a list of slots of one kind; cached per composition.
+struct Plan
+{
+	std::vector<Slot *> slots;
+	bool encoder = true;
+	DeviceBuffer mem;
+	FrameState *d_frames = nullptr;
+
+	DwtJob *d_dwt_jobs[kLevels] = {};
+	uint16_t *d_tile_job[kLevels] = {};
+	int tiles[kLevels] = {};
+	BandJob *d_bands = nullptr;
+	uint32_t *d_blk_band = nullptr;
+	int32_t *d_frame_block_base = nullptr;
+	uint32_t *d_grp_frame = nullptr;
+	int32_t *d_frame_group_base = nullptr;
+	int nblocks = 0;
+	int ngroups = 0;
+	int max_n32 = 0;
+	bool any_fp16 = false;
+
+	bool build(const std::vector<Slot *> &slots, bool encoder);
+	// Writes the per-call fields (call(i)) into the device FrameState array.
+	FrameState *stage_frames(hipStream_t stream);
+	CallParams &call(int i) { return params[i]; }
+	std::vector<CallParams> params;
+};
+
+// Shared helpers.
+float quant_resolution(int level, int component, int band);
+float rdo_distortion_scale(int chroma, int level, int component, int band);
+uint8_t encode_quant(float decoder_q_scale);
+float decode_quant(uint8_t code);
+
+void run_encode(Plan &plan, hipStream_t stream);
+void run_decode(Plan &plan, hipStream_t stream);
+} // namespace pw
+
+struct pyrowave_device_opaque
+{
+	pw::Device dev;
+};
+
+struct pyrowave_encoder_opaque
+{
+	pw::Slot slot;
+	pyrowave_device device = nullptr;
+};
+
+struct pyrowave_decoder_opaque
+{
+	pw::Slot slot;
+	pyrowave_device device = nullptr;
+};
