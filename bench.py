@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--cpu-frames", type=int, default=6, help="frames of the oracle CPU baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the oracle CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
@@ -79,7 +79,9 @@ def main():
     torch.cuda.synchronize()
 
     device = pw.Device()
-    stream = torch.cuda.current_stream()
+    # An explicit stream: the library launches on it, and the latency events
+    # below are recorded on the same stream.
+    stream = torch.cuda.Stream()
     sptr = stream.cuda_stream
 
     def make(prec, n):

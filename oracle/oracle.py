@@ -60,6 +60,9 @@ def lib():
         L.or_f16_to_f32.restype = C.c_float
         L.or_f16_to_f32.argtypes = [C.c_uint16]
         L.or_round_to_f16_array.argtypes = [P, C.c_long]
+        L.or_round_f16_fast.restype = C.c_float
+        L.or_round_f16_fast.argtypes = [C.c_float]
+        L.or_check_round_f16_fast.restype = C.c_long
         L.or_fwd_ext.argtypes = [C.c_int, C.c_int, C.c_int]
         for name in ("or_dwt_level_f64", "or_dwt_level_f32"):
             getattr(L, name).argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]

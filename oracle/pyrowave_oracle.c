@@ -295,6 +295,43 @@ float or_f16_to_f32(uint16_t h)
 	return f;
 }
 
+/* The kernels' shortcut for float(half(x)) on 0 <= x <= 60000: round the
+ * significand to 10 bits in the normal range, to a multiple of 2^-24 below.
+ * or_check_round_f16_fast() compares it with the general conversion for every
+ * float in that range. */
+float or_round_f16_fast(float x)
+{
+	if (x < 0x1p-14f)
+		return rintf(x * 0x1p24f) * 0x1p-24f;
+	uint32_t b;
+	memcpy(&b, &x, 4);
+	b = (b + 0xfffu + ((b >> 13) & 1u)) & ~0x1fffu;
+	float r;
+	memcpy(&r, &b, 4);
+	return r;
+}
+
+long or_check_round_f16_fast(void)
+{
+	long mismatches = 0;
+	float lim = 60000.0f;
+	uint32_t end;
+	memcpy(&end, &lim, 4);
+	for (uint32_t b = 0; b <= end; b++)
+	{
+		float x;
+		memcpy(&x, &b, 4);
+		float a = or_round_f16_fast(x);
+		float r = or_f16_to_f32(or_f32_to_f16(x));
+		uint32_t ab, rb;
+		memcpy(&ab, &a, 4);
+		memcpy(&rb, &r, 4);
+		if (ab != rb)
+			mismatches++;
+	}
+	return mismatches;
+}
+
 void or_round_to_f16_array(float *v, long n)
 {
 	for (long i = 0; i < n; i++)

@@ -91,6 +91,8 @@ float or_decode_quant(uint8_t code);
 uint16_t or_f32_to_f16(float f);
 float or_f16_to_f32(uint16_t h);
 void or_round_to_f16_array(float *v, long n);
+float or_round_f16_fast(float x);
+long or_check_round_f16_fast(void);
 
 /* Forward extension index for the encoder's sampler trick (dwt.comp:27-36). */
 int or_fwd_ext(int p, int R, int A);

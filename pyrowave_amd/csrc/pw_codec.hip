@@ -100,37 +100,69 @@ __device__ __forceinline__ void store_band(const BandJob &J, long idx, float v)
 		static_cast<float *>(J.data)[idx] = v;
 }
 
-// Butterfly sums / maxima over the 8 lanes of one 8x8 block.
+// Cross-lane moves inside 8-lane clusters with DPP: quad_perm [1,0,3,2] and
+// [2,3,0,1] swap neighbours at distance 1 and 2, row_half_mirror pairs lane i
+// with lane 7-i.  After the two quad steps all four lanes of a quad hold the
+// same value, so the mirror step completes the butterfly
+// ((v0+v1)+(v2+v3)) + ((v4+v5)+(v6+v7)) in every lane.
+constexpr int kDppXor1 = 0xB1;
+constexpr int kDppXor2 = 0x4E;
+constexpr int kDppHalfMirror = 0x141;
+
+template <int Ctrl>
+__device__ __forceinline__ int dpp_i(int v)
+{
+	return __builtin_amdgcn_update_dpp(0, v, Ctrl, 0xF, 0xF, false);
+}
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp_f(float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xF, 0xF, false));
+}
+
 __device__ __forceinline__ float cluster_sum_f(float v)
 {
-	v = v + __shfl_xor(v, 1, 8);
-	v = v + __shfl_xor(v, 2, 8);
-	v = v + __shfl_xor(v, 4, 8);
+	v = v + dpp_f<kDppXor1>(v);
+	v = v + dpp_f<kDppXor2>(v);
+	v = v + dpp_f<kDppHalfMirror>(v);
 	return v;
 }
 
 __device__ __forceinline__ int cluster_sum_i(int v)
 {
-	v += __shfl_xor(v, 1, 8);
-	v += __shfl_xor(v, 2, 8);
-	v += __shfl_xor(v, 4, 8);
+	v += dpp_i<kDppXor1>(v);
+	v += dpp_i<kDppXor2>(v);
+	v += dpp_i<kDppHalfMirror>(v);
 	return v;
 }
 
 __device__ __forceinline__ float cluster_max_f(float v)
 {
-	v = fmaxf(v, __shfl_xor(v, 1, 8));
-	v = fmaxf(v, __shfl_xor(v, 2, 8));
-	v = fmaxf(v, __shfl_xor(v, 4, 8));
+	v = fmaxf(v, dpp_f<kDppXor1>(v));
+	v = fmaxf(v, dpp_f<kDppXor2>(v));
+	v = fmaxf(v, dpp_f<kDppHalfMirror>(v));
 	return v;
 }
 
 __device__ __forceinline__ uint32_t cluster_or(uint32_t v)
 {
-	v |= __shfl_xor(v, 1, 8);
-	v |= __shfl_xor(v, 2, 8);
-	v |= __shfl_xor(v, 4, 8);
+	v |= (uint32_t)dpp_i<kDppXor1>((int)v);
+	v |= (uint32_t)dpp_i<kDppXor2>((int)v);
+	v |= (uint32_t)dpp_i<kDppHalfMirror>((int)v);
 	return v;
+}
+
+// fp16 rounding (nearest even) of 0 <= x <= 60000, returned as float: the
+// value float(half(x)).  Normal range: round the float's significand to 10
+// bits; subnormal range: round to a multiple of 2^-24.
+__device__ __forceinline__ float round_f16_nonneg(float x)
+{
+	if (x < 0x1p-14f)
+		return rintf(x * 0x1p24f) * 0x1p-24f;
+	uint32_t b = __float_as_uint(x);
+	b = (b + 0xfffu + ((b >> 13) & 1u)) & ~0x1fffu;
+	return __uint_as_float(b);
 }
 
 // Exclusive scan over the 128 lanes of a workgroup (two waves).
@@ -170,12 +202,38 @@ __device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, 
 {
 	const int x = bx * 32 + 8 * (k8 & 3) + 4 * (s >> 2);
 	const int y = by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
+	// Band storage is padded to whole 32x32 blocks (pitch and rows multiples
+	// of 32, 256-byte aligned), so both rows of the 4x2 subblock load as one
+	// aligned vector each; samples outside the band read as 0.
+	float row[2][4];
+	const long i0 = (long)y * J.pitch + x;
+	if (J.kind == kStoreF16)
+	{
+		const __half *b16 = static_cast<const __half *>(J.data);
+		const uint2 r0 = *reinterpret_cast<const uint2 *>(b16 + i0);
+		const uint2 r1 = *reinterpret_cast<const uint2 *>(b16 + i0 + J.pitch);
+		const uint32_t w[4] = {r0.x, r0.y, r1.x, r1.y};
+#pragma unroll
+		for (int j = 0; j < 4; j++)
+		{
+			row[j >> 1][2 * (j & 1)] = __half2float(__ushort_as_half((unsigned short)(w[j] & 0xffffu)));
+			row[j >> 1][2 * (j & 1) + 1] = __half2float(__ushort_as_half((unsigned short)(w[j] >> 16)));
+		}
+	}
+	else
+	{
+		const float *b32 = static_cast<const float *>(J.data);
+		const float4 r0 = *reinterpret_cast<const float4 *>(b32 + i0);
+		const float4 r1 = *reinterpret_cast<const float4 *>(b32 + i0 + J.pitch);
+		row[0][0] = r0.x; row[0][1] = r0.y; row[0][2] = r0.z; row[0][3] = r0.w;
+		row[1][0] = r1.x; row[1][1] = r1.y; row[1][2] = r1.z; row[1][3] = r1.w;
+	}
 	float ms = 0.0f;
 #pragma unroll
 	for (int p = 0; p < 8; p++)
 	{
 		int px = x + (p >> 1), py = y + (p & 1);
-		float v = (px < J.bw && py < J.bh) ? load_band(J, (long)py * J.pitch + px) : 0.0f;
+		float v = (px < J.bw && py < J.bh) ? row[p & 1][p >> 1] : 0.0f;
 		// Magnitudes beyond 16383.75 would overflow the 14 planes a block can
 		// describe; they are clamped (reachable only far outside the nominal
 		// coefficient range).
@@ -268,6 +326,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 {
 	__shared__ int s_cost[16][16];
 	__shared__ float s_dist[16][16];
+	__shared__ int s_msb[16];
+	__shared__ float s_disc[16];
 	__shared__ uint32_t s_words[16];
 	__shared__ float s_D[16];
 	__shared__ uint32_t s_v[16];
@@ -280,33 +340,48 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 
 	LaneQuant Q;
 	quantize_lane(J, C.bx, C.by, k8, s, Q);
-	const int msb = Q.msb;
+	const int msb = in_range ? Q.msb : -1;
 
 	// Distortion of discarding the whole 8x8 block; the reference leaves out
 	// rdo_distortion_scale here (reference shaders/wavelet_quant.comp:254).
 	float dd = cluster_sum_f((dot4_sq(Q.t[0], Q.t[1], Q.t[2], Q.t[3]) + dot4_sq(Q.t[4], Q.t[5], Q.t[6], Q.t[7])) * Q.iq2);
-	float dist_discard = msb >= 0 ? f16_bits_to_f32(f32_to_f16_bits(fminf(dd, 60000.0f))) : 0.0f;
+	if (s == 0)
+	{
+		s_msb[k8] = msb;
+		s_disc[k8] = msb >= 0 ? round_f16_nonneg(fminf(dd, 60000.0f)) : 0.0f;
+	}
+	__syncthreads();
+	int qmax = -1;
+#pragma unroll
+	for (int k = 0; k < 16; k++)
+		qmax = s_msb[k] > qmax ? s_msb[k] : qmax;
 
-	for (int q = 0; q < 16; q++)
+	float a[8];
+#pragma unroll
+	for (int p = 0; p < 8; p++)
+		a[p] = fabsf(Q.t[p]);
+	for (int q = 0; q <= qmax; q++)
 	{
 		// compute_quant_stats (reference shaders/wavelet_quant.comp:113-165).
 		float se_lane = 0.0f;
-		int late_lane = 0, nsig_lane = 0;
+		int packed = 0; // late planes | significant count << 16
 		if (q <= msb)
 		{
 			float err[8];
+			int nsig = 0;
+			const float half_step = q ? (float)(1 << (q - 1)) : 0.0f;
 #pragma unroll
 			for (int p = 0; p < 8; p++)
 			{
 				int iv = Q.n[p] >> q;
-				float a = fabsf(Q.t[p]);
 				float recon = 0.0f;
 				if (iv != 0)
 				{
-					nsig_lane++;
-					recon = q == 0 ? (float)iv : ((float)iv + 0.5f) * (float)(1 << q);
+					nsig++;
+					// (iv + 0.5) * 2^q, or iv at q = 0
+					recon = (float)(iv << q) + half_step;
 				}
-				err[p] = a - recon;
+				err[p] = a[p] - recon;
 			}
 			se_lane = ((dot4_sq(err[0], err[1], err[2], err[3]) + dot4_sq(err[4], err[5], err[6], err[7])) *
 			           J.rdo_scale) * Q.iq2;
@@ -320,27 +395,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 				b >>= qp;
 			}
 			early += find_msb((uint32_t)b) + 1;
-			late_lane = early - 1 > 0 ? early - 1 : 0;
+			packed = (early - 1 > 0 ? early - 1 : 0) | (nsig << 16);
 		}
 		float se = cluster_sum_f(se_lane);
-		int late = cluster_sum_i(late_lane);
-		int nsig = cluster_sum_i(nsig_lane);
-		if (s == 0)
+		int sums = cluster_sum_i(packed);
+		if (s == 0 && q <= msb)
 		{
-			int cost = 0;
-			float dist = 0.0f;
-			if (in_range)
-			{
-				if (q <= msb)
-				{
-					cost = 8 * late + nsig;
-					dist = q == 0 ? 0.0f : f16_bits_to_f32(f32_to_f16_bits(fminf(se, 60000.0f)));
-				}
-				else
-					dist = dist_discard;
-			}
-			s_cost[q][k8] = cost;
-			s_dist[q][k8] = dist;
+			s_cost[q][k8] = 8 * (sums & 0xffff) + (sums >> 16);
+			s_dist[q][k8] = q == 0 ? 0.0f : round_f16_nonneg(fminf(se, 60000.0f));
 		}
 	}
 	__syncthreads();
@@ -351,12 +413,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 		// reference shaders/analyze_rate_control.comp:169-215.
 		const int q = tid;
 		uint32_t total = 0;
+		float d[16];
 #pragma unroll
 		for (int k = 0; k < 16; k++)
-			total += s_cost[q][k] ? (uint32_t)s_cost[q][k] + 24u : 0u;
+		{
+			// errors[min(q, num_planes)]: past the last plane the entry is the
+			// cost-free "discard" one; zero 8x8 blocks are (0, 0) throughout.
+			const bool live = q <= s_msb[k];
+			const uint32_t c = live ? (uint32_t)s_cost[q][k] : 0u;
+			d[k] = live ? s_dist[q][k] : s_disc[k];
+			total += c ? c + 24u : 0u;
+		}
 		if (total)
 			total += 64;
-		const float *d = s_dist[q];
 		float D = (((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7]))) +
 		          (((d[8] + d[9]) + (d[10] + d[11])) + ((d[12] + d[13]) + (d[14] + d[15])));
 		s_words[q] = (total + 31) >> 5;
@@ -392,9 +461,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 		}
 		F.words[(long)C.b * 16 + q] = (uint16_t)s_words[q];
 		F.bucket[(long)C.b * 16 + q] = (uint8_t)Bq;
-		if (q == 0)
-			atomicAdd(&F.rc[kRcConsumed], s_words[0]);
-		else
+		if (q != 0)
 		{
 			uint32_t saving = s_words[q - 1] - s_words[q];
 			if (saving)
@@ -447,8 +514,26 @@ __global__ void k_rc_reset(FrameState *__restrict__ frames)
 __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ frames)
 {
 	__shared__ uint32_t s_scan[512];
+	__shared__ uint32_t s_consumed;
 	const FrameState &F = frames[blockIdx.x];
 	const int t = threadIdx.x;
+	// Words of every block at quant level 0 (the reference accumulates them
+	// with one atomic per block, reference shaders/analyze_rate_control.comp:127-131).
+	{
+		uint32_t acc = 0;
+		for (int b = t; b < F.n32; b += 512)
+			acc += F.words[(long)b * 16];
+		for (int d = 32; d >= 1; d >>= 1)
+			acc += __shfl_xor(acc, d, 64);
+		if (t == 0)
+			s_consumed = 0;
+		__syncthreads();
+		if ((t & 63) == 0)
+			atomicAdd(&s_consumed, acc);
+		__syncthreads();
+		if (t == 0)
+			F.rc[kRcConsumed] = s_consumed;
+	}
 	uint32_t v[4];
 #pragma unroll
 	for (int i = 0; i < 4; i++)
@@ -471,7 +556,7 @@ __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ fr
 		s_scan[t] = v[3];
 	}
 	__syncthreads();
-	const int required_base = (int)F.rc[kRcConsumed] - F.target_words;
+	const int required_base = (int)s_consumed - F.target_words;
 	uint32_t prev = t ? s_scan[t - 1] : 0u;
 #pragma unroll
 	for (int i = 0; i < 4; i++)
@@ -625,9 +710,9 @@ __device__ __forceinline__ uint32_t quantize_code_word(uint32_t cw, int quant)
 	return cw;
 }
 
-__device__ __forceinline__ void lds_or_byte(uint32_t *buf, uint32_t byte_pos, uint32_t value)
+__device__ __forceinline__ void lds_store_byte(uint32_t *buf, uint32_t byte_pos, uint32_t value)
 {
-	atomicOr(&buf[byte_pos >> 2], (value & 0xffu) << (8 * (byte_pos & 3)));
+	reinterpret_cast<uint8_t *>(buf)[byte_pos] = (uint8_t)value;
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restrict__ bands,
@@ -729,28 +814,40 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	int sig_total;
 	const int sig_excl = block_excl_scan128(nsig, s_tmp, sig_total);
 
+	// Lanes own disjoint byte ranges of the block, so control words, scale
+	// bytes and plane bytes are plain byte stores; sign bits share words and
+	// go in with at most two atomic ORs per lane.
 	const uint32_t plane_base = 8 + 3 * N;
 	if (active && s == 0)
 	{
 		uint32_t j = __popc(ballot & ((1u << k8) - 1u));
-		lds_or_byte(s_buf, 8 + 2 * j, cw & 0xffu);
-		lds_or_byte(s_buf, 9 + 2 * j, (cw >> 8) & 0xffu);
-		lds_or_byte(s_buf, 8 + 2 * N + j, (cw >> 16) & 0xffu);
+		lds_store_byte(s_buf, 8 + 2 * j, cw & 0xffu);
+		lds_store_byte(s_buf, 9 + 2 * j, (cw >> 8) & 0xffu);
+		lds_store_byte(s_buf, 8 + 2 * N + j, (cw >> 16) & 0xffu);
 	}
-	for (uint32_t pi = 0; pi < out_planes; pi++)
-		lds_or_byte(s_buf, plane_base + (uint32_t)plane_excl + pi, plane_bytes_lane[pi]);
+#pragma unroll
+	for (int pi = 0; pi < 14; pi++)
+		if ((uint32_t)pi < out_planes)
+			lds_store_byte(s_buf, plane_base + (uint32_t)plane_excl + pi, plane_bytes_lane[pi]);
+	if (sig)
 	{
-		uint32_t sbit = 8u * (plane_base + (uint32_t)planes_total) + (uint32_t)sig_excl;
+		uint32_t bits = 0;
+		int n = 0;
 #pragma unroll
 		for (int p = 0; p < 8; p++)
 		{
 			if ((sig >> p) & 1u)
 			{
-				if (Q.t[p] < 0.0f)
-					atomicOr(&s_buf[sbit >> 5], 1u << (sbit & 31));
-				sbit++;
+				bits |= (Q.t[p] < 0.0f ? 1u : 0u) << n;
+				n++;
 			}
 		}
+		const uint32_t sbit = 8u * (plane_base + (uint32_t)planes_total) + (uint32_t)sig_excl;
+		const uint64_t wide = (uint64_t)bits << (sbit & 31);
+		if ((uint32_t)wide)
+			atomicOr(&s_buf[sbit >> 5], (uint32_t)wide);
+		if ((uint32_t)(wide >> 32))
+			atomicOr(&s_buf[(sbit >> 5) + 1], (uint32_t)(wide >> 32));
 	}
 	if (tid == 0)
 	{
@@ -759,8 +856,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 		e = e > 0 ? e : 0;
 		uint32_t qc = (J.qcode & ~(31u << 3)) | ((uint32_t)e << 3);
 		uint32_t w1 = qc | ((uint32_t)C.b << 8);
-		atomicOr(&s_buf[0], w0);
-		atomicOr(&s_buf[1], w1);
+		s_buf[0] = w0;
+		s_buf[1] = w1;
 	}
 	__syncthreads();
 
@@ -780,19 +877,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	}
 }
 
-__device__ __forceinline__ uint32_t payload_byte(const uint32_t *payload, uint32_t payload_bytes, uint32_t pos)
+__device__ __forceinline__ uint32_t lds_byte(const uint32_t *buf, uint32_t nbytes, uint32_t pos)
 {
-	if (pos >= payload_bytes)
+	if (pos >= nbytes)
 		return 0;
-	return (payload[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
+	return (buf[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
 }
 
-// reference shaders/wavelet_dequant.comp:100-160 and 220-391.
+// reference shaders/wavelet_dequant.comp:100-160 and 220-391.  The coded
+// block (at most kMaxBlockWords words for any block an encoder can produce) is
+// first copied to LDS with coalesced loads; the parse then reads LDS only.
+// Bytes past the block, or past the payload, read as zero.
 __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
                                                            const uint32_t *__restrict__ blk_band,
                                                            const FrameState *__restrict__ frames,
                                                            const int32_t *__restrict__ frame_block_base)
 {
+	__shared__ uint32_t s_pay[kMaxBlockWords];
 	__shared__ uint32_t s_poff[16];
 	__shared__ uint32_t s_ctrl[16];
 	__shared__ uint32_t s_qb[16];
@@ -810,15 +911,24 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	bool missing = off == 0xffffffffu;
 	if (F.dec_offset_stride == 2 && F.dec_offsets[2L * C.b + 1] == 0)
 		missing = true;
+	const uint32_t payload_words = F.dec_payload_bytes >> 2;
+	if (!missing && off >= payload_words)
+		missing = true;
 
 	float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	if (!missing)
 	{
-		const uint32_t *payload = F.dec_payload;
-		const uint32_t pbytes = F.dec_payload_bytes;
-		const uint32_t base = off * 4u;
-		const uint32_t ballot = payload_byte(payload, pbytes, base) | (payload_byte(payload, pbytes, base + 1) << 8);
-		const uint32_t q_code = payload_byte(payload, pbytes, base + 4);
+		const uint32_t *payload = F.dec_payload + off;
+		const uint32_t w0 = payload[0];
+		uint32_t nwords = (w0 >> 16) & 0xfffu;
+		nwords = nwords < (uint32_t)kMaxBlockWords ? nwords : (uint32_t)kMaxBlockWords;
+		nwords = nwords < payload_words - off ? nwords : payload_words - off;
+		for (uint32_t i = tid; i < nwords; i += kBlockThreads)
+			s_pay[i] = payload[i];
+		__syncthreads();
+		const uint32_t nbytes = nwords * 4;
+		const uint32_t ballot = nbytes >= 4 ? (s_pay[0] & 0xffffu) : 0u;
+		const uint32_t q_code = lds_byte(s_pay, nbytes, 4);
 		const uint32_t N = __popc(ballot);
 		if (tid < 16)
 		{
@@ -826,9 +936,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			if ((ballot >> tid) & 1u)
 			{
 				uint32_t j = __popc(ballot & ((1u << tid) - 1u));
-				ctrl = payload_byte(payload, pbytes, base + 8 + 2 * j) |
-				       (payload_byte(payload, pbytes, base + 9 + 2 * j) << 8);
-				qb = payload_byte(payload, pbytes, base + 8 + 2 * N + j);
+				ctrl = lds_byte(s_pay, nbytes, 8 + 2 * j) | (lds_byte(s_pay, nbytes, 9 + 2 * j) << 8);
+				qb = lds_byte(s_pay, nbytes, 8 + 2 * N + j);
 			}
 			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
 			msbs |= msbs >> 1;
@@ -841,11 +950,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 				if (tid >= d)
 					incl += up;
 			}
-			s_poff[tid] = base + 8 + 3 * N + incl - cost;
+			s_poff[tid] = 8 + 3 * N + incl - cost;
 			s_ctrl[tid] = ctrl;
 			s_qb[tid] = qb;
 			if (tid == 15)
-				s_sign_base = 8u * (base + 8 + 3 * N + incl);
+				s_sign_base = 8u * (8 + 3 * N + incl);
 		}
 		__syncthreads();
 
@@ -863,7 +972,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			int mag[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 			for (int pl = planes - 1; pl >= 0; pl--)
 			{
-				uint32_t byte = payload_byte(payload, pbytes, byte_off++);
+				uint32_t byte = lds_byte(s_pay, nbytes, byte_off++);
 #pragma unroll
 				for (int p = 0; p < 8; p++)
 					mag[p] |= (int)((byte >> p) & 1u) << pl;
@@ -895,7 +1004,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		{
 			if (v[p] != 0.0f)
 			{
-				uint32_t byte = payload_byte(payload, pbytes, sbit >> 3);
+				uint32_t byte = lds_byte(s_pay, nbytes, sbit >> 3);
 				if ((byte >> (sbit & 7)) & 1u)
 					v[p] = -v[p];
 				sbit++;

@@ -26,8 +26,8 @@ constexpr float kDelta = float(0.443506852043971);
 constexpr float kK = float(1.230174104914001);
 constexpr float kInvK = float(1.0 / 1.230174104914001);
 
+constexpr int kLoadIters = (kTileIn * kTileIn + kDwtThreads - 1) / kDwtThreads;
 constexpr int kPitchA = kTileIn + 1; // odd pitch: row-per-lane reads are conflict-free
-constexpr int kPitchB = kTileIn + 1;
 
 __constant__ float c_u8_dc[256]; // u8 / 255 - 0.5 (UNORM8 texel with the level-0 DC shift)
 
@@ -120,12 +120,56 @@ __device__ __forceinline__ void lift_inv16(float (&v)[16])
 		v[i] = fmaf(-kAlpha, v[i - 1] + v[i + 1], v[i]);
 }
 
+// Loads the 72x72 source tile (rows / columns through the extension maps)
+// into registers: element i = tid + 256 * k; the last round (k = 20) is
+// done by the first wave only.  Every load is issued before any is consumed.
+template <typename T>
+__device__ __forceinline__ void load_tile(const T *src, long pitch, long step, const int *ymap, const int *xmap,
+                                          int tid, float (&vals)[kLoadIters])
+{
+#pragma unroll
+	for (int k = 0; k < kLoadIters; k++)
+	{
+		const int i = tid + k * kDwtThreads;
+		vals[k] = 0.0f;
+		if (k < kLoadIters - 1 || tid < kTileIn * kTileIn - (kLoadIters - 1) * kDwtThreads)
+		{
+			const int r = i / kTileIn, c = i - r * kTileIn;
+			const T v = src[(long)ymap[r] * pitch + (long)xmap[c] * step];
+			if constexpr (sizeof(T) == 1)
+				vals[k] = c_u8_dc[v];
+			else if constexpr (sizeof(T) == 2)
+				vals[k] = __half2float(v);
+			else
+				vals[k] = v;
+		}
+	}
+}
+
+__device__ __forceinline__ void store_tile(float *sA, int tid, const float (&vals)[kLoadIters])
+{
+#pragma unroll
+	for (int k = 0; k < kLoadIters; k++)
+	{
+		const int i = tid + k * kDwtThreads;
+		if (k < kLoadIters - 1 || tid < kTileIn * kTileIn - (kLoadIters - 1) * kDwtThreads)
+		{
+			const int r = i / kTileIn, c = i - r * kTileIn;
+			sA[r * kPitchA + c] = vals[k];
+		}
+	}
+}
+
+// First-pass work: 576 windows (72 lines x 8 segments); thread tid takes
+// tasks tid, tid + 256 and (first wave only) tid + 512.
+constexpr int kPass1Tasks = kTileIn * 8;
+constexpr int kPass1Rounds = (kPass1Tasks + kDwtThreads - 1) / kDwtThreads;
+
 __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__restrict__ jobs,
                                                              const uint16_t *__restrict__ tile_job,
                                                              const FrameState *__restrict__ frames)
 {
 	__shared__ float sA[kTileIn * kPitchA];
-	__shared__ float sB[kTileIn * kPitchB];
 	__shared__ int s_xmap[kTileIn];
 	__shared__ int s_ymap[kTileIn];
 
@@ -135,70 +179,78 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 	const int x0 = (local % J.tiles_x) * kTile;
 	const int y0 = (local / J.tiles_x) * kTile;
 
-	const void *src = J.src;
-	long pitch = J.src_pitch;
-	long step = 1;
-	if (J.src_kind == kStoreU8)
-	{
-		const FrameState &F = frames[J.frame];
-		src = F.src[J.component];
-		pitch = F.src_pitch[J.component];
-		step = F.src_step[J.component];
-	}
-
 	if (tid < kTileIn)
 		s_xmap[tid] = fwd_ext(x0 - kApron + tid, J.R_x, J.A_x);
 	else if (tid < 2 * kTileIn)
 		s_ymap[tid - kTileIn] = fwd_ext(y0 - kApron + tid - kTileIn, J.R_y, J.A_y);
 	__syncthreads();
 
+	float vals[kLoadIters];
 	if (J.src_kind == kStoreU8)
 	{
-		const uint8_t *s8 = static_cast<const uint8_t *>(src);
-		for (int i = tid; i < kTileIn * kTileIn; i += kDwtThreads)
-		{
-			int r = i / kTileIn, c = i - r * kTileIn;
-			sA[r * kPitchA + c] = c_u8_dc[s8[(long)s_ymap[r] * pitch + (long)s_xmap[c] * step]];
-		}
+		const FrameState &F = frames[J.frame];
+		load_tile(F.src[J.component], F.src_pitch[J.component], F.src_step[J.component], s_ymap, s_xmap, tid, vals);
 	}
+	else if (J.src_kind == kStoreF16)
+		load_tile(static_cast<const __half *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, tid, vals);
 	else
+		load_tile(static_cast<const float *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, tid, vals);
+	store_tile(sA, tid, vals);
+	__syncthreads();
+
+	// Horizontal pass: results stay in registers until every thread has read
+	// its windows, then overwrite the tile in place (low half | high half).
+	float h[kPass1Rounds][8];
+#pragma unroll
+	for (int t = 0; t < kPass1Rounds; t++)
 	{
-		for (int i = tid; i < kTileIn * kTileIn; i += kDwtThreads)
+		const int task = tid + t * kDwtThreads;
+		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
 		{
-			int r = i / kTileIn, c = i - r * kTileIn;
-			sA[r * kPitchA + c] = load_kind(src, J.src_kind, (long)s_ymap[r] * pitch + s_xmap[c]);
+			const int r = task % kTileIn, seg = task / kTileIn;
+			float v[16];
+#pragma unroll
+			for (int j = 0; j < 16; j++)
+				v[j] = sA[r * kPitchA + seg * 8 + j];
+			lift_fwd16(v);
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+			{
+				h[t][k] = v[4 + 2 * k] * kInvK;
+				h[t][4 + k] = v[5 + 2 * k] * kK;
+			}
 		}
 	}
 	__syncthreads();
-
-	// Horizontal pass: 72 rows x 8 segments of 8 outputs.
-	for (int task = tid; task < kTileIn * 8; task += kDwtThreads)
+#pragma unroll
+	for (int t = 0; t < kPass1Rounds; t++)
 	{
-		int r = task % kTileIn, seg = task / kTileIn;
-		float v[16];
-#pragma unroll
-		for (int j = 0; j < 16; j++)
-			v[j] = sA[r * kPitchA + seg * 8 + j];
-		lift_fwd16(v);
-#pragma unroll
-		for (int k = 0; k < 4; k++)
+		const int task = tid + t * kDwtThreads;
+		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
 		{
-			sB[r * kPitchB + seg * 4 + k] = v[4 + 2 * k] * kInvK;
-			sB[r * kPitchB + 32 + seg * 4 + k] = v[5 + 2 * k] * kK;
+			const int r = task % kTileIn, seg = task / kTileIn;
+#pragma unroll
+			for (int k = 0; k < 4; k++)
+			{
+				sA[r * kPitchA + seg * 4 + k] = h[t][k];
+				sA[r * kPitchA + 32 + seg * 4 + k] = h[t][4 + k];
+			}
 		}
 	}
 	__syncthreads();
 
 	// Vertical pass: 64 columns x 8 segments; lanes are consecutive columns.
-	for (int task = tid; task < kTile * 8; task += kDwtThreads)
+#pragma unroll
+	for (int t = 0; t < 2; t++)
 	{
-		int c = task & 63, seg = task >> 6;
+		const int task = tid + t * kDwtThreads;
+		const int c = task & 63, seg = task >> 6;
 		float v[16];
 #pragma unroll
 		for (int j = 0; j < 16; j++)
-			v[j] = sB[(seg * 8 + j) * kPitchB + c];
+			v[j] = sA[(seg * 8 + j) * kPitchA + c];
 		lift_fwd16(v);
-		int ox = (x0 >> 1) + (c & 31);
+		const int ox = (x0 >> 1) + (c & 31);
 		if (ox < J.bw)
 		{
 			void *lo = c < 32 ? J.band[0] : J.band[1];
@@ -207,10 +259,10 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 #pragma unroll
 			for (int k = 0; k < 4; k++)
 			{
-				int oy = (y0 >> 1) + seg * 4 + k;
+				const int oy = (y0 >> 1) + seg * 4 + k;
 				if (oy < J.bh)
 				{
-					long idx = (long)oy * J.band_pitch + ox;
+					const long idx = (long)oy * J.band_pitch + ox;
 					store_kind(lo, lo_kind, idx, v[4 + 2 * k] * kInvK);
 					store_kind(hi, J.band_kind, idx, v[5 + 2 * k] * kK);
 				}
@@ -229,12 +281,48 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
+// Loads one band's 36x36 share of the interleaved 72x72 tile: positions
+// whose parity (row, column) is (b >> 1, b & 1).
+template <typename T>
+__device__ __forceinline__ void load_band_quarter(const T *band, long pitch, int b, const int *ymap,
+                                                  const int *xmap, int tid, float *sA)
+{
+	constexpr int kQ = kTileIn / 2;
+	constexpr int kIters = (kQ * kQ + kDwtThreads - 1) / kDwtThreads;
+	float vals[kIters];
+#pragma unroll
+	for (int k = 0; k < kIters; k++)
+	{
+		const int i = tid + k * kDwtThreads;
+		vals[k] = 0.0f;
+		if (i < kQ * kQ)
+		{
+			const int rr = i / kQ, cc = i - rr * kQ;
+			const int r = 2 * rr + (b >> 1), c = 2 * cc + (b & 1);
+			const T v = band[(long)(ymap[r] >> 1) * pitch + (xmap[c] >> 1)];
+			if constexpr (sizeof(T) == 2)
+				vals[k] = __half2float(v);
+			else
+				vals[k] = v;
+		}
+	}
+#pragma unroll
+	for (int k = 0; k < kIters; k++)
+	{
+		const int i = tid + k * kDwtThreads;
+		if (i < kQ * kQ)
+		{
+			const int rr = i / kQ, cc = i - rr * kQ;
+			sA[(2 * rr + (b >> 1)) * kPitchA + 2 * cc + (b & 1)] = vals[k];
+		}
+	}
+}
+
 __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
                                                              const uint16_t *__restrict__ tile_job,
                                                              const FrameState *__restrict__ frames)
 {
 	__shared__ float sA[kTileIn * kPitchA];
-	__shared__ float sB[kTile * kPitchB];
 	__shared__ int s_xmap[kTileIn];
 	__shared__ int s_ymap[kTileIn];
 
@@ -250,32 +338,52 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 		s_ymap[tid - kTileIn] = inv_ext(oy0 - kApron + tid - kTileIn, J.A_y);
 	__syncthreads();
 
-	for (int i = tid; i < kTileIn * kTileIn; i += kDwtThreads)
+	// Interleaved position parity selects the band: LL (even, even), HL (x
+	// odd), LH (y odd), HH.  The extension preserves parity.
+	load_band_quarter(static_cast<const float *>(J.band[0]), J.band_pitch, 0, s_ymap, s_xmap, tid, sA);
+	if (J.band_kind == kStoreF16)
 	{
-		int r = i / kTileIn, c = i - r * kTileIn;
-		int py = s_ymap[r], px = s_xmap[c];
-		// Interleaved position parity selects the band: LL, HL (x odd),
-		// LH (y odd), HH.  Selects keep the job's pointers in registers.
-		const void *even_row = (px & 1) ? J.band[1] : J.band[0];
-		const void *odd_row = (px & 1) ? J.band[3] : J.band[2];
-		const void *bp = (py & 1) ? odd_row : even_row;
-		const int kind = ((py | px) & 1) ? J.band_kind : J.ll_kind;
-		sA[r * kPitchA + c] = load_kind(bp, kind, (long)(py >> 1) * J.band_pitch + (px >> 1));
+		for (int b = 1; b < 4; b++)
+			load_band_quarter(static_cast<const __half *>(J.band[b]), J.band_pitch, b, s_ymap, s_xmap, tid, sA);
+	}
+	else
+	{
+		for (int b = 1; b < 4; b++)
+			load_band_quarter(static_cast<const float *>(J.band[b]), J.band_pitch, b, s_ymap, s_xmap, tid, sA);
 	}
 	__syncthreads();
 
 	// Vertical pass over all 72 columns; rows of the tile are windows 4..11.
-	for (int task = tid; task < kTileIn * 8; task += kDwtThreads)
+	float vout[kPass1Rounds][8];
+#pragma unroll
+	for (int t = 0; t < kPass1Rounds; t++)
 	{
-		int c = task % kTileIn, seg = task / kTileIn;
-		float v[16];
+		const int task = tid + t * kDwtThreads;
+		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
+		{
+			const int c = task % kTileIn, seg = task / kTileIn;
+			float v[16];
 #pragma unroll
-		for (int j = 0; j < 16; j++)
-			v[j] = sA[(seg * 8 + j) * kPitchA + c] * ((j & 1) ? kInvK : kK);
-		lift_inv16(v);
+			for (int j = 0; j < 16; j++)
+				v[j] = sA[(seg * 8 + j) * kPitchA + c] * ((j & 1) ? kInvK : kK);
+			lift_inv16(v);
 #pragma unroll
-		for (int j = 4; j < 12; j++)
-			sB[(seg * 8 + j - 4) * kPitchB + c] = v[j];
+			for (int j = 0; j < 8; j++)
+				vout[t][j] = v[4 + j];
+		}
+	}
+	__syncthreads();
+#pragma unroll
+	for (int t = 0; t < kPass1Rounds; t++)
+	{
+		const int task = tid + t * kDwtThreads;
+		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
+		{
+			const int c = task % kTileIn, seg = task / kTileIn;
+#pragma unroll
+			for (int j = 0; j < 8; j++)
+				sA[(seg * 8 + j) * kPitchA + c] = vout[t][j];
+		}
 	}
 	__syncthreads();
 
@@ -287,15 +395,17 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 		plane = frames[J.frame].dst[J.component];
 		dpitch = frames[J.frame].dst_pitch[J.component];
 	}
-	for (int task = tid; task < kTile * 8; task += kDwtThreads)
+#pragma unroll
+	for (int t = 0; t < 2; t++)
 	{
-		int seg = task & 7, row = task >> 3;
+		const int task = tid + t * kDwtThreads;
+		const int seg = task & 7, row = task >> 3;
 		float v[16];
 #pragma unroll
 		for (int j = 0; j < 16; j++)
-			v[j] = sB[row * kPitchB + seg * 8 + j] * ((j & 1) ? kInvK : kK);
+			v[j] = sA[row * kPitchA + seg * 8 + j] * ((j & 1) ? kInvK : kK);
 		lift_inv16(v);
-		int py = oy0 + row;
+		const int py = oy0 + row;
 		if (J.dst_kind == kStoreU8)
 		{
 			if (py < J.R_y)
@@ -303,7 +413,7 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 #pragma unroll
 				for (int j = 4; j < 12; j++)
 				{
-					int px = ox0 + seg * 8 + j - 4;
+					const int px = ox0 + seg * 8 + j - 4;
 					if (px < J.R_x)
 						plane[(long)py * dpitch + px] = unorm8_store(v[j]);
 				}
@@ -314,7 +424,7 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 #pragma unroll
 			for (int j = 4; j < 12; j++)
 			{
-				int px = ox0 + seg * 8 + j - 4;
+				const int px = ox0 + seg * 8 + j - 4;
 				if (px < J.A_x)
 					store_kind(J.dst, J.dst_kind, (long)py * dpitch + px, v[j]);
 			}
