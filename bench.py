@@ -38,6 +38,11 @@ def parse():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the oracle CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain 256 MiB read + 256 MiB write elementwise kernel "
+                         "(counter calibration for HBM traffic)")
+    ap.add_argument("--trace-summary", metavar="CSV",
+                    help="summarize a rocprofv3 kernel trace CSV per (kernel, workgroups) and exit")
     return ap.parse_args()
 
 
@@ -60,8 +65,34 @@ def dwt_algorithmic_bytes(fp16):
     return total
 
 
+def trace_summary(path, top=40):
+    """Per-(kernel, workgroup count) dispatch statistics of a rocprofv3
+    kernel-trace CSV.  The workgroup count separates launches of one kernel
+    over different batch sizes and transform levels."""
+    import collections
+    import csv
+    import re
+    agg = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            m = re.search(r"(k_[a-z_0-9]+)", r["Kernel_Name"])
+            name = m.group(1) if m else r["Kernel_Name"][:48]
+            wgs = (int(r["Grid_Size_X"]) // max(int(r["Workgroup_Size_X"]), 1)) * \
+                  (int(r["Grid_Size_Y"]) // max(int(r["Workgroup_Size_Y"]), 1)) * \
+                  (int(r["Grid_Size_Z"]) // max(int(r["Workgroup_Size_Z"]), 1))
+            agg[(name, wgs)].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    total = sum(sum(v) for v in agg.values())
+    print(f"{'kernel':48s} {'workgroups':>10s} {'calls':>6s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'share':>6s}")
+    for (name, wgs), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
+        print(f"{name:48s} {wgs:10d} {len(v):6d} {sum(v) / len(v) / 1e3:9.2f} {min(v) / 1e3:9.2f} "
+              f"{max(v) / 1e3:9.2f} {sum(v) / total:6.1%}")
+
+
 def main():
     args = parse()
+    if args.trace_summary:
+        trace_summary(args.trace_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               "(frames shard across GPUs with no exchange; see DESIGN.md).", file=sys.stderr)
@@ -77,6 +108,14 @@ def main():
     print(f"[bench] generated {FRAMES} frames in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
     dev_frames = [[torch.from_numpy(p).cuda() for p in f] for f in host_frames]
     torch.cuda.synchronize()
+    if args.calibrate_copy:
+        # 64 Mi floats: the fill writes 256 MiB, the negation reads 256 MiB and
+        # writes 256 MiB.
+        a = torch.ones(64 << 20, device="cuda")
+        b = torch.empty_like(a)
+        torch.neg(a, out=b)
+        torch.cuda.synchronize()
+        del a, b
 
     device = pw.Device()
     # An explicit stream: the library launches on it, and the latency events

@@ -178,6 +178,12 @@ pyrowave_hip_device_query_stage_times(pyrowave_device device, double ms[7], uint
 PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_timing(pyrowave_device device, bool enable);
 
+/* Event timing for a subset of stages: bit s of stage_mask enables stage s
+ * (order as in pyrowave_hip_device_query_stage_times).  Two event records per
+ * timed stage and call. */
+PYROWAVE_PUBLIC_API void
+pyrowave_hip_device_set_timing_stages(pyrowave_device device, uint32_t stage_mask);
+
 #ifdef __cplusplus
 }
 #endif

@@ -96,6 +96,7 @@ def lib():
         L.or_decoder_offsets.restype = P
         L.or_decoder_offsets.argtypes = [P]
         L.or_decoder_decoded_blocks.argtypes = [P]
+        L.or_decoder_total_blocks.argtypes = [P]
         L.or_dequant.argtypes = [C.POINTER(Layout), P, C.c_long, P, P]
         L.or_idwt_frame.argtypes = [C.POINTER(Layout), P, C.c_int, P, P]
         L.or_idwt_frame_f64.argtypes = [C.POINTER(Layout), P, P, P]
@@ -252,6 +253,9 @@ class Decoder:
 
     def decoded_blocks(self):
         return lib().or_decoder_decoded_blocks(self.h)
+
+    def total_blocks(self):
+        return lib().or_decoder_total_blocks(self.h)
 
     def payload(self):
         n = lib().or_decoder_payload_words(self.h)

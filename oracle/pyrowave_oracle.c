@@ -1488,6 +1488,8 @@ void or_decoder_clear(or_decoder *d)
 	d->payload_size = 0;
 }
 
+/* pyrowave_decoder.cpp:126-150.  The block is counted and its offset set
+ * before the length check, as in the reference. */
 static int decode_packet(or_decoder *d, const uint32_t *words)
 {
 	uint32_t block = words[1] >> 8;
@@ -1548,10 +1550,6 @@ int or_decoder_push_packet(or_decoder *d, const void *data_, size_t size)
 		size_t packet_size = (size_t)((w[0] >> 16) & 0xfff) * 4;
 		if (packet_size > size)
 			return 0;
-		/* A block header must cover at least its own 8 bytes; the reference
-		 * would spin forever on a duplicate zero-length block. */
-		if (packet_size < 8)
-			return 0;
 		int restart;
 		if (d->last_seq == UINT32_MAX)
 			restart = 1;
@@ -1569,8 +1567,13 @@ int or_decoder_push_packet(or_decoder *d, const void *data_, size_t size)
 		}
 		if ((w[1] >> 8) >= (uint32_t)d->L->count32)
 			return 0;
-		uint32_t *tmp = (uint32_t *)malloc(packet_size);
-		memcpy(tmp, data, packet_size);
+		/* A zero-length block already present: the reference's loop would
+		 * never advance (pyrowave_decoder.cpp:236-246); reported as a parse
+		 * error instead. */
+		if (packet_size == 0 && d->offsets[w[1] >> 8] != UINT32_MAX)
+			return 0;
+		uint32_t *tmp = (uint32_t *)malloc(packet_size < 8 ? 8 : packet_size);
+		memcpy(tmp, data, packet_size < 8 ? 8 : packet_size); /* size >= 8 here */
 		int ok = decode_packet(d, tmp);
 		free(tmp);
 		if (!ok)
@@ -1598,6 +1601,7 @@ long or_decoder_payload_words(const or_decoder *d) { return d->payload_size; }
 const uint32_t *or_decoder_payload(const or_decoder *d) { return d->payload; }
 const uint32_t *or_decoder_offsets(const or_decoder *d) { return d->offsets; }
 int or_decoder_decoded_blocks(const or_decoder *d) { return d->decoded_blocks; }
+int or_decoder_total_blocks(const or_decoder *d) { return d->total_blocks; }
 
 /* ------------------------------------------------------------------------ */
 /* Dequant: shaders/wavelet_dequant.comp                                    */

@@ -167,6 +167,7 @@ long or_decoder_payload_words(const or_decoder *d);
 const uint32_t *or_decoder_payload(const or_decoder *d);
 const uint32_t *or_decoder_offsets(const or_decoder *d);
 int or_decoder_decoded_blocks(const or_decoder *d);
+int or_decoder_total_blocks(const or_decoder *d);
 
 /* wavelet_dequant.comp: writes every transmitted band of `coeffs`. */
 int or_dequant(const or_layout *L, const uint32_t *payload, long payload_words,

@@ -128,7 +128,7 @@ EXPORTS = (
     "pyrowave_hip_coefficient_count", "pyrowave_hip_encoder_read_coefficients",
     "pyrowave_hip_encoder_encode_coefficients", "pyrowave_hip_encoder_read_rate_control",
     "pyrowave_hip_decoder_read_coefficients", "pyrowave_hip_device_query_stage_times",
-    "pyrowave_hip_device_set_timing",
+    "pyrowave_hip_device_set_timing", "pyrowave_hip_device_set_timing_stages",
 )
 
 
@@ -188,6 +188,7 @@ def load():
         "pyrowave_hip_decoder_read_coefficients": (I, [P, P, SZ]),
         "pyrowave_hip_device_query_stage_times": (I, [P, P, P, C.c_bool]),
         "pyrowave_hip_device_set_timing": (None, [P, C.c_bool]),
+        "pyrowave_hip_device_set_timing_stages": (None, [P, C.c_uint32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -252,6 +253,13 @@ class Device:
 
     def set_timing(self, enable):
         load().pyrowave_hip_device_set_timing(self.handle, enable)
+
+    def set_timing_stages(self, stages):
+        """Times only the named stages (names as in STAGES)."""
+        mask = 0
+        for s in stages:
+            mask |= 1 << STAGES.index(s)
+        load().pyrowave_hip_device_set_timing_stages(self.handle, mask)
 
 
 def _cpu_buffer(planes, fmt, width, height, strides=None, sizes=None):

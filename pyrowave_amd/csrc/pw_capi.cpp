@@ -420,7 +420,7 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_create_device_by_compat(uint32_t vi
 		return PYROWAVE_ERROR_GENERIC;
 	upload_constants();
 	if (const char *env = getenv("PYROWAVE_TIMING"))
-		d->dev.timing = strtol(env, nullptr, 0) != 0;
+		d->dev.timing = strtol(env, nullptr, 0) != 0 ? (1u << pw::kStageCount) - 1 : 0u;
 	*device = d.release();
 	return PYROWAVE_SUCCESS;
 }
@@ -1248,6 +1248,14 @@ PYROWAVE_PUBLIC_API void pyrowave_hip_device_set_timing(pyrowave_device device, 
 	if (!device)
 		return;
 	std::lock_guard<std::recursive_mutex> g(device->dev.lock);
-	device->dev.timing = enable;
+	device->dev.timing = enable ? (1u << pw::kStageCount) - 1 : 0u;
+}
+
+PYROWAVE_PUBLIC_API void pyrowave_hip_device_set_timing_stages(pyrowave_device device, uint32_t stage_mask)
+{
+	if (!device)
+		return;
+	std::lock_guard<std::recursive_mutex> g(device->dev.lock);
+	device->dev.timing = stage_mask & ((1u << pw::kStageCount) - 1);
 }
 } // extern "C"

@@ -235,10 +235,9 @@ hipEvent_t Device::get_event()
 
 void Device::begin_stage(hipStream_t s, int stage, int frames, hipEvent_t *start)
 {
-	(void)stage;
 	(void)frames;
 	*start = nullptr;
-	if (!timing)
+	if (!((timing >> stage) & 1u))
 		return;
 	*start = get_event();
 	(void)hipEventRecord(*start, s);
@@ -246,7 +245,7 @@ void Device::begin_stage(hipStream_t s, int stage, int frames, hipEvent_t *start
 
 void Device::end_stage(hipStream_t s, int stage, int frames, hipEvent_t start)
 {
-	if (!timing || !start)
+	if (!start)
 		return;
 	hipEvent_t end = get_event();
 	(void)hipEventRecord(end, s);
@@ -428,10 +427,6 @@ bool Slot::push_packet(const void *data_, size_t size)
 		size_t packet_size = size_t((w[0] >> 16) & 0xfffu) * 4;
 		if (packet_size > size)
 			return false;
-		// A block must cover its own 8-byte header; a zero-length duplicate
-		// would otherwise never advance the parse.
-		if (packet_size < 8)
-			return false;
 		bool restart;
 		if (last_seq == UINT32_MAX)
 			restart = true;
@@ -452,11 +447,21 @@ bool Slot::push_packet(const void *data_, size_t size)
 			return false;
 		if (offsets[block] == UINT32_MAX)
 		{
+			// Counted and placed before the length check
+			// (pyrowave_decoder.cpp:126-150).
 			decoded_blocks++;
 			offsets[block] = uint32_t(payload.size());
+			if (packet_size < 8)
+				return false;
 			size_t n = payload.size();
 			payload.resize(n + packet_size / 4);
 			memcpy(payload.data() + n, data, packet_size);
+		}
+		else if (packet_size == 0)
+		{
+			// The reference's loop would never advance past a zero-length
+			// duplicate (pyrowave_decoder.cpp:236-246); a parse error here.
+			return false;
 		}
 		data += packet_size;
 		size -= packet_size;

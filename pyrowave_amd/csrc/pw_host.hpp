@@ -88,7 +88,7 @@ struct Device
 	int hip_device = 0;
 	hipStream_t stream = nullptr;
 	std::recursive_mutex lock;
-	bool timing = true;
+	uint32_t timing = (1u << kStageCount) - 1; // stages with event timing
 	double stage_ms[kStageCount] = {};
 	uint64_t stage_frames[kStageCount] = {};
 	std::vector<TimingRecord> pending;
