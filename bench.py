@@ -28,6 +28,8 @@ W, H = 3840, 2160
 BUDGET = 1_600_000
 FRAMES = 60
 HBM_PEAK_GBS = 8000.0
+LEVELS = 5
+ROOF_STAGES = ("DWT", "iDWT")
 
 
 def parse():
@@ -43,6 +45,9 @@ def parse():
                          "(counter calibration for HBM traffic)")
     ap.add_argument("--trace-summary", metavar="CSV",
                     help="summarize a rocprofv3 kernel trace CSV per (kernel, workgroups) and exit")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="per-launch HBM traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE counter CSVs of a "
+                         "--calibrate-copy run, scaled by the copy kernel's reading, and exit")
     return ap.parse_args()
 
 
@@ -88,10 +93,50 @@ def trace_summary(path, top=40):
               f"{max(v) / 1e3:9.2f} {sum(v) / total:6.1%}")
 
 
+def _counter_rows(path):
+    import csv
+    import re
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            name = r["Kernel_Name"]
+            m = re.search(r"(k_[a-z_0-9]+)", name)
+            short = m.group(1) if m else ("copy_neg" if "neg" in name else name[:40])
+            grid = int(r["Grid_Size"]) if "Grid_Size" in r else int(r["Grid_Size_X"])
+            wg = int(r["Workgroup_Size"]) if "Workgroup_Size" in r else int(r["Workgroup_Size_X"])
+            yield short, grid // max(wg, 1), float(r["Counter_Value"])
+
+
+def pmc_summary(fetch_csv, write_csv):
+    """Average FETCH_SIZE / WRITE_SIZE (KiB per dispatch) per (kernel,
+    workgroups); the elementwise negation of --calibrate-copy reads and
+    writes exactly 256 MiB, which gives each counter's scale factor."""
+    import collections
+    out = {}
+    for label, path in (("FETCH_SIZE", fetch_csv), ("WRITE_SIZE", write_csv)):
+        agg = collections.defaultdict(list)
+        for name, wgs, v in _counter_rows(path):
+            agg[(name, wgs)].append(v)
+        copy = [sum(v) / len(v) for (n, _), v in agg.items() if n == "copy_neg"]
+        factor = (256 * 1024) / copy[0] if copy else 1.0
+        out[label] = (agg, factor)
+        print(f"{label}: copy kernel reads {copy[0] if copy else float('nan'):.0f} KiB for 262144 KiB "
+              f"-> scale factor {factor:.4f}")
+    fagg, ff = out["FETCH_SIZE"]
+    wagg, wf = out["WRITE_SIZE"]
+    print(f"{'kernel':40s} {'workgroups':>10s} {'calls':>6s} {'fetch_MB':>10s} {'write_MB':>10s} {'traffic_MB':>10s}")
+    for key in sorted(fagg, key=lambda k: -sum(fagg[k])):
+        fv = sum(fagg[key]) / len(fagg[key]) * ff * 1024
+        wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
+        print(f"{key[0]:40s} {key[1]:10d} {len(fagg[key]):6d} {fv / 1e6:10.3f} {wv / 1e6:10.3f} {(fv + wv) / 1e6:10.3f}")
+
+
 def main():
     args = parse()
     if args.trace_summary:
         trace_summary(args.trace_summary)
+        return
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -151,11 +196,16 @@ def main():
         device.set_timing(False)
         run_steps(encs, decs, tgts, warmup, 0)
         torch.cuda.synchronize()
+        # Timed region: HIP events on the launch stream around the forward and
+        # inverse transform stages only (four event records per step).
+        device.set_timing_stages(ROOF_STAGES)
+        device.stage_times(reset=True)
         lat = []
         t = time.perf_counter()
         run_steps(encs, decs, tgts, steps, warmup * B, lat)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
+        roof = device.stage_times(reset=True)
         lat_ms = np.array([a.elapsed_time(b) for a, b in lat])
         # Stage times (events on the launch stream) over a separate pass.
         device.set_timing(True)
@@ -164,7 +214,7 @@ def main():
         torch.cuda.synchronize()
         stages = device.stage_times(reset=True)
         device.set_timing(False)
-        return dict(encs=encs, decs=decs, outs=outs, wall=wall, lat_ms=lat_ms, stages=stages, B=B)
+        return dict(encs=encs, decs=decs, outs=outs, wall=wall, lat_ms=lat_ms, stages=stages, roof=roof, B=B)
 
     B = args.batch
     fp16 = timed(pw.PRECISION_FP16, B, args.steps, args.warmup)
@@ -196,12 +246,21 @@ def main():
                 max_bytes = max(max_bytes, sum(sz for _, sz in pk))
         psnr[name] = float(np.mean(vals[:4]))
 
-    # Roofline of the forward DWT (5 level launches per batch) from stage events.
-    st = fp16["stages"]
-    dwt_ms, dwt_frames = st["DWT"]
-    dwt_bytes = dwt_algorithmic_bytes(True)
-    dwt_us_per_frame = dwt_ms * 1e3 / max(dwt_frames, 1)
-    achieved = dwt_bytes / (dwt_us_per_frame * 1e-6) / 1e9
+    # Rooflines of the transform kernels from the events of the timed region.
+    # One batch runs LEVELS launches of each kernel; a launch's algorithmic
+    # bytes are the batch's compulsory bytes / LEVELS, its duration the
+    # stage time / LEVELS, so achieved = compulsory bytes / stage time.
+    def roofline(stage, kernel):
+        ms, frames = fp16["roof"][stage]
+        launches = args.steps * LEVELS
+        per_frame = dwt_algorithmic_bytes(True)
+        bytes_per_launch = per_frame * frames / max(launches, 1)
+        us_per_launch = ms * 1e3 / max(launches, 1)
+        achieved = bytes_per_launch / (us_per_launch * 1e-6) / 1e9
+        return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
+                "launches_timed": launches}
 
     cpu = None
     if not args.no_cpu:
@@ -248,10 +307,8 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 1)},
         "stage_us_per_frame_fp16": stage_us,
         "stage_us_per_frame_fp32_single": stage_us32,
-        "roofline": {"kernel": "k_dwt_forward (5 level launches)", "bound": "hbm",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_frame": dwt_bytes},
+        "roofline": roofline("iDWT", "k_dwt_inverse"),
+        "roofline_forward": roofline("DWT", "k_dwt_forward"),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -308,11 +308,12 @@ struct BlockCoord
 };
 
 __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const uint32_t *blk_band,
-                                                   const int32_t *frame_block_base)
+                                                   const int32_t *frame_block_base, int nxcd)
 {
 	BlockCoord C;
-	C.J = bands[blk_band[blockIdx.x]];
-	C.b = (int)blockIdx.x - frame_block_base[C.J.frame];
+	const int g = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
+	C.J = bands[blk_band[g]];
+	C.b = g - frame_block_base[C.J.frame];
 	int r = C.b - C.J.off32;
 	C.by = r / C.J.bx32;
 	C.bx = r - C.by * C.J.bx32;
@@ -322,7 +323,7 @@ __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const u
 __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
                                                                  const uint32_t *__restrict__ blk_band,
                                                                  const FrameState *__restrict__ frames,
-                                                                 const int32_t *__restrict__ frame_block_base)
+                                                                 const int32_t *__restrict__ frame_block_base, int nxcd)
 {
 	__shared__ int s_cost[16][16];
 	__shared__ float s_dist[16][16];
@@ -332,7 +333,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 	__shared__ float s_D[16];
 	__shared__ uint32_t s_v[16];
 
-	const BlockCoord C = locate_block(bands, blk_band, frame_block_base);
+	const BlockCoord C = locate_block(bands, blk_band, frame_block_base, nxcd);
 	const BandJob &J = C.J;
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
 	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
@@ -718,7 +719,7 @@ __device__ __forceinline__ void lds_store_byte(uint32_t *buf, uint32_t byte_pos,
 __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restrict__ bands,
                                                         const uint32_t *__restrict__ blk_band,
                                                         FrameState *__restrict__ frames,
-                                                        const int32_t *__restrict__ frame_block_base)
+                                                        const int32_t *__restrict__ frame_block_base, int nxcd)
 {
 	__shared__ uint32_t s_buf[kMaxBlockWords];
 	__shared__ int s_tmp[2];
@@ -726,7 +727,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	__shared__ uint32_t s_bits;
 	__shared__ uint32_t s_group_base;
 
-	const BlockCoord C = locate_block(bands, blk_band, frame_block_base);
+	const BlockCoord C = locate_block(bands, blk_band, frame_block_base, nxcd);
 	const BandJob &J = C.J;
 	FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
@@ -877,12 +878,24 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	}
 }
 
-__device__ __forceinline__ uint32_t lds_byte(const uint32_t *buf, uint32_t nbytes, uint32_t pos)
+// Byte `pos` of a coded block: from the LDS copy of its first `nbytes`
+// bytes, else (only a malformed block reads past its own length) from the
+// payload itself, and zero past the payload's end.
+struct BlockBytes
 {
-	if (pos >= nbytes)
+	const uint32_t *lds;
+	const uint32_t *global;
+	uint32_t nbytes, avail;
+
+	__device__ __forceinline__ uint32_t operator()(uint32_t pos) const
+	{
+		if (pos < nbytes)
+			return (lds[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
+		if (pos < avail)
+			return (global[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
 		return 0;
-	return (buf[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
-}
+	}
+};
 
 // reference shaders/wavelet_dequant.comp:100-160 and 220-391.  The coded
 // block (at most kMaxBlockWords words for any block an encoder can produce) is
@@ -891,7 +904,7 @@ __device__ __forceinline__ uint32_t lds_byte(const uint32_t *buf, uint32_t nbyte
 __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
                                                            const uint32_t *__restrict__ blk_band,
                                                            const FrameState *__restrict__ frames,
-                                                           const int32_t *__restrict__ frame_block_base)
+                                                           const int32_t *__restrict__ frame_block_base, int nxcd)
 {
 	__shared__ uint32_t s_pay[kMaxBlockWords];
 	__shared__ uint32_t s_poff[16];
@@ -900,7 +913,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	__shared__ uint32_t s_sign_base;
 	__shared__ int s_tmp[2];
 
-	const BlockCoord C = locate_block(bands, blk_band, frame_block_base);
+	const BlockCoord C = locate_block(bands, blk_band, frame_block_base, nxcd);
 	const BandJob &J = C.J;
 	const FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
@@ -926,9 +939,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		for (uint32_t i = tid; i < nwords; i += kBlockThreads)
 			s_pay[i] = payload[i];
 		__syncthreads();
-		const uint32_t nbytes = nwords * 4;
-		const uint32_t ballot = nbytes >= 4 ? (s_pay[0] & 0xffffu) : 0u;
-		const uint32_t q_code = lds_byte(s_pay, nbytes, 4);
+		const BlockBytes byte_at{s_pay, payload, nwords * 4, (payload_words - off) * 4};
+		const uint32_t ballot = w0 & 0xffffu;
+		const uint32_t q_code = byte_at(4);
 		const uint32_t N = __popc(ballot);
 		if (tid < 16)
 		{
@@ -936,8 +949,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			if ((ballot >> tid) & 1u)
 			{
 				uint32_t j = __popc(ballot & ((1u << tid) - 1u));
-				ctrl = lds_byte(s_pay, nbytes, 8 + 2 * j) | (lds_byte(s_pay, nbytes, 9 + 2 * j) << 8);
-				qb = lds_byte(s_pay, nbytes, 8 + 2 * N + j);
+				ctrl = byte_at(8 + 2 * j) | (byte_at(9 + 2 * j) << 8);
+				qb = byte_at(8 + 2 * N + j);
 			}
 			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
 			msbs |= msbs >> 1;
@@ -972,7 +985,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			int mag[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 			for (int pl = planes - 1; pl >= 0; pl--)
 			{
-				uint32_t byte = lds_byte(s_pay, nbytes, byte_off++);
+				uint32_t byte = byte_at(byte_off++);
 #pragma unroll
 				for (int p = 0; p < 8; p++)
 					mag[p] |= (int)((byte >> p) & 1u) << pl;
@@ -1004,7 +1017,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		{
 			if (v[p] != 0.0f)
 			{
-				uint32_t byte = lds_byte(s_pay, nbytes, sbit >> 3);
+				uint32_t byte = byte_at(sbit >> 3);
 				if ((byte >> (sbit & 7)) & 1u)
 					v[p] = -v[p];
 				sbit++;
@@ -1042,7 +1055,7 @@ void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nb
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_quant_analyze, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   frame_block_base);
+		                   frame_block_base, xcd_count());
 }
 
 void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream)
@@ -1068,7 +1081,7 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   frame_block_base);
+		                   frame_block_base, xcd_count());
 }
 
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
@@ -1076,6 +1089,6 @@ void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks,
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   frame_block_base);
+		                   frame_block_base, xcd_count());
 }
 } // namespace pw

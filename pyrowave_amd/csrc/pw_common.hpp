@@ -168,6 +168,23 @@ struct BandJob
 	int32_t pad;
 };
 
+// Workgroups are dispatched round-robin over the XCDs (workgroup i runs on
+// XCD i % nxcd), and each XCD has its own L2.  This maps the dispatch order
+// to a contiguous range of work items per XCD, so that neighbouring tiles and
+// blocks, which share cache lines and filter aprons, meet in one L2.  A
+// bijection on [0, n) for any n.
+__host__ __device__ inline int xcd_swizzle(int b, int n, int nxcd)
+{
+	if (nxcd <= 1 || n < nxcd)
+		return b;
+	const int q = n / nxcd, r = n % nxcd;
+	const int x = b % nxcd, i = b / nxcd;
+	return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Number of XCDs of the current device (1 where the runtime cannot tell).
+int xcd_count();
+
 // Launch wrappers (defined in the .hip files).
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         bool fp16, hipStream_t stream);

@@ -125,7 +125,7 @@ __device__ __forceinline__ void lift_inv16(float (&v)[16])
 // done by the first wave only.  Every load is issued before any is consumed.
 template <typename T>
 __device__ __forceinline__ void load_tile(const T *src, long pitch, long step, const int *ymap, const int *xmap,
-                                          int tid, float (&vals)[kLoadIters])
+                                          const float *lut, int tid, float (&vals)[kLoadIters])
 {
 #pragma unroll
 	for (int k = 0; k < kLoadIters; k++)
@@ -137,7 +137,7 @@ __device__ __forceinline__ void load_tile(const T *src, long pitch, long step, c
 			const int r = i / kTileIn, c = i - r * kTileIn;
 			const T v = src[(long)ymap[r] * pitch + (long)xmap[c] * step];
 			if constexpr (sizeof(T) == 1)
-				vals[k] = c_u8_dc[v];
+				vals[k] = lut[v];
 			else if constexpr (sizeof(T) == 2)
 				vals[k] = __half2float(v);
 			else
@@ -167,15 +167,17 @@ constexpr int kPass1Rounds = (kPass1Tasks + kDwtThreads - 1) / kDwtThreads;
 
 __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__restrict__ jobs,
                                                              const uint16_t *__restrict__ tile_job,
-                                                             const FrameState *__restrict__ frames)
+                                                             const FrameState *__restrict__ frames, int nxcd)
 {
 	__shared__ float sA[kTileIn * kPitchA];
 	__shared__ int s_xmap[kTileIn];
 	__shared__ int s_ymap[kTileIn];
+	__shared__ float s_lut[256];
 
 	const int tid = threadIdx.x;
-	const DwtJob J = jobs[tile_job[blockIdx.x]];
-	const int local = blockIdx.x - J.tile_base;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
+	const DwtJob J = jobs[tile_job[tile]];
+	const int local = tile - J.tile_base;
 	const int x0 = (local % J.tiles_x) * kTile;
 	const int y0 = (local / J.tiles_x) * kTile;
 
@@ -183,18 +185,20 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 		s_xmap[tid] = fwd_ext(x0 - kApron + tid, J.R_x, J.A_x);
 	else if (tid < 2 * kTileIn)
 		s_ymap[tid - kTileIn] = fwd_ext(y0 - kApron + tid - kTileIn, J.R_y, J.A_y);
+	s_lut[tid] = c_u8_dc[tid];
 	__syncthreads();
 
 	float vals[kLoadIters];
 	if (J.src_kind == kStoreU8)
 	{
 		const FrameState &F = frames[J.frame];
-		load_tile(F.src[J.component], F.src_pitch[J.component], F.src_step[J.component], s_ymap, s_xmap, tid, vals);
+		load_tile(F.src[J.component], F.src_pitch[J.component], F.src_step[J.component], s_ymap, s_xmap, s_lut, tid,
+		          vals);
 	}
 	else if (J.src_kind == kStoreF16)
-		load_tile(static_cast<const __half *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, tid, vals);
+		load_tile(static_cast<const __half *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, s_lut, tid, vals);
 	else
-		load_tile(static_cast<const float *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, tid, vals);
+		load_tile(static_cast<const float *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, s_lut, tid, vals);
 	store_tile(sA, tid, vals);
 	__syncthreads();
 
@@ -281,29 +285,44 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
-// Loads one band's 36x36 share of the interleaved 72x72 tile: positions
-// whose parity (row, column) is (b >> 1, b & 1).
-template <typename T>
-__device__ __forceinline__ void load_band_quarter(const T *band, long pitch, int b, const int *ymap,
-                                                  const int *xmap, int tid, float *sA)
+// Loads the four bands' 36x36 shares of the interleaved 72x72 tile (band b
+// holds the positions whose (row, column) parity is (b >> 1, b & 1)) into
+// LDS; every global load is issued before the first LDS store.
+template <typename TH>
+__device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, const int *xmap, int tid, float *sA)
 {
 	constexpr int kQ = kTileIn / 2;
 	constexpr int kIters = (kQ * kQ + kDwtThreads - 1) / kDwtThreads;
-	float vals[kIters];
+	const long pitch = J.band_pitch;
+	const float *ll = static_cast<const float *>(J.band[0]);
+	const TH *hb[3] = {static_cast<const TH *>(J.band[1]), static_cast<const TH *>(J.band[2]),
+	                   static_cast<const TH *>(J.band[3])};
+	float vals[4][kIters];
 #pragma unroll
 	for (int k = 0; k < kIters; k++)
 	{
 		const int i = tid + k * kDwtThreads;
-		vals[k] = 0.0f;
+#pragma unroll
+		for (int b = 0; b < 4; b++)
+			vals[b][k] = 0.0f;
 		if (i < kQ * kQ)
 		{
 			const int rr = i / kQ, cc = i - rr * kQ;
-			const int r = 2 * rr + (b >> 1), c = 2 * cc + (b & 1);
-			const T v = band[(long)(ymap[r] >> 1) * pitch + (xmap[c] >> 1)];
-			if constexpr (sizeof(T) == 2)
-				vals[k] = __half2float(v);
+			const long row0 = (long)(ymap[2 * rr] >> 1) * pitch, row1 = (long)(ymap[2 * rr + 1] >> 1) * pitch;
+			const int col0 = xmap[2 * cc] >> 1, col1 = xmap[2 * cc + 1] >> 1;
+			vals[0][k] = ll[row0 + col0];
+			if constexpr (sizeof(TH) == 2)
+			{
+				vals[1][k] = __half2float(hb[0][row0 + col1]);
+				vals[2][k] = __half2float(hb[1][row1 + col0]);
+				vals[3][k] = __half2float(hb[2][row1 + col1]);
+			}
 			else
-				vals[k] = v;
+			{
+				vals[1][k] = hb[0][row0 + col1];
+				vals[2][k] = hb[1][row1 + col0];
+				vals[3][k] = hb[2][row1 + col1];
+			}
 		}
 	}
 #pragma unroll
@@ -313,22 +332,25 @@ __device__ __forceinline__ void load_band_quarter(const T *band, long pitch, int
 		if (i < kQ * kQ)
 		{
 			const int rr = i / kQ, cc = i - rr * kQ;
-			sA[(2 * rr + (b >> 1)) * kPitchA + 2 * cc + (b & 1)] = vals[k];
+#pragma unroll
+			for (int b = 0; b < 4; b++)
+				sA[(2 * rr + (b >> 1)) * kPitchA + 2 * cc + (b & 1)] = vals[b][k];
 		}
 	}
 }
 
 __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
                                                              const uint16_t *__restrict__ tile_job,
-                                                             const FrameState *__restrict__ frames)
+                                                             const FrameState *__restrict__ frames, int nxcd)
 {
 	__shared__ float sA[kTileIn * kPitchA];
 	__shared__ int s_xmap[kTileIn];
 	__shared__ int s_ymap[kTileIn];
 
 	const int tid = threadIdx.x;
-	const DwtJob J = jobs[tile_job[blockIdx.x]];
-	const int local = blockIdx.x - J.tile_base;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
+	const DwtJob J = jobs[tile_job[tile]];
+	const int local = tile - J.tile_base;
 	const int ox0 = (local % J.tiles_x) * kTile;
 	const int oy0 = (local / J.tiles_x) * kTile;
 
@@ -340,17 +362,10 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 
 	// Interleaved position parity selects the band: LL (even, even), HL (x
 	// odd), LH (y odd), HH.  The extension preserves parity.
-	load_band_quarter(static_cast<const float *>(J.band[0]), J.band_pitch, 0, s_ymap, s_xmap, tid, sA);
 	if (J.band_kind == kStoreF16)
-	{
-		for (int b = 1; b < 4; b++)
-			load_band_quarter(static_cast<const __half *>(J.band[b]), J.band_pitch, b, s_ymap, s_xmap, tid, sA);
-	}
+		load_bands<__half>(J, s_ymap, s_xmap, tid, sA);
 	else
-	{
-		for (int b = 1; b < 4; b++)
-			load_band_quarter(static_cast<const float *>(J.band[b]), J.band_pitch, b, s_ymap, s_xmap, tid, sA);
-	}
+		load_bands<float>(J, s_ymap, s_xmap, tid, sA);
 	__syncthreads();
 
 	// Vertical pass over all 72 columns; rows of the tile are windows 4..11.
@@ -410,12 +425,27 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 		{
 			if (py < J.R_y)
 			{
-#pragma unroll
-				for (int j = 4; j < 12; j++)
+				const int px0 = ox0 + seg * 8;
+				uint8_t *dst = plane + (long)py * dpitch + px0;
+				if (px0 + 8 <= J.R_x && ((uintptr_t)dst & 7) == 0)
 				{
-					const int px = ox0 + seg * 8 + j - 4;
-					if (px < J.R_x)
-						plane[(long)py * dpitch + px] = unorm8_store(v[j]);
+					uint32_t lo = 0, hi = 0;
+#pragma unroll
+					for (int j = 0; j < 4; j++)
+					{
+						lo |= (uint32_t)unorm8_store(v[4 + j]) << (8 * j);
+						hi |= (uint32_t)unorm8_store(v[8 + j]) << (8 * j);
+					}
+					*reinterpret_cast<uint2 *>(dst) = make_uint2(lo, hi);
+				}
+				else
+				{
+#pragma unroll
+					for (int j = 4; j < 12; j++)
+					{
+						if (px0 + j - 4 < J.R_x)
+							dst[j - 4] = unorm8_store(v[j]);
+					}
 				}
 			}
 		}
@@ -438,7 +468,8 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	(void)fp16;
 	if (tiles > 0)
-		hipLaunchKernelGGL(k_dwt_forward, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames);
+		hipLaunchKernelGGL(k_dwt_forward, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames,
+		                   xcd_count());
 }
 
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -446,7 +477,24 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	(void)fp16;
 	if (tiles > 0)
-		hipLaunchKernelGGL(k_dwt_inverse, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames);
+		hipLaunchKernelGGL(k_dwt_inverse, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames,
+		                   xcd_count());
+}
+
+int xcd_count()
+{
+	static int cache[64] = {};
+	int dev = 0;
+	if (hipGetDevice(&dev) != hipSuccess || dev < 0)
+		return 1;
+	if (dev < 64 && cache[dev])
+		return cache[dev];
+	int n = 1;
+	if (hipDeviceGetAttribute(&n, hipDeviceAttributeNumberOfXccs, dev) != hipSuccess || n < 1)
+		n = 1;
+	if (dev < 64)
+		cache[dev] = n;
+	return n;
 }
 
 void upload_constants()
