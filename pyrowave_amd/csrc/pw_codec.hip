@@ -312,7 +312,7 @@ __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const u
 {
 	BlockCoord C;
 	const int g = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
-	C.J = bands[blk_band[g]];
+	C.J = bands[__builtin_amdgcn_readfirstlane(blk_band[g])];
 	C.b = g - frame_block_base[C.J.frame];
 	int r = C.b - C.J.off32;
 	C.by = r / C.J.bx32;

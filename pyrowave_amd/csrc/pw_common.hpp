@@ -20,11 +20,16 @@ constexpr int kSubdiv = 16;
 constexpr int kBuckets = 128;
 constexpr int kFlat = kBuckets * kSubdiv;
 
-// Forward / inverse DWT tile: 64x64 samples of the level domain, 4-sample
-// apron, so 72x72 staged in LDS.  One 256-thread workgroup per tile.
-constexpr int kTile = 64;
+// DWT tiles, one 256-thread workgroup each, with a 4-sample apron on every
+// side.  Forward: 128 x 56 samples of the level domain (136 x 64 staged), so
+// the horizontal pass is 64 rows x 4 windows of 40 and the vertical pass 128
+// columns x 2 windows of 36.  Inverse: 64 x 64 output samples (72 x 72
+// staged) in 16-sample windows.
 constexpr int kApron = 4;
-constexpr int kTileIn = kTile + 2 * kApron;
+constexpr int kFwdTileW = 128;
+constexpr int kFwdTileH = 56;
+constexpr int kInvTileW = 64;
+constexpr int kInvTileH = 64;
 constexpr int kDwtThreads = 256;
 
 // Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
@@ -186,8 +191,9 @@ __host__ __device__ inline int xcd_swizzle(int b, int n, int nxcd)
 int xcd_count();
 
 // Launch wrappers (defined in the .hip files).
+// u8_only: every job of the launch reads a u8 plane (smaller prefetch state).
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                        bool fp16, hipStream_t stream);
+                        bool u8_only, hipStream_t stream);
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         bool fp16, hipStream_t stream);
 void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream);

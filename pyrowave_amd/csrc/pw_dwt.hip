@@ -1,13 +1,22 @@
 // pw_dwt.hip -- forward and inverse CDF 9/7 DWT levels on gfx950.
 //
-// One 256-thread workgroup transforms a 64x64 tile of a level's domain.  The
-// tile plus a 4-sample apron (72x72) is staged in LDS once; the first 1-D pass
-// runs over LDS rows (forward: horizontal; inverse: vertical columns), writes a
-// second LDS buffer, and the second pass writes coalesced rows of the output
-// bands straight to HBM.  Every lifting step is fmaf(c, l + r, v), the order
-// of the reference shaders (reference shaders/dwt.comp:108-115,
-// shaders/idwt.comp:95-108), so a tile reproduces the whole-line transform
-// bit for bit.
+// One 256-thread workgroup transforms one tile of a level; the tile and its
+// 4-sample apron are staged in LDS, the two 1-D passes run over LDS rows and
+// columns, and every lifting step is fmaf(c, l + r, v), the order of the
+// reference shaders (reference shaders/dwt.comp:108-115,
+// shaders/idwt.comp:95-108).  A window of n samples yields exact outputs at
+// positions 4 .. n-5, so tiles reproduce the whole-line transform bit for bit.
+//
+// Forward: 128 x 56 domain samples per tile (136 x 64 staged).  Coalesced
+// 4-sample vector loads, then each thread lifts one long window in registers
+// (horizontal: 64 rows x 4 windows of 40 -> 32 outputs; vertical: 128 columns
+// x 2 windows of 36 -> 28 outputs), so window overlap costs 25-29 % extra
+// arithmetic; band stores are coalesced rows.
+//
+// Inverse: 64 x 64 output samples per tile (72 x 72 staged), 16-sample
+// windows.  The small tile keeps LDS (21.6 KB) and registers low, for seven
+// workgroups per CU: the inverse stages four bands per tile and needs the
+// extra waves in flight to cover its load latency.
 #include "pw_common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -26,8 +35,25 @@ constexpr float kDelta = float(0.443506852043971);
 constexpr float kK = float(1.230174104914001);
 constexpr float kInvK = float(1.0 / 1.230174104914001);
 
-constexpr int kLoadIters = (kTileIn * kTileIn + kDwtThreads - 1) / kDwtThreads;
-constexpr int kPitchA = kTileIn + 1; // odd pitch: row-per-lane reads are conflict-free
+// Forward tile.
+constexpr int kFwdInW = kFwdTileW + 2 * kApron;               // 136
+constexpr int kFwdInH = kFwdTileH + 2 * kApron;               // 64
+constexpr int kFwdPitch = kFwdInW + 1;                        // odd: row-per-lane reads are conflict-free
+constexpr int kFwdGroups = kFwdInW / 4;                       // 34 groups of 4 samples per row
+constexpr int kFwdSlots = kFwdInH * kFwdGroups;               // 2176
+constexpr int kFwdSlotIters = (kFwdSlots + kDwtThreads - 1) / kDwtThreads;
+constexpr int kFwdHWin = 40, kFwdHOut = 32;                   // horizontal window / outputs
+constexpr int kFwdVWin = 36, kFwdVOut = 28;                   // vertical window / outputs
+static_assert(kFwdInH * (kFwdTileW / kFwdHOut) == kDwtThreads, "one horizontal window per thread");
+static_assert(kFwdTileW * (kFwdTileH / kFwdVOut) == kDwtThreads, "one vertical window per thread");
+
+// Inverse tile.
+static_assert(kInvTileW == kInvTileH, "square inverse tiles");
+constexpr int kInvTile = kInvTileW;                           // 64
+constexpr int kInvIn = kInvTile + 2 * kApron;                 // 72
+constexpr int kInvPitch = kInvIn + 1;                         // 73
+constexpr int kInvPass1Tasks = kInvIn * 8;                    // 72 lines x 8 windows of 16
+constexpr int kInvPass1Rounds = (kInvPass1Tasks + kDwtThreads - 1) / kDwtThreads;
 
 __constant__ float c_u8_dc[256]; // u8 / 255 - 0.5 (UNORM8 texel with the level-0 DC shift)
 
@@ -44,7 +70,7 @@ __device__ __forceinline__ int mirror_repeat(int t, int R)
 
 // Encoder extension: clamp-to-edge from R to A folded with whole-sample
 // mirroring about 0 and A-1 (reference shaders/dwt.comp:27-36 with a 2x2
-// gather footprint and a MIRRORED_REPEAT sampler).
+// gather footprint and a MIRRORED_REPEAT sampler).  The identity on [0, R).
 __device__ __forceinline__ int fwd_ext(int p, int R, int A)
 {
 	int c = p & ~1;
@@ -57,7 +83,7 @@ __device__ __forceinline__ int fwd_ext(int p, int R, int A)
 }
 
 // Decoder extension: whole-sample symmetric about 0 and N-1 of the
-// interleaved signal (reference shaders/idwt.comp:25-32).
+// interleaved signal (reference shaders/idwt.comp:25-32).  Preserves parity.
 __device__ __forceinline__ int inv_ext(int p, int N)
 {
 	while (p < 0 || p >= N)
@@ -70,13 +96,6 @@ __device__ __forceinline__ int inv_ext(int p, int N)
 	return p;
 }
 
-__device__ __forceinline__ float load_kind(const void *base, int kind, long idx)
-{
-	if (kind == kStoreF16)
-		return __half2float(static_cast<const __half *>(base)[idx]);
-	return static_cast<const float *>(base)[idx];
-}
-
 __device__ __forceinline__ void store_kind(void *base, int kind, long idx, float v)
 {
 	if (kind == kStoreF16)
@@ -85,25 +104,27 @@ __device__ __forceinline__ void store_kind(void *base, int kind, long idx, float
 		static_cast<float *>(base)[idx] = v;
 }
 
-// Forward lifting on a 16-sample window whose first sample is at an even
-// position; outputs are windows 4..11 (low at even, high at odd).
-__device__ __forceinline__ void lift_fwd16(float (&v)[16])
+// Forward lifting on an N-sample window starting at an even position;
+// outputs at window positions 4 .. N-5 (low at even, high at odd).
+template <int N>
+__device__ __forceinline__ void lift_fwd(float (&v)[N])
 {
 #pragma unroll
-	for (int i = 1; i <= 13; i += 2)
+	for (int i = 1; i <= N - 3; i += 2)
 		v[i] = fmaf(kAlpha, v[i - 1] + v[i + 1], v[i]);
 #pragma unroll
-	for (int i = 2; i <= 12; i += 2)
+	for (int i = 2; i <= N - 4; i += 2)
 		v[i] = fmaf(kBeta, v[i - 1] + v[i + 1], v[i]);
 #pragma unroll
-	for (int i = 3; i <= 11; i += 2)
+	for (int i = 3; i <= N - 5; i += 2)
 		v[i] = fmaf(kGamma, v[i - 1] + v[i + 1], v[i]);
 #pragma unroll
-	for (int i = 4; i <= 10; i += 2)
+	for (int i = 4; i <= N - 6; i += 2)
 		v[i] = fmaf(kDelta, v[i - 1] + v[i + 1], v[i]);
 }
 
-// Inverse lifting on a 16-sample window (already scaled by K / 1/K).
+// Inverse lifting on a 16-sample window starting at an even position, already
+// scaled by K (even) / 1/K (odd); outputs at window positions 4 .. 11.
 __device__ __forceinline__ void lift_inv16(float (&v)[16])
 {
 #pragma unroll
@@ -120,150 +141,220 @@ __device__ __forceinline__ void lift_inv16(float (&v)[16])
 		v[i] = fmaf(-kAlpha, v[i - 1] + v[i + 1], v[i]);
 }
 
-// Loads the 72x72 source tile (rows / columns through the extension maps)
-// into registers: element i = tid + 256 * k; the last round (k = 20) is
-// done by the first wave only.  Every load is issued before any is consumed.
+// --- forward ----------------------------------------------------------------
+
+// Four consecutive source samples as raw storage: u8 -> one dword, f16 ->
+// two dwords, f32 -> four.
 template <typename T>
-__device__ __forceinline__ void load_tile(const T *src, long pitch, long step, const int *ymap, const int *xmap,
-                                          const float *lut, int tid, float (&vals)[kLoadIters])
+struct Raw4;
+template <>
+struct Raw4<uint8_t>
 {
-#pragma unroll
-	for (int k = 0; k < kLoadIters; k++)
+	uint32_t w;
+};
+template <>
+struct Raw4<__half>
+{
+	uint2 w;
+};
+template <>
+struct Raw4<float>
+{
+	float4 w;
+};
+
+template <typename T>
+__device__ __forceinline__ Raw4<T> load4_direct(const T *p)
+{
+	Raw4<T> r;
+	if constexpr (sizeof(T) == 1)
+		r.w = *reinterpret_cast<const uint32_t *>(p);
+	else if constexpr (sizeof(T) == 2)
+		r.w = *reinterpret_cast<const uint2 *>(p);
+	else
+		r.w = *reinterpret_cast<const float4 *>(p);
+	return r;
+}
+
+template <typename T>
+__device__ __forceinline__ Raw4<T> load4_gather(const T *row, const int *xmap, int c, uint32_t step)
+{
+	Raw4<T> r;
+	if constexpr (sizeof(T) == 1)
 	{
-		const int i = tid + k * kDwtThreads;
-		vals[k] = 0.0f;
-		if (k < kLoadIters - 1 || tid < kTileIn * kTileIn - (kLoadIters - 1) * kDwtThreads)
+		r.w = 0;
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			r.w |= (uint32_t)row[__umul24((uint32_t)xmap[c + i], step)] << (8 * i);
+	}
+	else if constexpr (sizeof(T) == 2)
+	{
+		__half h[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			h[i] = row[xmap[c + i]];
+		r.w.x = (uint32_t)__half_as_ushort(h[0]) | ((uint32_t)__half_as_ushort(h[1]) << 16);
+		r.w.y = (uint32_t)__half_as_ushort(h[2]) | ((uint32_t)__half_as_ushort(h[3]) << 16);
+	}
+	else
+	{
+		r.w.x = row[xmap[c + 0]];
+		r.w.y = row[xmap[c + 1]];
+		r.w.z = row[xmap[c + 2]];
+		r.w.w = row[xmap[c + 3]];
+	}
+	return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void unpack4(const Raw4<T> &r, const float *lut, float (&f)[4])
+{
+	if constexpr (sizeof(T) == 1)
+	{
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			f[i] = lut[(r.w >> (8 * i)) & 0xffu];
+	}
+	else if constexpr (sizeof(T) == 2)
+	{
+		f[0] = __half2float(__ushort_as_half((unsigned short)(r.w.x & 0xffffu)));
+		f[1] = __half2float(__ushort_as_half((unsigned short)(r.w.x >> 16)));
+		f[2] = __half2float(__ushort_as_half((unsigned short)(r.w.y & 0xffffu)));
+		f[3] = __half2float(__ushort_as_half((unsigned short)(r.w.y >> 16)));
+	}
+	else
+	{
+		f[0] = r.w.x;
+		f[1] = r.w.y;
+		f[2] = r.w.z;
+		f[3] = r.w.w;
+	}
+}
+
+// Stages the 136 x 64 source window in LDS.  Slot s covers row s / 34,
+// samples 4 * (s % 34) .. +3.  A group whose four samples lie inside the
+// valid source needs no extension and is one vector load; others gather
+// through the extension maps.  Offsets are 32-bit element offsets from the
+// uniform base (planes and bands stay below 4 GiB).  All loads are issued
+// before the first LDS store.
+template <typename T>
+__device__ __forceinline__ void fwd_stage(const T *src, uint32_t pitch, uint32_t step, bool vec_ok, int x0, int R_x,
+                                          const int *ymap, const int *xmap, const float *lut, int tid, float *sA)
+{
+	Raw4<T> raw[kFwdSlotIters];
+#pragma unroll
+	for (int it = 0; it < kFwdSlotIters; it++)
+	{
+		const int slot = tid + it * kDwtThreads;
+		if (it < kFwdSlotIters - 1 || slot < kFwdSlots)
 		{
-			const int r = i / kTileIn, c = i - r * kTileIn;
-			const T v = src[(long)ymap[r] * pitch + (long)xmap[c] * step];
-			if constexpr (sizeof(T) == 1)
-				vals[k] = lut[v];
-			else if constexpr (sizeof(T) == 2)
-				vals[k] = __half2float(v);
+			const int r = slot / kFwdGroups, k = slot - r * kFwdGroups;
+			const int gx = x0 - kApron + 4 * k;
+			const uint32_t roff = __umul24((uint32_t)ymap[r], pitch);
+			if (vec_ok && gx >= 0 && gx + 3 < R_x)
+				raw[it] = load4_direct(src + (roff + (uint32_t)gx));
 			else
-				vals[k] = v;
+				raw[it] = load4_gather(src + roff, xmap, 4 * k, step);
 		}
 	}
-}
-
-__device__ __forceinline__ void store_tile(float *sA, int tid, const float (&vals)[kLoadIters])
-{
 #pragma unroll
-	for (int k = 0; k < kLoadIters; k++)
+	for (int it = 0; it < kFwdSlotIters; it++)
 	{
-		const int i = tid + k * kDwtThreads;
-		if (k < kLoadIters - 1 || tid < kTileIn * kTileIn - (kLoadIters - 1) * kDwtThreads)
+		const int slot = tid + it * kDwtThreads;
+		if (it < kFwdSlotIters - 1 || slot < kFwdSlots)
 		{
-			const int r = i / kTileIn, c = i - r * kTileIn;
-			sA[r * kPitchA + c] = vals[k];
+			const int r = slot / kFwdGroups, k = slot - r * kFwdGroups;
+			float f[4];
+			unpack4(raw[it], lut, f);
+			float *d = sA + r * kFwdPitch + 4 * k;
+#pragma unroll
+			for (int i = 0; i < 4; i++)
+				d[i] = f[i];
 		}
 	}
 }
-
-// First-pass work: 576 windows (72 lines x 8 segments); thread tid takes
-// tasks tid, tid + 256 and (first wave only) tid + 512.
-constexpr int kPass1Tasks = kTileIn * 8;
-constexpr int kPass1Rounds = (kPass1Tasks + kDwtThreads - 1) / kDwtThreads;
 
 __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__restrict__ jobs,
                                                              const uint16_t *__restrict__ tile_job,
                                                              const FrameState *__restrict__ frames, int nxcd)
 {
-	__shared__ float sA[kTileIn * kPitchA];
-	__shared__ int s_xmap[kTileIn];
-	__shared__ int s_ymap[kTileIn];
+	__shared__ float sA[kFwdInH * kFwdPitch];
+	__shared__ int s_xmap[kFwdInW];
+	__shared__ int s_ymap[kFwdInH];
 	__shared__ float s_lut[256];
 
 	const int tid = threadIdx.x;
 	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
-	const DwtJob J = jobs[tile_job[tile]];
+	// The job index is uniform: readfirstlane keeps J in scalar registers.
+	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
-	const int x0 = (local % J.tiles_x) * kTile;
-	const int y0 = (local / J.tiles_x) * kTile;
+	const int x0 = (local % J.tiles_x) * kFwdTileW;
+	const int y0 = (local / J.tiles_x) * kFwdTileH;
 
-	if (tid < kTileIn)
+	if (tid < kFwdInW)
 		s_xmap[tid] = fwd_ext(x0 - kApron + tid, J.R_x, J.A_x);
-	else if (tid < 2 * kTileIn)
-		s_ymap[tid - kTileIn] = fwd_ext(y0 - kApron + tid - kTileIn, J.R_y, J.A_y);
+	else if (tid < kFwdInW + kFwdInH)
+		s_ymap[tid - kFwdInW] = fwd_ext(y0 - kApron + tid - kFwdInW, J.R_y, J.A_y);
 	s_lut[tid] = c_u8_dc[tid];
 	__syncthreads();
 
-	float vals[kLoadIters];
 	if (J.src_kind == kStoreU8)
 	{
 		const FrameState &F = frames[J.frame];
-		load_tile(F.src[J.component], F.src_pitch[J.component], F.src_step[J.component], s_ymap, s_xmap, s_lut, tid,
-		          vals);
+		const uint8_t *src = F.src[J.component];
+		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
+		const bool vec_ok = step == 1 && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)pitch) & 3) == 0;
+		fwd_stage(src, pitch, step, vec_ok, x0, J.R_x, s_ymap, s_xmap, s_lut, tid, sA);
 	}
 	else if (J.src_kind == kStoreF16)
-		load_tile(static_cast<const __half *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, s_lut, tid, vals);
+		fwd_stage(static_cast<const __half *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
+		          s_lut, tid, sA);
 	else
-		load_tile(static_cast<const float *>(J.src), J.src_pitch, 1, s_ymap, s_xmap, s_lut, tid, vals);
-	store_tile(sA, tid, vals);
+		fwd_stage(static_cast<const float *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
+		          s_lut, tid, sA);
 	__syncthreads();
 
-	// Horizontal pass: results stay in registers until every thread has read
-	// its windows, then overwrite the tile in place (low half | high half).
-	float h[kPass1Rounds][8];
-#pragma unroll
-	for (int t = 0; t < kPass1Rounds; t++)
+	// Horizontal pass: wave w takes window w (outputs 32w .. 32w+31) of the
+	// 64 staged rows, one row per lane.
 	{
-		const int task = tid + t * kDwtThreads;
-		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
+		const int row = tid & 63, seg = tid >> 6;
+		float v[kFwdHWin];
+#pragma unroll
+		for (int j = 0; j < kFwdHWin; j++)
+			v[j] = sA[row * kFwdPitch + seg * kFwdHOut + j];
+		lift_fwd(v);
+		__syncthreads();
+		// Low outputs to columns 0..63, high outputs to 64..127.
+#pragma unroll
+		for (int k = 0; k < kFwdHOut / 2; k++)
 		{
-			const int r = task % kTileIn, seg = task / kTileIn;
-			float v[16];
-#pragma unroll
-			for (int j = 0; j < 16; j++)
-				v[j] = sA[r * kPitchA + seg * 8 + j];
-			lift_fwd16(v);
-#pragma unroll
-			for (int k = 0; k < 4; k++)
-			{
-				h[t][k] = v[4 + 2 * k] * kInvK;
-				h[t][4 + k] = v[5 + 2 * k] * kK;
-			}
-		}
-	}
-	__syncthreads();
-#pragma unroll
-	for (int t = 0; t < kPass1Rounds; t++)
-	{
-		const int task = tid + t * kDwtThreads;
-		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
-		{
-			const int r = task % kTileIn, seg = task / kTileIn;
-#pragma unroll
-			for (int k = 0; k < 4; k++)
-			{
-				sA[r * kPitchA + seg * 4 + k] = h[t][k];
-				sA[r * kPitchA + 32 + seg * 4 + k] = h[t][4 + k];
-			}
+			sA[row * kFwdPitch + seg * (kFwdHOut / 2) + k] = v[4 + 2 * k] * kInvK;
+			sA[row * kFwdPitch + 64 + seg * (kFwdHOut / 2) + k] = v[5 + 2 * k] * kK;
 		}
 	}
 	__syncthreads();
 
-	// Vertical pass: 64 columns x 8 segments; lanes are consecutive columns.
-#pragma unroll
-	for (int t = 0; t < 2; t++)
+	// Vertical pass: lanes are consecutive columns; threads 0..127 take
+	// output rows 0..27, threads 128..255 rows 28..55; each store
+	// instruction writes one band row.
 	{
-		const int task = tid + t * kDwtThreads;
-		const int c = task & 63, seg = task >> 6;
-		float v[16];
+		const int col = tid & 127, seg = tid >> 7;
+		float v[kFwdVWin];
 #pragma unroll
-		for (int j = 0; j < 16; j++)
-			v[j] = sA[(seg * 8 + j) * kPitchA + c];
-		lift_fwd16(v);
-		const int ox = (x0 >> 1) + (c & 31);
+		for (int j = 0; j < kFwdVWin; j++)
+			v[j] = sA[(seg * kFwdVOut + j) * kFwdPitch + col];
+		lift_fwd(v);
+		const int ox = (x0 >> 1) + (col & 63);
 		if (ox < J.bw)
 		{
-			void *lo = c < 32 ? J.band[0] : J.band[1];
-			void *hi = c < 32 ? J.band[2] : J.band[3];
-			const int lo_kind = c < 32 ? J.ll_kind : J.band_kind;
+			void *lo = col < 64 ? J.band[0] : J.band[1];
+			void *hi = col < 64 ? J.band[2] : J.band[3];
+			const int lo_kind = col < 64 ? J.ll_kind : J.band_kind;
+			const int oy0 = (y0 >> 1) + seg * (kFwdVOut / 2);
 #pragma unroll
-			for (int k = 0; k < 4; k++)
+			for (int k = 0; k < kFwdVOut / 2; k++)
 			{
-				const int oy = (y0 >> 1) + seg * 4 + k;
+				const int oy = oy0 + k;
 				if (oy < J.bh)
 				{
 					const long idx = (long)oy * J.band_pitch + ox;
@@ -274,6 +365,8 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 		}
 	}
 }
+
+// --- inverse ----------------------------------------------------------------
 
 __device__ __forceinline__ uint8_t unorm8_store(float v)
 {
@@ -291,7 +384,7 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 template <typename TH>
 __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, const int *xmap, int tid, float *sA)
 {
-	constexpr int kQ = kTileIn / 2;
+	constexpr int kQ = kInvIn / 2;
 	constexpr int kIters = (kQ * kQ + kDwtThreads - 1) / kDwtThreads;
 	const long pitch = J.band_pitch;
 	const float *ll = static_cast<const float *>(J.band[0]);
@@ -334,7 +427,7 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 			const int rr = i / kQ, cc = i - rr * kQ;
 #pragma unroll
 			for (int b = 0; b < 4; b++)
-				sA[(2 * rr + (b >> 1)) * kPitchA + 2 * cc + (b & 1)] = vals[b][k];
+				sA[(2 * rr + (b >> 1)) * kInvPitch + 2 * cc + (b & 1)] = vals[b][k];
 		}
 	}
 }
@@ -343,21 +436,22 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
                                                              const uint16_t *__restrict__ tile_job,
                                                              const FrameState *__restrict__ frames, int nxcd)
 {
-	__shared__ float sA[kTileIn * kPitchA];
-	__shared__ int s_xmap[kTileIn];
-	__shared__ int s_ymap[kTileIn];
+	__shared__ float sA[kInvIn * kInvPitch];
+	__shared__ int s_xmap[kInvIn];
+	__shared__ int s_ymap[kInvIn];
 
 	const int tid = threadIdx.x;
 	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
-	const DwtJob J = jobs[tile_job[tile]];
+	// The job index is uniform: readfirstlane keeps J in scalar registers.
+	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
-	const int ox0 = (local % J.tiles_x) * kTile;
-	const int oy0 = (local / J.tiles_x) * kTile;
+	const int ox0 = (local % J.tiles_x) * kInvTile;
+	const int oy0 = (local / J.tiles_x) * kInvTile;
 
-	if (tid < kTileIn)
+	if (tid < kInvIn)
 		s_xmap[tid] = inv_ext(ox0 - kApron + tid, J.A_x);
-	else if (tid < 2 * kTileIn)
-		s_ymap[tid - kTileIn] = inv_ext(oy0 - kApron + tid - kTileIn, J.A_y);
+	else if (tid < 2 * kInvIn)
+		s_ymap[tid - kInvIn] = inv_ext(oy0 - kApron + tid - kInvIn, J.A_y);
 	__syncthreads();
 
 	// Interleaved position parity selects the band: LL (even, even), HL (x
@@ -369,18 +463,18 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 	__syncthreads();
 
 	// Vertical pass over all 72 columns; rows of the tile are windows 4..11.
-	float vout[kPass1Rounds][8];
+	float vout[kInvPass1Rounds][8];
 #pragma unroll
-	for (int t = 0; t < kPass1Rounds; t++)
+	for (int t = 0; t < kInvPass1Rounds; t++)
 	{
 		const int task = tid + t * kDwtThreads;
-		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
+		if (t < kInvPass1Rounds - 1 || task < kInvPass1Tasks)
 		{
-			const int c = task % kTileIn, seg = task / kTileIn;
+			const int c = task % kInvIn, seg = task / kInvIn;
 			float v[16];
 #pragma unroll
 			for (int j = 0; j < 16; j++)
-				v[j] = sA[(seg * 8 + j) * kPitchA + c] * ((j & 1) ? kInvK : kK);
+				v[j] = sA[(seg * 8 + j) * kInvPitch + c] * ((j & 1) ? kInvK : kK);
 			lift_inv16(v);
 #pragma unroll
 			for (int j = 0; j < 8; j++)
@@ -389,15 +483,15 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 	}
 	__syncthreads();
 #pragma unroll
-	for (int t = 0; t < kPass1Rounds; t++)
+	for (int t = 0; t < kInvPass1Rounds; t++)
 	{
 		const int task = tid + t * kDwtThreads;
-		if (t < kPass1Rounds - 1 || task < kPass1Tasks)
+		if (t < kInvPass1Rounds - 1 || task < kInvPass1Tasks)
 		{
-			const int c = task % kTileIn, seg = task / kTileIn;
+			const int c = task % kInvIn, seg = task / kInvIn;
 #pragma unroll
 			for (int j = 0; j < 8; j++)
-				sA[(seg * 8 + j) * kPitchA + c] = vout[t][j];
+				sA[(seg * 8 + j) * kInvPitch + c] = vout[t][j];
 		}
 	}
 	__syncthreads();
@@ -418,7 +512,7 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 		float v[16];
 #pragma unroll
 		for (int j = 0; j < 16; j++)
-			v[j] = sA[row * kPitchA + seg * 8 + j] * ((j & 1) ? kInvK : kK);
+			v[j] = sA[row * kInvPitch + seg * 8 + j] * ((j & 1) ? kInvK : kK);
 		lift_inv16(v);
 		const int py = oy0 + row;
 		if (J.dst_kind == kStoreU8)
@@ -464,9 +558,9 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 } // namespace
 
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                        bool fp16, hipStream_t stream)
+                        bool u8_only, hipStream_t stream)
 {
-	(void)fp16;
+	(void)u8_only;
 	if (tiles > 0)
 		hipLaunchKernelGGL(k_dwt_forward, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames,
 		                   xcd_count());

@@ -571,8 +571,9 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
 						J.R_y = J.A_y;
 					}
 				}
-				J.tiles_x = (J.A_x + kTile - 1) / kTile;
-				int tiles_y = (J.A_y + kTile - 1) / kTile;
+				const int tw = encoder ? kFwdTileW : kInvTileW, th = encoder ? kFwdTileH : kInvTileH;
+				J.tiles_x = (J.A_x + tw - 1) / tw;
+				int tiles_y = (J.A_y + th - 1) / th;
 				J.tile_base = int(tile_map[l].size());
 				uint16_t idx = uint16_t(jobs[l].size());
 				jobs[l].push_back(J);
@@ -615,6 +616,9 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
 	ngroups = int(grp_frame.size());
 	for (int l = 0; l < kLevels; l++)
 	{
+		u8_only[l] = !jobs[l].empty();
+		for (const DwtJob &J : jobs[l])
+			u8_only[l] = u8_only[l] && J.src_kind == kStoreU8;
 		tiles[l] = int(tile_map[l].size());
 		if (jobs[l].size() > 65535)
 			return false;
@@ -693,7 +697,7 @@ void run_encode(Plan &P, hipStream_t stream)
 
 	D.begin_stage(stream, kStageDWT, n, &ev);
 	for (int l = 0; l < kLevels; l++)
-		launch_dwt_forward(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.any_fp16, stream);
+		launch_dwt_forward(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.u8_only[l], stream);
 	D.end_stage(stream, kStageDWT, n, ev);
 
 	D.begin_stage(stream, kStageQuant, n, &ev);

@@ -174,6 +174,7 @@ struct Plan
 	int ngroups = 0;
 	int max_n32 = 0;
 	bool any_fp16 = false;
+	bool u8_only[kLevels] = {};  // every forward job of the level reads a u8 plane
 
 	bool build(const std::vector<Slot *> &slots, bool encoder);
 	// Writes the per-call fields (call(i)) into the device FrameState array.

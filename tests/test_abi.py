@@ -28,7 +28,7 @@ def declared_functions():
 def lib():
     if not os.path.exists(pw.LIB_PATH):
         from pyrowave_amd import build
-        build.main()
+        build.build()
     return pw.load()
 
 
