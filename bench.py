@@ -45,10 +45,29 @@ def parse():
                          "(counter calibration for HBM traffic)")
     ap.add_argument("--trace-summary", metavar="CSV",
                     help="summarize a rocprofv3 kernel trace CSV per (kernel, workgroups) and exit")
+    ap.add_argument("--counter-table", nargs="+", metavar="CSV",
+                    help="average of every counter per (kernel, workgroups) in rocprofv3 --pmc CSVs, and exit")
     ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="per-launch HBM traffic from rocprofv3 FETCH_SIZE / WRITE_SIZE counter CSVs of a "
                          "--calibrate-copy run, scaled by the copy kernel's reading, and exit")
     return ap.parse_args()
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_4k420_fp16_b16.txt")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
+    summary (a line `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                    return int(parts[2])
+    except OSError:
+        pass
+    return None
 
 
 def dwt_algorithmic_bytes(fp16):
@@ -91,6 +110,29 @@ def trace_summary(path, top=40):
     for (name, wgs), v in sorted(agg.items(), key=lambda kv: -sum(kv[1]))[:top]:
         print(f"{name:48s} {wgs:10d} {len(v):6d} {sum(v) / len(v) / 1e3:9.2f} {min(v) / 1e3:9.2f} "
               f"{max(v) / 1e3:9.2f} {sum(v) / total:6.1%}")
+
+
+def counter_table(paths):
+    """Average of each counter per (kernel, workgroup count) over the
+    dispatches of rocprofv3 --pmc CSV files."""
+    import collections
+    import csv
+    import re
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                m = re.search(r"(k_[a-z_0-9]+)", r["Kernel_Name"])
+                if not m:
+                    continue
+                wgs = int(r["Grid_Size"]) // max(int(r["Workgroup_Size"]), 1)
+                agg[(m.group(1), wgs)][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({c for v in agg.values() for c in v})
+    print(f"{'kernel':20s} {'workgroups':>10s} " + " ".join(f"{n:>22s}" for n in names))
+    for key in sorted(agg, key=lambda k: (k[0], -k[1])):
+        v = agg[key]
+        print(f"{key[0]:20s} {key[1]:10d} " + " ".join(
+            f"{sum(v[n]) / len(v[n]):22.4g}" if v[n] else f"{'-':>22s}" for n in names))
 
 
 def _counter_rows(path):
@@ -137,6 +179,9 @@ def main():
         return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
+        return
+    if args.counter_table:
+        counter_table(args.counter_table)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -258,7 +303,7 @@ def main():
         us_per_launch = ms * 1e3 / max(launches, 1)
         achieved = bytes_per_launch / (us_per_launch * 1e-6) / 1e9
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
                 "launches_timed": launches}
 

@@ -899,8 +899,9 @@ struct BlockBytes
 
 // reference shaders/wavelet_dequant.comp:100-160 and 220-391.  The coded
 // block (at most kMaxBlockWords words for any block an encoder can produce) is
-// first copied to LDS with coalesced loads; the parse then reads LDS only.
-// Bytes past the block, or past the payload, read as zero.
+// first copied to LDS with coalesced loads and parsed from there.  A malformed
+// block that reads past its own length reads the following payload bytes, as
+// the reference does; bytes past the payload read as zero.
 __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
                                                            const uint32_t *__restrict__ blk_band,
                                                            const FrameState *__restrict__ frames,

@@ -545,12 +545,23 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 		}
 		else if (py < J.A_y)
 		{
-#pragma unroll
-			for (int j = 4; j < 12; j++)
+			const int px0 = ox0 + seg * 8;
+			const long idx = (long)py * dpitch + px0;
+			if (J.dst_kind == kStoreF32 && px0 + 8 <= J.A_x)
 			{
-				const int px = ox0 + seg * 8 + j - 4;
-				if (px < J.A_x)
-					store_kind(J.dst, J.dst_kind, (long)py * dpitch + px, v[j]);
+				// 32-byte aligned: px0 is a multiple of 8, the pitch of 32.
+				float4 *d = reinterpret_cast<float4 *>(static_cast<float *>(J.dst) + idx);
+				d[0] = make_float4(v[4], v[5], v[6], v[7]);
+				d[1] = make_float4(v[8], v[9], v[10], v[11]);
+			}
+			else
+			{
+#pragma unroll
+				for (int j = 4; j < 12; j++)
+				{
+					if (px0 + j - 4 < J.A_x)
+						store_kind(J.dst, J.dst_kind, idx + j - 4, v[j]);
+				}
 			}
 		}
 	}
