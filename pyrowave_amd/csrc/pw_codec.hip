@@ -108,6 +108,7 @@ __device__ __forceinline__ void store_band(const BandJob &J, long idx, float v)
 constexpr int kDppXor1 = 0xB1;
 constexpr int kDppXor2 = 0x4E;
 constexpr int kDppHalfMirror = 0x141;
+constexpr int kDppRowMirror = 0x140; // lane i <-> 15 - i within each 16-lane row
 
 template <int Ctrl>
 __device__ __forceinline__ int dpp_i(int v)
@@ -135,6 +136,19 @@ __device__ __forceinline__ int cluster_sum_i(int v)
 	v += dpp_i<kDppXor2>(v);
 	v += dpp_i<kDppHalfMirror>(v);
 	return v;
+}
+
+// Sum over the 64 lanes of a wave (exact for integers): row butterfly with
+// DPP, then the four row sums through readlane.
+__device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
+{
+	int x = (int)v;
+	x += dpp_i<kDppXor1>(x);
+	x += dpp_i<kDppXor2>(x);
+	x += dpp_i<kDppHalfMirror>(x);
+	x += dpp_i<kDppRowMirror>(x);
+	return (uint32_t)(__builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) +
+	                  __builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48));
 }
 
 __device__ __forceinline__ float cluster_max_f(float v)
@@ -409,28 +423,34 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 	__syncthreads();
 
 	const FrameState &F = frames[J.frame];
-	if (tid < 16)
 	{
-		// reference shaders/analyze_rate_control.comp:169-215.
-		const int q = tid;
-		uint32_t total = 0;
-		float d[16];
+		// reference shaders/analyze_rate_control.comp:169-215.  Lanes 8q .. 8q+7
+		// sum quant level q over the 16 8x8 blocks, two blocks per lane; the
+		// cluster butterfly adds the pair sums in the tree order
+		// (((d0+d1)+(d2+d3))+((d4+d5)+(d6+d7))) + (same for d8..d15).
+		const int q = tid >> 3, j = tid & 7;
+		int cpair = 0;
+		float d[2];
 #pragma unroll
-		for (int k = 0; k < 16; k++)
+		for (int i = 0; i < 2; i++)
 		{
+			const int k = 2 * j + i;
 			// errors[min(q, num_planes)]: past the last plane the entry is the
 			// cost-free "discard" one; zero 8x8 blocks are (0, 0) throughout.
 			const bool live = q <= s_msb[k];
-			const uint32_t c = live ? (uint32_t)s_cost[q][k] : 0u;
-			d[k] = live ? s_dist[q][k] : s_disc[k];
-			total += c ? c + 24u : 0u;
+			const int c = live ? s_cost[q][k] : 0;
+			d[i] = live ? s_dist[q][k] : s_disc[k];
+			cpair += c ? c + 24 : 0;
 		}
-		if (total)
-			total += 64;
-		float D = (((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7]))) +
-		          (((d[8] + d[9]) + (d[10] + d[11])) + ((d[12] + d[13]) + (d[14] + d[15])));
-		s_words[q] = (total + 31) >> 5;
-		s_D[q] = D;
+		uint32_t total = (uint32_t)cluster_sum_i(cpair);
+		const float D = cluster_sum_f(d[0] + d[1]);
+		if (j == 0)
+		{
+			if (total)
+				total += 64;
+			s_words[q] = (total + 31) >> 5;
+			s_D[q] = D;
+		}
 	}
 	__syncthreads();
 	if (tid < 16)
@@ -758,9 +778,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 		uint32_t acc = 0;
 		for (int i = tid; i < groups; i += 64)
 			acc += F.group_sum[i];
-#pragma unroll
-		for (int d = 32; d >= 1; d >>= 1)
-			acc += __shfl_xor(acc, d, 64);
+		acc = wave_sum_u32(acc);
 		if (tid == 0)
 			s_group_base = acc;
 	}
@@ -776,6 +794,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	const uint32_t in_q = (cw0 >> 16) & 0xf, out_q = (cw >> 16) & 0xf;
 	const uint32_t in_planes = ((cw0 >> (2 * s)) & 3u) + in_q;
 	const uint32_t out_planes = active ? ((cw >> (2 * s)) & 3u) + out_q : 0u;
+	// Bit planes, MSB first: plane byte bit p is bit `bit` of magnitude p.
+	// Magnitudes p and p+4 share a register (low / high half), so one plane
+	// of all eight is four shift-and-mask steps and a fold.
+	uint32_t pair[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+		pair[i] = (uint32_t)Q.n[i] | ((uint32_t)Q.n[i + 4] << 16);
 	uint32_t sig = 0;
 	uint32_t plane_bytes_lane[14];
 #pragma unroll
@@ -784,10 +809,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 		uint32_t byte = 0;
 		if ((uint32_t)pi < out_planes)
 		{
-			int bit = (int)in_planes - 1 - pi;
+			const int bit = (int)in_planes - 1 - pi;
+			uint32_t c = 0;
 #pragma unroll
-			for (int p = 0; p < 8; p++)
-				byte |= (uint32_t)((Q.n[p] >> bit) & 1) << p;
+			for (int i = 0; i < 4; i++)
+				c |= ((pair[i] >> bit) & 0x00010001u) << i;
+			byte = (c & 0xfu) | ((c >> 12) & 0xf0u);
 		}
 		plane_bytes_lane[pi] = byte;
 		sig |= byte;
@@ -810,10 +837,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	if (wcount)
 		wcount += 2;
 
-	int planes_total;
-	const int plane_excl = block_excl_scan128((int)out_planes, s_tmp, planes_total);
-	int sig_total;
-	const int sig_excl = block_excl_scan128(nsig, s_tmp, sig_total);
+	// One scan of (planes << 16 | significant count): both sums stay below
+	// 2^16 (128 lanes x 14 planes, 1024 coefficients).
+	int packed_total;
+	const int packed_excl = block_excl_scan128(((int)out_planes << 16) | nsig, s_tmp, packed_total);
+	const int plane_excl = packed_excl >> 16, sig_excl = packed_excl & 0xffff;
+	const int planes_total = packed_total >> 16;
 
 	// Lanes own disjoint byte ranges of the block, so control words, scale
 	// bytes and plane bytes are plain byte stores; sign bits share words and
@@ -984,12 +1013,36 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			uint32_t byte_off = s_poff[k8] + (uint32_t)__popc(lsbs & lmask) + (uint32_t)__popc(msbs & lmask) + q_bits * s;
 			const int planes = (int)(q_bits + ((ctrl >> bo) & 3u));
 			int mag[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-			for (int pl = planes - 1; pl >= 0; pl--)
+			if (planes <= 16)
 			{
-				uint32_t byte = byte_at(byte_off++);
+				// Magnitudes p and p+4 accumulate in the low / high half of one
+				// register: a plane byte is spread once, then four shifts.
+				uint32_t pair[4] = {0, 0, 0, 0};
+				for (int pl = planes - 1; pl >= 0; pl--)
+				{
+					const uint32_t byte = byte_at(byte_off++);
+					const uint32_t c = (byte & 0xfu) | ((byte & 0xf0u) << 12);
 #pragma unroll
-				for (int p = 0; p < 8; p++)
-					mag[p] |= (int)((byte >> p) & 1u) << pl;
+					for (int i = 0; i < 4; i++)
+						pair[i] |= ((c >> i) & 0x00010001u) << pl;
+				}
+#pragma unroll
+				for (int i = 0; i < 4; i++)
+				{
+					mag[i] = (int)(pair[i] & 0xffffu);
+					mag[i + 4] = (int)(pair[i] >> 16);
+				}
+			}
+			else
+			{
+				// Up to 18 planes (only in malformed blocks).
+				for (int pl = planes - 1; pl >= 0; pl--)
+				{
+					uint32_t byte = byte_at(byte_off++);
+#pragma unroll
+					for (int p = 0; p < 8; p++)
+						mag[p] |= (int)((byte >> p) & 1u) << pl;
+				}
 			}
 			// reference shaders/wavelet_dequant.comp:168-175, 312-315.
 			int ef = (int)(q_code >> 3);
@@ -1012,16 +1065,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		}
 		int sig_total;
 		const int sig_excl = block_excl_scan128(sig_count, s_tmp, sig_total);
-		uint32_t sbit = s_sign_base + (uint32_t)sig_excl;
+		// This lane's sign bits are consecutive (at most 8): two bytes hold
+		// them all.
+		const uint32_t sbit = s_sign_base + (uint32_t)sig_excl;
+		const uint32_t signs = (byte_at(sbit >> 3) | (byte_at((sbit >> 3) + 1) << 8)) >> (sbit & 7);
+		int n = 0;
 #pragma unroll
 		for (int p = 0; p < 8; p++)
 		{
 			if (v[p] != 0.0f)
 			{
-				uint32_t byte = byte_at(sbit >> 3);
-				if ((byte >> (sbit & 7)) & 1u)
+				if ((signs >> n) & 1u)
 					v[p] = -v[p];
-				sbit++;
+				n++;
 			}
 		}
 	}
