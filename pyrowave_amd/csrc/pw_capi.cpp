@@ -1175,11 +1175,11 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encoder_encode_coefficients(pyr
 	set_encode_params(F, S, rate_control->maximum_bitstream_size);
 	FrameState *frames = P->stage_frames(D.stream);
 	launch_rc_reset(frames, 1, P->max_n32, D.stream);
-	launch_quant_analyze(P->d_bands, P->d_blk_band, P->nblocks, frames, P->d_frame_block_base, D.stream);
+	launch_quant_analyze(P->d_bands, P->d_blk_band, P->nblocks, frames, D.stream);
 	launch_rc_finalize(frames, 1, D.stream);
 	launch_rc_partial(frames, 1, D.stream);
 	launch_rc_prepare(frames, P->d_grp_frame, P->ngroups, P->d_frame_group_base, D.stream);
-	launch_pack(P->d_bands, P->d_blk_band, P->nblocks, frames, P->d_frame_block_base, D.stream);
+	launch_pack(P->d_bands, P->d_blk_band, P->nblocks, frames, D.stream);
 	if (!queue_result_download(S, D.stream, uint32_t(F.target_words) + 64))
 		return PYROWAVE_ERROR_OUT_OF_HOST_MEMORY;
 	(void)hipEventRecord(S.result_ready, D.stream);

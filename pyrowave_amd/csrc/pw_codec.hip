@@ -179,18 +179,26 @@ __device__ __forceinline__ float round_f16_nonneg(float x)
 	return __uint_as_float(b);
 }
 
+// Inclusive scan over the 64 lanes of a wave with DPP (all lanes active):
+// row_shr 1, 2, 4, 8 scan each 16-lane row (lanes without a source read 0),
+// then row_bcast 15 carries row 0 into row 1 and row 2 into row 3, and
+// row_bcast 31 carries the first two rows into rows 2 and 3.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+	v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, true);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, true);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, true);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, true);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+	v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
+	return v;
+}
+
 // Exclusive scan over the 128 lanes of a workgroup (two waves).
 __device__ __forceinline__ int block_excl_scan128(int v, int *s_tmp, int &total)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	int incl = v;
-#pragma unroll
-	for (int d = 1; d < 64; d <<= 1)
-	{
-		int up = __shfl_up(incl, d, 64);
-		if (lane >= d)
-			incl += up;
-	}
+	const int incl = wave_incl_scan(v);
 	if (lane == 63)
 		s_tmp[wave] = incl;
 	__syncthreads();
@@ -321,23 +329,25 @@ struct BlockCoord
 	int bx, by;
 };
 
-__device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const uint32_t *blk_band,
-                                                   const int32_t *frame_block_base, int nxcd)
+// blk_band holds two words per block of the launch: the band job index and
+// the block position bx | by << 16 within its band.
+__device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const uint32_t *blk_band, int nxcd)
 {
 	BlockCoord C;
 	const int g = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
-	C.J = bands[__builtin_amdgcn_readfirstlane(blk_band[g])];
-	C.b = g - frame_block_base[C.J.frame];
-	int r = C.b - C.J.off32;
-	C.by = r / C.J.bx32;
-	C.bx = r - C.by * C.J.bx32;
+	const uint2 d = reinterpret_cast<const uint2 *>(blk_band)[g];
+	C.J = bands[__builtin_amdgcn_readfirstlane(d.x)];
+	const uint32_t pos = __builtin_amdgcn_readfirstlane(d.y);
+	C.bx = (int)(pos & 0xffffu);
+	C.by = (int)(pos >> 16);
+	C.b = C.J.off32 + C.by * C.J.bx32 + C.bx;
 	return C;
 }
 
 __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
                                                                  const uint32_t *__restrict__ blk_band,
                                                                  const FrameState *__restrict__ frames,
-                                                                 const int32_t *__restrict__ frame_block_base, int nxcd)
+                                                                 int nxcd)
 {
 	__shared__ int s_cost[16][16];
 	__shared__ float s_dist[16][16];
@@ -347,7 +357,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 	__shared__ float s_D[16];
 	__shared__ uint32_t s_v[16];
 
-	const BlockCoord C = locate_block(bands, blk_band, frame_block_base, nxcd);
+	const BlockCoord C = locate_block(bands, blk_band, nxcd);
 	const BandJob &J = C.J;
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
 	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
@@ -608,14 +618,7 @@ __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ fr
 __device__ __forceinline__ int block_incl_scan256(int v, int *s_tmp, int &total)
 {
 	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	int incl = v;
-#pragma unroll
-	for (int d = 1; d < 64; d <<= 1)
-	{
-		int up = __shfl_up(incl, d, 64);
-		if (lane >= d)
-			incl += up;
-	}
+	const int incl = wave_incl_scan(v);
 	if (lane == 63)
 		s_tmp[wave] = incl;
 	__syncthreads();
@@ -739,7 +742,7 @@ __device__ __forceinline__ void lds_store_byte(uint32_t *buf, uint32_t byte_pos,
 __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restrict__ bands,
                                                         const uint32_t *__restrict__ blk_band,
                                                         FrameState *__restrict__ frames,
-                                                        const int32_t *__restrict__ frame_block_base, int nxcd)
+                                                        int nxcd)
 {
 	__shared__ uint32_t s_buf[kMaxBlockWords];
 	__shared__ int s_tmp[2];
@@ -747,7 +750,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	__shared__ uint32_t s_bits;
 	__shared__ uint32_t s_group_base;
 
-	const BlockCoord C = locate_block(bands, blk_band, frame_block_base, nxcd);
+	const BlockCoord C = locate_block(bands, blk_band, nxcd);
 	const BandJob &J = C.J;
 	FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
@@ -934,7 +937,7 @@ struct BlockBytes
 __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
                                                            const uint32_t *__restrict__ blk_band,
                                                            const FrameState *__restrict__ frames,
-                                                           const int32_t *__restrict__ frame_block_base, int nxcd)
+                                                           int nxcd)
 {
 	__shared__ uint32_t s_pay[kMaxBlockWords];
 	__shared__ uint32_t s_poff[16];
@@ -943,7 +946,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	__shared__ uint32_t s_sign_base;
 	__shared__ int s_tmp[2];
 
-	const BlockCoord C = locate_block(bands, blk_band, frame_block_base, nxcd);
+	const BlockCoord C = locate_block(bands, blk_band, nxcd);
 	const BandJob &J = C.J;
 	const FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
@@ -1108,11 +1111,11 @@ void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t s
 }
 
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
-                          const int32_t *frame_block_base, hipStream_t stream)
+                          hipStream_t stream)
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_quant_analyze, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   frame_block_base, xcd_count());
+		                   xcd_count());
 }
 
 void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream)
@@ -1134,18 +1137,18 @@ void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroup
 }
 
 void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
-                 const int32_t *frame_block_base, hipStream_t stream)
+                 hipStream_t stream)
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   frame_block_base, xcd_count());
+		                   xcd_count());
 }
 
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
-                    const int32_t *frame_block_base, hipStream_t stream)
+                    hipStream_t stream)
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   frame_block_base, xcd_count());
+		                   xcd_count());
 }
 } // namespace pw

@@ -197,16 +197,17 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         bool fp16, hipStream_t stream);
 void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream);
+// blk_band: two words per block (band job index, bx | by << 16).
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
-                          const int32_t *frame_block_base, hipStream_t stream);
+                          hipStream_t stream);
 void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream);
 void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream);
 void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroups_total,
                        const int32_t *frame_group_base, hipStream_t stream);
 void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
-                 const int32_t *frame_block_base, hipStream_t stream);
+                 hipStream_t stream);
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
-                    const int32_t *frame_block_base, hipStream_t stream);
+                    hipStream_t stream);
 void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream);
 void upload_constants();
 } // namespace pw

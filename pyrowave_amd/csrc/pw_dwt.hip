@@ -372,9 +372,7 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 {
 	// +0.5 DC shift, clamp to [0,1], round to nearest (reference
 	// shaders/idwt.comp:207-211 with an R8_UNORM store).
-	v = v + 0.5f;
-	v = v > 0.0f ? v : 0.0f;
-	v = v < 1.0f ? v : 1.0f;
+	v = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
@@ -386,7 +384,9 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 {
 	constexpr int kQ = kInvIn / 2;
 	constexpr int kIters = (kQ * kQ + kDwtThreads - 1) / kDwtThreads;
-	const long pitch = J.band_pitch;
+	// 32-bit element offsets from the uniform band bases (bands stay below
+	// 4 GiB), so loads use a scalar base and a 32-bit vector offset.
+	const uint32_t pitch = (uint32_t)J.band_pitch;
 	const float *ll = static_cast<const float *>(J.band[0]);
 	const TH *hb[3] = {static_cast<const TH *>(J.band[1]), static_cast<const TH *>(J.band[2]),
 	                   static_cast<const TH *>(J.band[3])};
@@ -401,8 +401,9 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 		if (i < kQ * kQ)
 		{
 			const int rr = i / kQ, cc = i - rr * kQ;
-			const long row0 = (long)(ymap[2 * rr] >> 1) * pitch, row1 = (long)(ymap[2 * rr + 1] >> 1) * pitch;
-			const int col0 = xmap[2 * cc] >> 1, col1 = xmap[2 * cc + 1] >> 1;
+			const uint32_t row0 = __umul24((uint32_t)(ymap[2 * rr] >> 1), pitch);
+			const uint32_t row1 = __umul24((uint32_t)(ymap[2 * rr + 1] >> 1), pitch);
+			const uint32_t col0 = (uint32_t)(xmap[2 * cc] >> 1), col1 = (uint32_t)(xmap[2 * cc + 1] >> 1);
 			vals[0][k] = ll[row0 + col0];
 			if constexpr (sizeof(TH) == 2)
 			{

@@ -581,7 +581,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
 			}
 		}
 
-		frame_block_base[f] = int32_t(blk_band.size());
+		frame_block_base[f] = int32_t(blk_band.size() / 2);
 		for (int l = kLevels - 1; l >= 0; l--)
 			for (int c = 0; c < kComponents; c++)
 				for (int b = 0; b < kBands; b++)
@@ -607,12 +607,18 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
 					J.qcode = encode_quant(1.0f / res);
 					uint32_t idx = uint32_t(band_jobs.size());
 					band_jobs.push_back(J);
-					blk_band.insert(blk_band.end(), size_t(B.bx32) * B.by32, idx);
+					// Two words per block: band job, position bx | by << 16.
+					for (int by = 0; by < B.by32; by++)
+						for (int bx = 0; bx < B.bx32; bx++)
+						{
+							blk_band.push_back(idx);
+							blk_band.push_back(uint32_t(bx) | (uint32_t(by) << 16));
+						}
 				}
 		frame_group_base[f] = int32_t(grp_frame.size());
 		grp_frame.insert(grp_frame.end(), size_t(G.ngroups), uint32_t(f));
 	}
-	nblocks = int(blk_band.size());
+	nblocks = int(blk_band.size() / 2);
 	ngroups = int(grp_frame.size());
 	for (int l = 0; l < kLevels; l++)
 	{
@@ -701,7 +707,7 @@ void run_encode(Plan &P, hipStream_t stream)
 	D.end_stage(stream, kStageDWT, n, ev);
 
 	D.begin_stage(stream, kStageQuant, n, &ev);
-	launch_quant_analyze(P.d_bands, P.d_blk_band, P.nblocks, frames, P.d_frame_block_base, stream);
+	launch_quant_analyze(P.d_bands, P.d_blk_band, P.nblocks, frames, stream);
 	D.end_stage(stream, kStageQuant, n, ev);
 
 	D.begin_stage(stream, kStageResolve, n, &ev);
@@ -711,7 +717,7 @@ void run_encode(Plan &P, hipStream_t stream)
 	D.end_stage(stream, kStageResolve, n, ev);
 
 	D.begin_stage(stream, kStagePacking, n, &ev);
-	launch_pack(P.d_bands, P.d_blk_band, P.nblocks, frames, P.d_frame_block_base, stream);
+	launch_pack(P.d_bands, P.d_blk_band, P.nblocks, frames, stream);
 	D.end_stage(stream, kStagePacking, n, ev);
 }
 
@@ -723,7 +729,7 @@ void run_decode(Plan &P, hipStream_t stream)
 	hipEvent_t ev;
 
 	D.begin_stage(stream, kStageDequant, n, &ev);
-	launch_dequant(P.d_bands, P.d_blk_band, P.nblocks, frames, P.d_frame_block_base, stream);
+	launch_dequant(P.d_bands, P.d_blk_band, P.nblocks, frames, stream);
 	D.end_stage(stream, kStageDequant, n, ev);
 
 	D.begin_stage(stream, kStageIDWT, n, &ev);
