@@ -9,9 +9,10 @@ pyrowave_hip_decode_encoded_batch straight from the encoders' device results.
 
 Reported: µs per frame of the FP16-storage batched path (value), the plain
 FP32 single-frame path beside it, ΔPSNR between the two, the byte budget
-check, per-batch p50/p99 latency, the forward-DWT roofline from HIP events
-recorded on the launch stream, and the oracle (single-threaded CPU
-restatement) timed on a bounded sample as the CPU baseline.
+check, per-batch p50/p99 latency, the transform rooflines from HIP events
+recorded on the launch stream, the serving loop (64 mixed sessions: latency
+p50/p99 at cadence and saturated frames/s), and the oracle (single-threaded
+CPU restatement) timed on a bounded sample as the CPU baseline.
 """
 import argparse
 import json
@@ -40,6 +41,8 @@ def parse():
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the oracle CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serving-seconds", type=float, default=2.0,
+                    help="length of each serving-loop run (64 sessions); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run a plain 256 MiB read + 256 MiB write elementwise kernel "
                          "(counter calibration for HBM traffic)")
@@ -170,6 +173,97 @@ def pmc_summary(fetch_csv, write_csv):
         fv = sum(fagg[key]) / len(fagg[key]) * ff * 1024
         wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
         print(f"{key[0]:40s} {key[1]:10d} {len(fagg[key]):6d} {fv / 1e6:10.3f} {wv / 1e6:10.3f} {(fv + wv) / 1e6:10.3f}")
+
+
+SERVING_SHAPES = ((1280, 720, 0), (1920, 1080, 0), (2560, 1440, 1), (3840, 2160, 0))
+
+
+def serving_leg(pw, torch, device, dev_frames, seconds, sessions=64, seed=4):
+    """The serving loop: `sessions` sessions cycling through 720p/1080p 4:2:0,
+    1440p 4:4:4 and 4K 4:2:0 at 60 or 120 fps, Poisson arrivals with 1 ms
+    jitter, 1.2-2.0 bits/pixel, admitted by footprint, encode+decode per frame.
+    Two runs: at cadence (latency p50/p99 of submit -> bitstream on the host)
+    and saturated (every session keeps two frames queued: frames/s)."""
+    rng = np.random.default_rng(seed)
+    plan = []
+    for i in range(sessions):
+        w, h, c = SERVING_SHAPES[i % 4]
+        plan.append(dict(shape=(w, h, c), fps=(60, 120)[(i // 4) % 2],
+                         budget=int(w * h * rng.uniform(1.2, 2.0) / 8)))
+
+    # Content: crops of the resident 4K sequence (4:4:4 chroma planes are luma
+    # crops of neighbouring frames); every session gets its own decode target.
+    def frame_of(shape, k):
+        w, h, c = shape
+        if c:
+            planes = [dev_frames[(k + j) % len(dev_frames)][0][:h, :w] for j in range(3)]
+        else:
+            f = dev_frames[k % len(dev_frames)]
+            planes = [f[0][:h, :w], f[1][:h // 2, :w // 2], f[2][:h // 2, :w // 2]]
+        return pw.hip_frame([(t.data_ptr(), t.stride(0), 1) for t in planes])
+
+    frames = {s: [frame_of(s, k) for k in range(8)] for s in SERVING_SHAPES}
+    targets = []
+    for s in plan:
+        w, h, c = s["shape"]
+        ch, cw = (h, w) if c else (h // 2, w // 2)
+        outs = [torch.empty((h, w), dtype=torch.uint8, device="cuda")] + \
+               [torch.empty((ch, cw), dtype=torch.uint8, device="cuda") for _ in range(2)]
+        targets.append((outs, pw.hip_target([(o.data_ptr(), o.stride(0)) for o in outs])))
+
+    probe = pw.Server(device, 1 << 40)
+    footprint = sum(probe.footprint(*s["shape"]) for s in plan)
+    probe.close()
+    server = pw.Server(device, footprint, streams=4, precision=pw.PRECISION_FP16, packet_boundary=8192)
+    sids = [server.open_session(*s["shape"]) for s in plan]
+    assert all(x is not None for x in sids)
+
+    def run(saturated):
+        lat, starts, count = [], set(), 0
+        base = [rng.uniform(0, 1.0 / s["fps"]) for s in plan]
+        next_t = list(base)
+        k = [0] * len(plan)
+        inflight = [0] * len(plan)
+        t0 = time.perf_counter()
+        end = t0 + seconds
+        while True:
+            now = time.perf_counter()
+            if now < end:
+                for i, s in enumerate(plan):
+                    due = inflight[i] < 2 if saturated else (now - t0) >= next_t[i]
+                    if due:
+                        server.submit(sids[i], frames[s["shape"]][k[i] % 8], s["budget"], targets[i][1])
+                        k[i] += 1
+                        inflight[i] += 1
+                        base[i] += rng.exponential(1.0 / s["fps"])
+                        next_t[i] = base[i] + rng.normal(0, 1e-3)
+            elif sum(inflight) == 0:
+                break
+            for c in server.poll(256, timeout_us=0 if saturated else 50):
+                assert c.status == pw.SUCCESS, c.status
+                server.take_bitstream(c)
+                i = sids.index(c.session)
+                inflight[i] -= 1
+                count += 1
+                lat.append((c.done_ns - c.submit_ns) / 1e3)
+                starts.add(c.start_ns)
+        wall = time.perf_counter() - t0
+        lat = np.array(lat)
+        return {"frames": count, "seconds": round(wall, 3), "frames_per_s": round(count / wall, 1),
+                "latency_us": {"p50": round(float(np.percentile(lat, 50)), 1),
+                               "p99": round(float(np.percentile(lat, 99)), 1)},
+                "launches": len(starts), "mean_frames_per_launch": round(count / max(len(starts), 1), 2)}
+
+    run(True)  # warm the lanes' table storage
+    cadence = run(False)
+    saturated = run(True)
+    for sid in sids:
+        server.close_session(sid)
+    server.close()
+    offered = sum(s["fps"] for s in plan)
+    return {"sessions": sessions, "mix": "720p/1080p 4:2:0, 1440p 4:4:4, 4K 4:2:0 at 60/120 fps, 1.2-2.0 bpp",
+            "streams": 4, "footprint_bytes": int(footprint), "offered_frames_per_s": offered,
+            "cadence": cadence, "saturated": saturated}
 
 
 def main():
@@ -307,6 +401,11 @@ def main():
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
                 "launches_timed": launches}
 
+    serving = None
+    if args.serving_seconds > 0:
+        serving = serving_leg(pw, torch, device, dev_frames, args.serving_seconds)
+        print(f"[bench] serving done", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu:
         from oracle import oracle as O
@@ -354,6 +453,7 @@ def main():
         "stage_us_per_frame_fp32_single": stage_us32,
         "roofline": roofline("iDWT", "k_dwt_inverse"),
         "roofline_forward": roofline("DWT", "k_dwt_forward"),
+        "serving": serving,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

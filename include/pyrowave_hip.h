@@ -184,6 +184,82 @@ pyrowave_hip_device_set_timing(pyrowave_device device, bool enable);
 PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_timing_stages(pyrowave_device device, uint32_t stage_mask);
 
+/* ---- Serving loop ------------------------------------------------------
+ *
+ * No counterpart in the reference: its callers (pyrofling) run one encoder
+ * per stream themselves.  A server owns one encoder + decoder pair per
+ * session (device-resident state kept between frames), admits sessions by
+ * their HBM footprint against a budget, and batches continuously: whenever a
+ * stream is free, the frames ready at that moment (the oldest queued frame of
+ * each session whose previous frame is done, up to max_batch_frames) go to
+ * the device as one batch, one launch per stage, so a request that arrives
+ * while others are on the GPU joins the next batch and its work overlaps
+ * theirs on another stream.  Each finished frame leaves its packetized
+ * bitstream on the host (and its decode in the request's target). */
+typedef struct pyrowave_hip_server_opaque *pyrowave_hip_server;
+
+typedef struct pyrowave_hip_server_create_info
+{
+	pyrowave_device device;
+	size_t hbm_budget_bytes;    /* admission limit for session state */
+	uint32_t streams;           /* concurrent streams (0: 4) */
+	pyrowave_hip_precision precision;
+	size_t packet_boundary;     /* packet split for the host bitstreams (0: one packet) */
+	uint32_t max_batch_frames;  /* frames per launch (0: 16) */
+} pyrowave_hip_server_create_info;
+
+typedef struct pyrowave_hip_server_completion
+{
+	uint64_t ticket;
+	uint32_t session;
+	pyrowave_result status;
+	size_t bytes;               /* packetized bitstream size */
+	uint32_t packets;
+	uint64_t submit_ns, start_ns, done_ns; /* steady-clock times */
+} pyrowave_hip_server_completion;
+
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_create(const pyrowave_hip_server_create_info *info, pyrowave_hip_server *server);
+
+/* Waits for all work, closes every session. */
+PYROWAVE_PUBLIC_API void
+pyrowave_hip_server_destroy(pyrowave_hip_server server);
+
+/* HBM bytes one session of this geometry holds (encoder + decoder). */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_session_footprint(pyrowave_hip_server server, int width, int height,
+                                      pyrowave_chroma_subsampling chroma, size_t *bytes);
+
+/* Admits a session if its footprint fits the remaining budget; otherwise
+ * PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY (retry once sessions close). */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_open_session(pyrowave_hip_server server, int width, int height,
+                                 pyrowave_chroma_subsampling chroma, uint32_t *session);
+
+/* Waits for the session's queued frames, then frees its state. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_close_session(pyrowave_hip_server server, uint32_t session);
+
+/* Queues one frame of a session (device planes, budget; target may be null to
+ * skip the decode).  Frames of one session run in submission order.  The
+ * planes must stay valid until the frame completes. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_submit(pyrowave_hip_server server, uint32_t session, const pyrowave_hip_frame *frame,
+                           size_t maximum_bitstream_size, const pyrowave_hip_decode_target *target,
+                           uint64_t *ticket);
+
+/* Returns up to `max` finished frames (oldest first); waits up to timeout_us
+ * for the first one. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_poll(pyrowave_hip_server server, pyrowave_hip_server_completion *completions, uint32_t max,
+                         uint32_t *count, uint32_t timeout_us);
+
+/* Copies a finished frame's packetized bitstream (and its packet table, two
+ * size_t per packet, if `packets` is not null) out and releases it. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_server_take_bitstream(pyrowave_hip_server server, uint64_t ticket, void *data, size_t size,
+                                   pyrowave_packet *packets, uint32_t max_packets);
+
 #ifdef __cplusplus
 }
 #endif

@@ -105,6 +105,17 @@ class HipEncodedFrame(C.Structure):
                 ("sequence", C.c_uint32)]
 
 
+class ServerCreateInfo(C.Structure):
+    _fields_ = [("device", C.c_void_p), ("hbm_budget_bytes", C.c_size_t), ("streams", C.c_uint32),
+                ("precision", C.c_int), ("packet_boundary", C.c_size_t), ("max_batch_frames", C.c_uint32)]
+
+
+class ServerCompletion(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("session", C.c_uint32), ("status", C.c_int), ("bytes", C.c_size_t),
+                ("packets", C.c_uint32), ("submit_ns", C.c_uint64), ("start_ns", C.c_uint64),
+                ("done_ns", C.c_uint64)]
+
+
 MESSAGE_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
 
 _lib = None
@@ -129,6 +140,9 @@ EXPORTS = (
     "pyrowave_hip_encoder_encode_coefficients", "pyrowave_hip_encoder_read_rate_control",
     "pyrowave_hip_decoder_read_coefficients", "pyrowave_hip_device_query_stage_times",
     "pyrowave_hip_device_set_timing", "pyrowave_hip_device_set_timing_stages",
+    "pyrowave_hip_server_create", "pyrowave_hip_server_destroy", "pyrowave_hip_server_session_footprint",
+    "pyrowave_hip_server_open_session", "pyrowave_hip_server_close_session", "pyrowave_hip_server_submit",
+    "pyrowave_hip_server_poll", "pyrowave_hip_server_take_bitstream",
 )
 
 
@@ -189,6 +203,15 @@ def load():
         "pyrowave_hip_device_query_stage_times": (I, [P, P, P, C.c_bool]),
         "pyrowave_hip_device_set_timing": (None, [P, C.c_bool]),
         "pyrowave_hip_device_set_timing_stages": (None, [P, C.c_uint32]),
+        "pyrowave_hip_server_create": (I, [C.POINTER(ServerCreateInfo), C.POINTER(P)]),
+        "pyrowave_hip_server_destroy": (None, [P]),
+        "pyrowave_hip_server_session_footprint": (I, [P, I, I, I, C.POINTER(SZ)]),
+        "pyrowave_hip_server_open_session": (I, [P, I, I, I, C.POINTER(U32)]),
+        "pyrowave_hip_server_close_session": (I, [P, U32]),
+        "pyrowave_hip_server_submit": (I, [P, U32, C.POINTER(HipFrame), SZ, C.POINTER(HipDecodeTarget),
+                                           C.POINTER(C.c_uint64)]),
+        "pyrowave_hip_server_poll": (I, [P, C.POINTER(ServerCompletion), U32, C.POINTER(U32), U32]),
+        "pyrowave_hip_server_take_bitstream": (I, [P, C.c_uint64, P, SZ, C.POINTER(Packet), U32]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -439,3 +462,67 @@ def decode_encoded_batch(decoders, encoders, targets, stream=None):
     ta = (HipDecodeTarget * n)(*targets)
     _check(load().pyrowave_hip_decode_encoded_batch(_handles(decoders), _handles(encoders), ta, n, stream),
            "decode_encoded_batch")
+
+
+class Server:
+    """Serving loop: sessions admitted by device footprint, frames batched
+    continuously over several streams (pyrowave_hip_server_* entry points)."""
+
+    def __init__(self, device, hbm_budget_bytes, streams=4, precision=PRECISION_FP16, packet_boundary=0,
+                 max_batch_frames=16):
+        info = ServerCreateInfo(device.handle, hbm_budget_bytes, streams, precision, packet_boundary,
+                                max_batch_frames)
+        h = C.c_void_p()
+        _check(load().pyrowave_hip_server_create(C.byref(info), C.byref(h)), "pyrowave_hip_server_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            load().pyrowave_hip_server_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def footprint(self, width, height, chroma):
+        n = C.c_size_t()
+        _check(load().pyrowave_hip_server_session_footprint(self.handle, width, height, chroma, C.byref(n)),
+               "session_footprint")
+        return n.value
+
+    def open_session(self, width, height, chroma):
+        """Returns the session id, or None when the footprint does not fit."""
+        sid = C.c_uint32()
+        r = load().pyrowave_hip_server_open_session(self.handle, width, height, chroma, C.byref(sid))
+        if r == ERROR_OUT_OF_DEVICE_MEMORY:
+            return None
+        _check(r, "open_session")
+        return sid.value
+
+    def close_session(self, session):
+        _check(load().pyrowave_hip_server_close_session(self.handle, session), "close_session")
+
+    def submit(self, session, frame, budget, target=None):
+        t = C.c_uint64()
+        _check(load().pyrowave_hip_server_submit(self.handle, session, C.byref(frame), budget,
+                                                 C.byref(target) if target is not None else None, C.byref(t)),
+               "submit")
+        return t.value
+
+    def poll(self, max_count=64, timeout_us=0):
+        arr = (ServerCompletion * max_count)()
+        n = C.c_uint32()
+        _check(load().pyrowave_hip_server_poll(self.handle, arr, max_count, C.byref(n), timeout_us), "poll")
+        return [arr[i] for i in range(n.value)]
+
+    def take_bitstream(self, completion):
+        """(bytes as uint8 array, [(offset, size)] packets) of a finished frame."""
+        import numpy as np
+        out = np.empty(max(1, completion.bytes), np.uint8)
+        pk = (Packet * max(1, completion.packets))()
+        _check(load().pyrowave_hip_server_take_bitstream(self.handle, completion.ticket, out.ctypes.data,
+                                                         out.size, pk, completion.packets), "take_bitstream")
+        return out[:completion.bytes], [(pk[i].offset, pk[i].size) for i in range(completion.packets)]

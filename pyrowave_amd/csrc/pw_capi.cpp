@@ -14,6 +14,78 @@
 
 using namespace pw;
 
+namespace pw
+{
+int target_words_of(size_t target_bytes)
+{
+	// reference pyrowave_encoder.cpp:290-291, 326: the sequence header is
+	// taken off the budget, the rest is counted in words.
+	if (target_bytes >= 8)
+		target_bytes -= 8;
+	return int(target_bytes / 4);
+}
+
+// Copies the encoder's meta (and rc words) and the first `words` bitstream
+// words to its pinned host mirrors on `stream`.
+bool queue_result_download(Slot &S, hipStream_t stream, uint32_t words)
+{
+	const size_t meta_bytes = (size_t(S.geom.count32) * 2 + kRcCount) * sizeof(uint32_t);
+	if (!S.hmeta.ptr && !S.hmeta.alloc(meta_bytes))
+		return false;
+	words = std::min(words, S.fs.capacity_words);
+	if (S.hbits.size < size_t(words) * 4)
+		if (!S.hbits.alloc(std::max<size_t>(size_t(words) * 4, 4096)))
+			return false;
+	uint32_t *hm = static_cast<uint32_t *>(S.hmeta.ptr);
+	(void)hipMemcpyAsync(hm, S.fs.meta, size_t(S.geom.count32) * 8, hipMemcpyDeviceToHost, stream);
+	(void)hipMemcpyAsync(hm + 2 * S.geom.count32, S.fs.rc, kRcCount * sizeof(uint32_t), hipMemcpyDeviceToHost,
+	                     stream);
+	if (words)
+		(void)hipMemcpyAsync(S.hbits.ptr, S.fs.bitstream, size_t(words) * 4, hipMemcpyDeviceToHost, stream);
+	S.hbits_words = words;
+	return true;
+}
+
+void set_encode_params(CallParams &F, Slot &S, size_t budget)
+{
+	S.sequence = (S.sequence + 1) & 7u;
+	S.last_budget = budget;
+	F.sequence = S.sequence;
+	F.target_words = target_words_of(budget & ~size_t(3));
+	F.flags = S.exact_prefix ? 1u : 0u;
+}
+
+void fill_encode_call(CallParams &F, Slot &S, const pyrowave_hip_frame &frame, size_t budget)
+{
+	for (int c = 0; c < 3; c++)
+	{
+		const pyrowave_hip_plane &p = frame.planes[c];
+		F.src[c] = static_cast<const uint8_t *>(p.data);
+		F.src_pitch[c] = int32_t(p.row_stride_in_bytes);
+		F.src_step[c] = int32_t(p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes : 1);
+	}
+	set_encode_params(F, S, budget);
+}
+
+void fill_decode_call(CallParams &F, Slot &S, const Slot *E, const pyrowave_hip_decode_target &target)
+{
+	if (E)
+	{
+		F.dec_offsets = E->fs.meta;
+		F.dec_offset_stride = 2;
+		F.dec_payload = E->fs.bitstream;
+		F.dec_payload_bytes = E->fs.capacity_words * 4u;
+	}
+	for (int c = 0; c < 3; c++)
+	{
+		F.dst[c] = static_cast<uint8_t *>(target.planes[c].data);
+		F.dst_pitch[c] = int32_t(target.planes[c].row_stride_in_bytes);
+	}
+	S.decoded_frame = true;
+}
+
+} // namespace pw
+
 namespace
 {
 std::mutex g_device_creation_lock;
@@ -88,15 +160,6 @@ void forget_slot(Device *dev, Slot *slot)
 		it->second.forget(slot);
 }
 
-int target_words_of(size_t target_bytes)
-{
-	// reference pyrowave_encoder.cpp:290-291, 326: the sequence header is
-	// taken off the budget, the rest is counted in words.
-	if (target_bytes >= 8)
-		target_bytes -= 8;
-	return int(target_bytes / 4);
-}
-
 bool valid_budget(const pyrowave_rate_control *rc)
 {
 	if (!rc)
@@ -105,35 +168,6 @@ bool valid_budget(const pyrowave_rate_control *rc)
 	return target != 0 && target <= UINT32_MAX;
 }
 
-void set_encode_params(CallParams &F, Slot &S, size_t budget)
-{
-	S.sequence = (S.sequence + 1) & 7u;
-	S.last_budget = budget;
-	F.sequence = S.sequence;
-	F.target_words = target_words_of(budget & ~size_t(3));
-	F.flags = S.exact_prefix ? 1u : 0u;
-}
-
-// Copies the encoder's meta (and rc words) and the first `words` bitstream
-// words to its pinned host mirrors on `stream`.
-bool queue_result_download(Slot &S, hipStream_t stream, uint32_t words)
-{
-	const size_t meta_bytes = (size_t(S.geom.count32) * 2 + kRcCount) * sizeof(uint32_t);
-	if (!S.hmeta.ptr && !S.hmeta.alloc(meta_bytes))
-		return false;
-	words = std::min(words, S.fs.capacity_words);
-	if (S.hbits.size < size_t(words) * 4)
-		if (!S.hbits.alloc(std::max<size_t>(size_t(words) * 4, 4096)))
-			return false;
-	uint32_t *hm = static_cast<uint32_t *>(S.hmeta.ptr);
-	(void)hipMemcpyAsync(hm, S.fs.meta, size_t(S.geom.count32) * 8, hipMemcpyDeviceToHost, stream);
-	(void)hipMemcpyAsync(hm + 2 * S.geom.count32, S.fs.rc, kRcCount * sizeof(uint32_t), hipMemcpyDeviceToHost,
-	                     stream);
-	if (words)
-		(void)hipMemcpyAsync(S.hbits.ptr, S.fs.bitstream, size_t(words) * 4, hipMemcpyDeviceToHost, stream);
-	S.hbits_words = words;
-	return true;
-}
 
 // Waits for the last encode and makes sure the host mirrors hold the whole
 // result.  Returns false if there is none or the device reported an error.
@@ -978,17 +1012,7 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encode_batch(pyrowave_encoder *
 	if (!P)
 		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
 	for (uint32_t i = 0; i < count; i++)
-	{
-		CallParams &F = P->call(int(i));
-		for (int c = 0; c < 3; c++)
-		{
-			const pyrowave_hip_plane &p = frames[i].planes[c];
-			F.src[c] = static_cast<const uint8_t *>(p.data);
-			F.src_pitch[c] = int32_t(p.row_stride_in_bytes);
-			F.src_step[c] = int32_t(p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes : 1);
-		}
-		set_encode_params(F, *slots[i], rate_controls[i].maximum_bitstream_size);
-	}
+		fill_encode_call(P->call(int(i)), *slots[i], frames[i], rate_controls[i].maximum_bitstream_size);
 	run_encode(*P, stream);
 	for (uint32_t i = 0; i < count; i++)
 	{
@@ -1051,22 +1075,9 @@ static pyrowave_result decode_batch_common(pyrowave_decoder *decoders, pyrowave_
 	for (uint32_t i = 0; i < count; i++)
 	{
 		CallParams &F = P->call(int(i));
-		if (encoders)
-		{
-			const Slot &E = encoders[i]->slot;
-			F.dec_offsets = E.fs.meta;
-			F.dec_offset_stride = 2;
-			F.dec_payload = E.fs.bitstream;
-			F.dec_payload_bytes = E.fs.capacity_words * 4u;
-		}
-		else if (!upload_packets(*slots[i], stream, F))
+		fill_decode_call(F, *slots[i], encoders ? &encoders[i]->slot : nullptr, targets[i]);
+		if (!encoders && !upload_packets(*slots[i], stream, F))
 			return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
-		for (int c = 0; c < 3; c++)
-		{
-			F.dst[c] = static_cast<uint8_t *>(targets[i].planes[c].data);
-			F.dst_pitch[c] = int32_t(targets[i].planes[c].row_stride_in_bytes);
-		}
-		slots[i]->decoded_frame = true;
 	}
 	run_decode(*P, stream);
 	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;

@@ -289,6 +289,37 @@ Device::~Device()
 // Slot
 // ---------------------------------------------------------------------------
 
+// Device memory of one slot: band storage, then (encoder only) the
+// rate-control arrays and the bitstream, each 256-byte aligned.
+SlotLayout slot_layout(const Geometry &G, bool encoder)
+{
+	SlotLayout L = {};
+	const size_t n32 = size_t(G.count32);
+	auto take = [&](size_t bytes) {
+		size_t off = L.total;
+		L.total += align_size(bytes, 256);
+		return off;
+	};
+	L.bands = take(G.store_bytes);
+	if (encoder)
+	{
+		L.words = take(n32 * 16 * sizeof(uint16_t));
+		L.bucket = take(n32 * 16);
+		L.partial_quant = take(n32 * sizeof(int32_t));
+		L.quant = take(n32);
+		L.excl = take(n32 * sizeof(uint32_t));
+		L.group_sum = take(size_t(G.ngroups) * sizeof(uint32_t));
+		L.meta = take(n32 * 2 * sizeof(uint32_t));
+		L.totals = take(kFlat * sizeof(uint32_t));
+		L.prefix = take(kFlat * sizeof(uint32_t));
+		L.cls = take(kFlat);
+		L.required = take(kFlat * sizeof(int32_t));
+		L.rc = take(kRcCount * sizeof(uint32_t));
+		L.bitstream = take(size_t(G.capacity_words()) * sizeof(uint32_t));
+	}
+	return L;
+}
+
 bool Slot::init(Device *dev_, int w, int h, int chroma, int precision, bool encoder_)
 {
 	dev = dev_;
@@ -296,60 +327,30 @@ bool Slot::init(Device *dev_, int w, int h, int chroma, int precision, bool enco
 	if (!geom.init(w, h, chroma, precision))
 		return false;
 	const Geometry &G = geom;
-	const size_t n32 = size_t(G.count32);
-
-	struct Part
-	{
-		size_t off, bytes;
-	};
-	size_t total = 0;
-	auto take = [&](size_t bytes) {
-		Part p{total, bytes};
-		total += align_size(bytes, 256);
-		return p;
-	};
-	Part p_bands = take(G.store_bytes);
-	Part p_words{}, p_bucket{}, p_pq{}, p_quant{}, p_excl{}, p_gsum{}, p_meta{}, p_totals{}, p_prefix{}, p_cls{},
-	    p_req{}, p_rc{}, p_bits{};
-	if (encoder)
-	{
-		p_words = take(n32 * 16 * sizeof(uint16_t));
-		p_bucket = take(n32 * 16);
-		p_pq = take(n32 * sizeof(int32_t));
-		p_quant = take(n32);
-		p_excl = take(n32 * sizeof(uint32_t));
-		p_gsum = take(size_t(G.ngroups) * sizeof(uint32_t));
-		p_meta = take(n32 * 2 * sizeof(uint32_t));
-		p_totals = take(kFlat * sizeof(uint32_t));
-		p_prefix = take(kFlat * sizeof(uint32_t));
-		p_cls = take(kFlat);
-		p_req = take(kFlat * sizeof(int32_t));
-		p_rc = take(kRcCount * sizeof(uint32_t));
-		p_bits = take(size_t(G.capacity_words()) * sizeof(uint32_t));
-	}
-	if (!mem.alloc(total))
+	const SlotLayout L = slot_layout(G, encoder);
+	if (!mem.alloc(L.total))
 		return false;
-	if (hipMemset(mem.ptr, 0, total) != hipSuccess)
+	if (hipMemset(mem.ptr, 0, L.total) != hipSuccess)
 		return false;
 	uint8_t *base = static_cast<uint8_t *>(mem.ptr);
-	bands = base + p_bands.off;
+	bands = base + L.bands;
 
 	memset(&fs, 0, sizeof(fs));
 	if (encoder)
 	{
-		fs.words = reinterpret_cast<uint16_t *>(base + p_words.off);
-		fs.bucket = base + p_bucket.off;
-		fs.partial_quant = reinterpret_cast<int32_t *>(base + p_pq.off);
-		fs.quant = base + p_quant.off;
-		fs.excl = reinterpret_cast<uint32_t *>(base + p_excl.off);
-		fs.group_sum = reinterpret_cast<uint32_t *>(base + p_gsum.off);
-		fs.meta = reinterpret_cast<uint32_t *>(base + p_meta.off);
-		fs.totals = reinterpret_cast<uint32_t *>(base + p_totals.off);
-		fs.prefix = reinterpret_cast<uint32_t *>(base + p_prefix.off);
-		fs.cls = base + p_cls.off;
-		fs.required = reinterpret_cast<int32_t *>(base + p_req.off);
-		fs.rc = reinterpret_cast<uint32_t *>(base + p_rc.off);
-		fs.bitstream = reinterpret_cast<uint32_t *>(base + p_bits.off);
+		fs.words = reinterpret_cast<uint16_t *>(base + L.words);
+		fs.bucket = base + L.bucket;
+		fs.partial_quant = reinterpret_cast<int32_t *>(base + L.partial_quant);
+		fs.quant = base + L.quant;
+		fs.excl = reinterpret_cast<uint32_t *>(base + L.excl);
+		fs.group_sum = reinterpret_cast<uint32_t *>(base + L.group_sum);
+		fs.meta = reinterpret_cast<uint32_t *>(base + L.meta);
+		fs.totals = reinterpret_cast<uint32_t *>(base + L.totals);
+		fs.prefix = reinterpret_cast<uint32_t *>(base + L.prefix);
+		fs.cls = base + L.cls;
+		fs.required = reinterpret_cast<int32_t *>(base + L.required);
+		fs.rc = reinterpret_cast<uint32_t *>(base + L.rc);
+		fs.bitstream = reinterpret_cast<uint32_t *>(base + L.bitstream);
 		fs.capacity_words = G.capacity_words();
 	}
 	fs.n32 = G.count32;
@@ -365,7 +366,7 @@ bool Slot::init(Device *dev_, int w, int h, int chroma, int precision, bool enco
 	if (hipEventCreateWithFlags(&result_ready, hipEventDisableTiming) != hipSuccess)
 		return false;
 
-	offsets.assign(n32, UINT32_MAX);
+	offsets.assign(size_t(G.count32), UINT32_MAX);
 	clear_packets();
 	return true;
 }
@@ -485,7 +486,7 @@ bool Slot::decode_is_ready(bool allow_partial) const
 // Plan
 // ---------------------------------------------------------------------------
 
-bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
+bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *storage, hipStream_t stream)
 {
 	slots = slots_;
 	encoder = encoder_;
@@ -649,13 +650,30 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
 	size_t o_fbb = take(sizeof(int32_t) * n);
 	size_t o_grp = take(sizeof(uint32_t) * grp_frame.size());
 	size_t o_fgb = take(sizeof(int32_t) * n);
-	if (!mem.alloc(total))
-		return false;
-	uint8_t *base = static_cast<uint8_t *>(mem.ptr);
-	std::vector<uint8_t> host(total, 0);
+	uint8_t *base = nullptr, *host = nullptr;
+	std::vector<uint8_t> host_vec;
+	if (storage)
+	{
+		// Grow by half again so a stream of slightly larger batches does not
+		// reallocate every time.
+		if (storage->dev.size < total && !storage->dev.alloc(total + total / 2))
+			return false;
+		if (storage->host.size < total && !storage->host.alloc(total + total / 2))
+			return false;
+		base = static_cast<uint8_t *>(storage->dev.ptr);
+		host = static_cast<uint8_t *>(storage->host.ptr);
+	}
+	else
+	{
+		if (!mem.alloc(total))
+			return false;
+		base = static_cast<uint8_t *>(mem.ptr);
+		host_vec.assign(total, 0);
+		host = host_vec.data();
+	}
 	auto put = [&](size_t off, const void *src, size_t bytes) {
 		if (bytes)
-			memcpy(host.data() + off, src, bytes);
+			memcpy(host + off, src, bytes);
 	};
 	for (int l = 0; l < kLevels; l++)
 	{
@@ -670,7 +688,12 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_)
 	put(o_grp, grp_frame.data(), sizeof(uint32_t) * grp_frame.size());
 	put(o_fgb, frame_group_base.data(), sizeof(int32_t) * n);
 	put(o_frames, host_frames.data(), sizeof(FrameState) * n);
-	if (hipMemcpy(base, host.data(), total, hipMemcpyHostToDevice) != hipSuccess)
+	if (storage)
+	{
+		if (hipMemcpyAsync(base, host, total, hipMemcpyHostToDevice, stream) != hipSuccess)
+			return false;
+	}
+	else if (hipMemcpy(base, host, total, hipMemcpyHostToDevice) != hipSuccess)
 		return false;
 	d_frames = reinterpret_cast<FrameState *>(base + o_frames);
 	d_bands = reinterpret_cast<BandJob *>(base + o_bands);

@@ -104,6 +104,14 @@ struct Device
 
 struct Plan;
 
+// Byte offsets of a slot's parts inside its one device allocation.
+struct SlotLayout
+{
+	size_t bands, words, bucket, partial_quant, quant, excl, group_sum, meta, totals, prefix, cls, required, rc,
+	    bitstream, total;
+};
+SlotLayout slot_layout(const Geometry &G, bool encoder);
+
 // One frame slot: everything one encode or decode of a frame needs on the
 // device, plus the host mirrors of the CPU-buffer paths.
 struct Slot
@@ -154,6 +162,15 @@ struct Slot
 	bool decode_is_ready(bool allow_partial) const;
 };
 
+// Reusable memory for launch tables: a caller that builds a plan per batch
+// (a serving loop) keeps one of these per stream instead of allocating.  The
+// previous plan built into it must have finished on the device.
+struct PlanStorage
+{
+	DeviceBuffer dev;
+	HostBuffer host;
+};
+
 // Launch tables for a list of slots of one kind; cached per composition.
 struct Plan
 {
@@ -176,7 +193,10 @@ struct Plan
 	bool any_fp16 = false;
 	bool u8_only[kLevels] = {};  // every forward job of the level reads a u8 plane
 
-	bool build(const std::vector<Slot *> &slots, bool encoder);
+	// Without storage the plan owns its tables (synchronous upload); with
+	// storage they go into it, uploaded on `stream`.
+	bool build(const std::vector<Slot *> &slots, bool encoder, PlanStorage *storage = nullptr,
+	           hipStream_t stream = nullptr);
 	// Writes the per-call fields (call(i)) into the device FrameState array.
 	FrameState *stage_frames(hipStream_t stream);
 	CallParams &call(int i) { return params[i]; }
@@ -191,6 +211,16 @@ float decode_quant(uint8_t code);
 
 void run_encode(Plan &plan, hipStream_t stream);
 void run_decode(Plan &plan, hipStream_t stream);
+
+// Budget in bytes -> words of coded blocks (the sequence header comes off).
+int target_words_of(size_t target_bytes);
+// Per-call fields of an encode (advances the slot's 3-bit sequence counter)
+// and of a decode that reads an encoder slot's device result.
+void fill_encode_call(CallParams &F, Slot &S, const pyrowave_hip_frame &frame, size_t budget);
+void fill_decode_call(CallParams &F, Slot &S, const Slot *encoder, const pyrowave_hip_decode_target &target);
+// Copies an encoder slot's meta, rc words and first `words` bitstream words to
+// its pinned host mirrors on `stream`.
+bool queue_result_download(Slot &S, hipStream_t stream, uint32_t words);
 } // namespace pw
 
 struct pyrowave_device_opaque
