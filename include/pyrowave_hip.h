@@ -130,6 +130,65 @@ PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_decode_encoded_batch(pyrowave_decoder *decoders, pyrowave_encoder *encoders,
                                   const pyrowave_hip_decode_target *targets, uint32_t count, void *hip_stream);
 
+/* ---- Packets on the device ------------------------------------------------
+ *
+ * The reference packetizes on the host after a fence wait
+ * (pyrowave_encoder.cpp:596-623, 1048-1115, reached through
+ * pyrowave_encoder_compute_num_packets / pyrowave_encoder_packetize) and
+ * parses packets on the host (pyrowave_decoder.cpp:126-262, reached through
+ * pyrowave_decoder_push_packet).  These entry points do both on the device,
+ * so a serving loop never walks or copies a bitstream on the host. */
+
+/* A frame's packets in device memory. */
+typedef struct pyrowave_hip_packet_stream
+{
+	const void *data;        /* packets back to back, 4-byte aligned */
+	size_t data_bytes;       /* readable bytes at data (below 4 GiB) */
+	const uint32_t *packets; /* {offset, size} in bytes per packet, in push order */
+	const uint32_t *count;   /* one device word: the number of packets */
+	uint32_t max_packets;    /* entries the packet table holds */
+} pyrowave_hip_packet_stream;
+
+/* After each encoder's last encode (same stream): writes the sequence header
+ * in front of the coded blocks, so header + blocks form one contiguous byte
+ * stream, and the packet table for packet_boundary, with the split rule of
+ * pyrowave_encoder_packetize.  Asynchronous. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_packetize_batch(pyrowave_encoder *encoders, uint32_t count, size_t packet_boundary,
+                                     void *hip_stream);
+
+/* Device pointers of the encoder's packetized stream: `packets` describes it
+ * (data = the stream, count = the packet count word); *stream_bytes (if not
+ * null) receives the device word holding the stream's length in bytes. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_get_packets(pyrowave_encoder encoder, pyrowave_hip_packet_stream *packets,
+                                 const uint32_t **stream_bytes);
+
+/* Waits for the encoder's last device packetize and copies the stream (if
+ * `data` is not null and `size` holds it) and the packet table (if `packets`
+ * is not null and max_packets holds it) to the host; *bytes and *num_packets
+ * receive the sizes. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_encoder_read_packets(pyrowave_encoder encoder, void *data, size_t size, pyrowave_packet *packets,
+                                  uint32_t max_packets, size_t *bytes, uint32_t *num_packets);
+
+/* Parses each decoder's packets on the device and decodes the frame into the
+ * target.  Equivalent to pyrowave_decoder_clear then
+ * pyrowave_decoder_push_packet of every packet in order, for packets that all
+ * carry one sequence number; a packet that push_packet would reject, a second
+ * sequence number, or a block shorter than 8 bytes (even a duplicate) is a
+ * parse error, reported by pyrowave_hip_decoder_parse_result.  Missing blocks
+ * decode as zero.  Asynchronous. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decode_packets_batch(pyrowave_decoder *decoders, const pyrowave_hip_packet_stream *packets,
+                                  const pyrowave_hip_decode_target *targets, uint32_t count, void *hip_stream);
+
+/* Waits for the decoder's last device parse.  Blocks present and blocks the
+ * sequence header announced; PYROWAVE_ERROR_INVALID_ARGUMENT after a parse
+ * error, PYROWAVE_ERROR_GENERIC if there was no device parse. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decoder_parse_result(pyrowave_decoder decoder, uint32_t *blocks_present, uint32_t *blocks_announced);
+
 /* ---- Introspection (synchronous; for parity tests and tools) ---- */
 
 /* Number of floats of the flat coefficient layout for a frame geometry:

@@ -105,6 +105,11 @@ class HipEncodedFrame(C.Structure):
                 ("sequence", C.c_uint32)]
 
 
+class PacketStream(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("data_bytes", C.c_size_t), ("packets", C.c_void_p), ("count", C.c_void_p),
+                ("max_packets", C.c_uint32)]
+
+
 class ServerCreateInfo(C.Structure):
     _fields_ = [("device", C.c_void_p), ("hbm_budget_bytes", C.c_size_t), ("streams", C.c_uint32),
                 ("precision", C.c_int), ("packet_boundary", C.c_size_t), ("max_batch_frames", C.c_uint32)]
@@ -140,6 +145,8 @@ EXPORTS = (
     "pyrowave_hip_encoder_encode_coefficients", "pyrowave_hip_encoder_read_rate_control",
     "pyrowave_hip_decoder_read_coefficients", "pyrowave_hip_device_query_stage_times",
     "pyrowave_hip_device_set_timing", "pyrowave_hip_device_set_timing_stages",
+    "pyrowave_hip_encoder_packetize_batch", "pyrowave_hip_encoder_get_packets", "pyrowave_hip_decode_packets_batch",
+    "pyrowave_hip_decoder_parse_result", "pyrowave_hip_encoder_read_packets",
     "pyrowave_hip_server_create", "pyrowave_hip_server_destroy", "pyrowave_hip_server_session_footprint",
     "pyrowave_hip_server_open_session", "pyrowave_hip_server_close_session", "pyrowave_hip_server_submit",
     "pyrowave_hip_server_poll", "pyrowave_hip_server_take_bitstream",
@@ -203,6 +210,12 @@ def load():
         "pyrowave_hip_device_query_stage_times": (I, [P, P, P, C.c_bool]),
         "pyrowave_hip_device_set_timing": (None, [P, C.c_bool]),
         "pyrowave_hip_device_set_timing_stages": (None, [P, C.c_uint32]),
+        "pyrowave_hip_encoder_packetize_batch": (I, [P, U32, SZ, P]),
+        "pyrowave_hip_encoder_get_packets": (I, [P, C.POINTER(PacketStream), C.POINTER(P)]),
+        "pyrowave_hip_decode_packets_batch": (I, [P, P, P, U32, P]),
+        "pyrowave_hip_decoder_parse_result": (I, [P, C.POINTER(U32), C.POINTER(U32)]),
+        "pyrowave_hip_encoder_read_packets": (I, [P, P, SZ, C.POINTER(Packet), U32, C.POINTER(SZ),
+                                                  C.POINTER(U32)]),
         "pyrowave_hip_server_create": (I, [C.POINTER(ServerCreateInfo), C.POINTER(P)]),
         "pyrowave_hip_server_destroy": (None, [P]),
         "pyrowave_hip_server_session_footprint": (I, [P, I, I, I, C.POINTER(SZ)]),
@@ -462,6 +475,54 @@ def decode_encoded_batch(decoders, encoders, targets, stream=None):
     ta = (HipDecodeTarget * n)(*targets)
     _check(load().pyrowave_hip_decode_encoded_batch(_handles(decoders), _handles(encoders), ta, n, stream),
            "decode_encoded_batch")
+
+
+def packetize_batch(encoders, packet_boundary, stream=None):
+    """Device packetize after the encoders' last encode on `stream`."""
+    _check(load().pyrowave_hip_encoder_packetize_batch(_handles(encoders), len(encoders), packet_boundary, stream),
+           "packetize_batch")
+
+
+def device_packets(encoder):
+    """(PacketStream, device address of the stream-length word) of the
+    encoder's last device packetize."""
+    ps = PacketStream()
+    nbytes = C.c_void_p()
+    _check(load().pyrowave_hip_encoder_get_packets(encoder.handle, C.byref(ps), C.byref(nbytes)), "get_packets")
+    return ps, nbytes.value
+
+
+def read_packets(encoder):
+    """(stream bytes as uint8 array, [(offset, size)]) of the encoder's last
+    device packetize, copied to the host."""
+    import numpy as np
+    nb, npk = C.c_size_t(), C.c_uint32()
+    L = load()
+    _check(L.pyrowave_hip_encoder_read_packets(encoder.handle, None, 0, None, 0, C.byref(nb), C.byref(npk)),
+           "read_packets")
+    out = np.empty(max(1, nb.value), np.uint8)
+    pk = (Packet * max(1, npk.value))()
+    _check(L.pyrowave_hip_encoder_read_packets(encoder.handle, out.ctypes.data, out.size, pk, npk.value,
+                                               C.byref(nb), C.byref(npk)), "read_packets")
+    return out[:nb.value], [(pk[i].offset, pk[i].size) for i in range(npk.value)]
+
+
+def packet_stream(data_ptr, data_bytes, packets_ptr, count_ptr, max_packets):
+    return PacketStream(data_ptr, data_bytes, packets_ptr, count_ptr, max_packets)
+
+
+def decode_packets_batch(decoders, streams, targets, stream=None):
+    n = len(decoders)
+    pa = (PacketStream * n)(*streams)
+    ta = (HipDecodeTarget * n)(*targets)
+    _check(load().pyrowave_hip_decode_packets_batch(_handles(decoders), pa, ta, n, stream), "decode_packets_batch")
+
+
+def parse_result(decoder):
+    """(result code, blocks present, blocks announced) of the last device parse."""
+    a, b = C.c_uint32(), C.c_uint32()
+    r = load().pyrowave_hip_decoder_parse_result(decoder.handle, C.byref(a), C.byref(b))
+    return r, a.value, b.value
 
 
 class Server:

@@ -1100,6 +1100,186 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_encoded_batch(pyrowave_d
 	return decode_batch_common(decoders, encoders, targets, count, hip_stream);
 }
 
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encoder_packetize_batch(pyrowave_encoder *encoders, uint32_t count,
+                                                                         size_t packet_boundary, void *hip_stream)
+{
+	if (!encoders || count == 0)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	std::vector<PacketizeJob> jobs(count);
+	for (uint32_t i = 0; i < count; i++)
+	{
+		if (!encoders[i] || encoders[i]->device != encoders[0]->device || !encoders[i]->slot.have_result)
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
+		Slot &S = encoders[i]->slot;
+		const Geometry &G = S.geom;
+		PacketizeJob &J = jobs[i];
+		J.meta = S.fs.meta;
+		J.stream = S.fs.bitstream - 2;
+		J.table = S.dtable;
+		J.info = S.dpinfo;
+		J.n32 = uint32_t(G.count32);
+		// Sequence header (bitstream/bitstream.md:204-315), as the host
+		// packetize writes it.
+		J.h0 = uint32_t(G.width - 1) | (uint32_t(G.height - 1) << 14) | ((S.sequence & 7u) << 28) | (1u << 31);
+		J.h1 = uint32_t(G.chroma == PYROWAVE_CHROMA_SUBSAMPLING_444) << 26;
+		J.boundary = uint32_t(std::min<size_t>(packet_boundary, UINT32_MAX));
+	}
+	Device &D = *encoders[0]->slot.dev;
+	std::lock_guard<std::recursive_mutex> g(D.lock);
+	(void)hipSetDevice(D.hip_device);
+	hipStream_t stream = pick_stream(D, hip_stream);
+	launch_packetize(jobs.data(), int(count), stream);
+	for (uint32_t i = 0; i < count; i++)
+	{
+		Slot &S = encoders[i]->slot;
+		(void)hipEventRecord(S.result_ready, stream);
+		S.device_packets = true;
+	}
+	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
+}
+
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encoder_read_packets(pyrowave_encoder encoder, void *data, size_t size,
+                                                                      pyrowave_packet *packets, uint32_t max_packets,
+                                                                      size_t *bytes, uint32_t *num_packets)
+{
+	if (!encoder)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	Slot &S = encoder->slot;
+	if (!S.device_packets)
+		return PYROWAVE_ERROR_GENERIC;
+	(void)hipSetDevice(S.dev->hip_device);
+	if (hipEventSynchronize(S.result_ready) != hipSuccess)
+		return PYROWAVE_ERROR_GENERIC;
+	uint32_t info[2] = {};
+	if (hipMemcpy(info, S.dpinfo, sizeof(info), hipMemcpyDeviceToHost) != hipSuccess)
+		return PYROWAVE_ERROR_GENERIC;
+	if (bytes)
+		*bytes = info[0];
+	if (num_packets)
+		*num_packets = info[1];
+	if (data && size >= info[0] &&
+	    hipMemcpy(data, S.fs.bitstream - 2, info[0], hipMemcpyDeviceToHost) != hipSuccess)
+		return PYROWAVE_ERROR_GENERIC;
+	if (packets && max_packets >= info[1])
+	{
+		std::vector<uint32_t> t(size_t(info[1]) * 2);
+		if (hipMemcpy(t.data(), S.dtable, t.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+			return PYROWAVE_ERROR_GENERIC;
+		for (uint32_t i = 0; i < info[1]; i++)
+			packets[i] = {t[2 * i], t[2 * i + 1]};
+	}
+	return PYROWAVE_SUCCESS;
+}
+
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encoder_get_packets(pyrowave_encoder encoder,
+                                                                     pyrowave_hip_packet_stream *packets,
+                                                                     const uint32_t **stream_bytes)
+{
+	if (!encoder || !packets)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	const Slot &S = encoder->slot;
+	packets->data = S.fs.bitstream - 2;
+	packets->data_bytes = (size_t(S.fs.capacity_words) + 2) * 4;
+	packets->packets = S.dtable;
+	packets->count = S.dpinfo + 1;
+	packets->max_packets = uint32_t(S.geom.count32) + 1;
+	if (stream_bytes)
+		*stream_bytes = S.dpinfo;
+	return PYROWAVE_SUCCESS;
+}
+
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_batch(pyrowave_decoder *decoders,
+                                                                      const pyrowave_hip_packet_stream *packets,
+                                                                      const pyrowave_hip_decode_target *targets,
+                                                                      uint32_t count, void *hip_stream)
+{
+	if (!decoders || !packets || !targets || count == 0)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	std::vector<Slot *> slots(count);
+	for (uint32_t i = 0; i < count; i++)
+	{
+		if (!decoders[i] || decoders[i]->device != decoders[0]->device)
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
+		slots[i] = &decoders[i]->slot;
+		for (uint32_t j = 0; j < i; j++)
+			if (slots[j] == slots[i])
+				return PYROWAVE_ERROR_INVALID_ARGUMENT;
+		const Geometry &G = slots[i]->geom;
+		for (int c = 0; c < 3; c++)
+			if (!targets[i].planes[c].data || targets[i].planes[c].row_stride_in_bytes < size_t(G.plane_w(c)))
+				return PYROWAVE_ERROR_INVALID_ARGUMENT;
+		const pyrowave_hip_packet_stream &P = packets[i];
+		if (!P.data || (reinterpret_cast<uintptr_t>(P.data) & 3) || P.data_bytes > UINT32_MAX || !P.packets ||
+		    !P.count)
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	}
+	Device &D = *slots[0]->dev;
+	std::lock_guard<std::recursive_mutex> g(D.lock);
+	(void)hipSetDevice(D.hip_device);
+	hipStream_t stream = pick_stream(D, hip_stream);
+	std::vector<ParseJob> jobs(count);
+	for (uint32_t i = 0; i < count; i++)
+	{
+		Slot &S = *slots[i];
+		const Geometry &G = S.geom;
+		const size_t bytes = (size_t(G.count32) + 4) * 4;
+		if (S.dparse.size < bytes && !S.dparse.alloc(bytes))
+			return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
+		const pyrowave_hip_packet_stream &P = packets[i];
+		ParseJob &J = jobs[i];
+		J.data = static_cast<const uint32_t *>(P.data);
+		J.packets = P.packets;
+		J.count = P.count;
+		J.offsets = static_cast<uint32_t *>(S.dparse.ptr);
+		J.info = J.offsets + G.count32;
+		J.data_bytes = uint32_t(P.data_bytes);
+		J.max_packets = P.max_packets;
+		J.n32 = uint32_t(G.count32);
+		J.h0_size = uint32_t(G.width - 1) | (uint32_t(G.height - 1) << 14);
+		J.chroma = G.chroma == PYROWAVE_CHROMA_SUBSAMPLING_444 ? 1u : 0u;
+	}
+	Plan *Pl = get_plan(&D, slots, false);
+	if (!Pl)
+		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
+	launch_parse(jobs.data(), int(count), stream);
+	for (uint32_t i = 0; i < count; i++)
+	{
+		Slot &S = *slots[i];
+		(void)hipEventRecord(S.result_ready, stream);
+		S.parsed_on_device = true;
+		CallParams &F = Pl->call(int(i));
+		fill_decode_call(F, S, nullptr, targets[i]);
+		F.dec_offsets = jobs[i].offsets;
+		F.dec_offset_stride = 1;
+		F.dec_payload = jobs[i].data;
+		F.dec_payload_bytes = jobs[i].data_bytes;
+	}
+	run_decode(*Pl, stream);
+	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
+}
+
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decoder_parse_result(pyrowave_decoder decoder, uint32_t *blocks_present,
+                                                                      uint32_t *blocks_announced)
+{
+	if (!decoder)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	Slot &S = decoder->slot;
+	if (!S.parsed_on_device)
+		return PYROWAVE_ERROR_GENERIC;
+	(void)hipSetDevice(S.dev->hip_device);
+	if (hipEventSynchronize(S.result_ready) != hipSuccess)
+		return PYROWAVE_ERROR_GENERIC;
+	uint32_t info[4] = {};
+	const uint32_t *d = static_cast<const uint32_t *>(S.dparse.ptr) + S.geom.count32;
+	if (hipMemcpy(info, d, sizeof(info), hipMemcpyDeviceToHost) != hipSuccess)
+		return PYROWAVE_ERROR_GENERIC;
+	if (blocks_present)
+		*blocks_present = info[0];
+	if (blocks_announced)
+		*blocks_announced = info[1];
+	return info[2] ? PYROWAVE_ERROR_INVALID_ARGUMENT : PYROWAVE_SUCCESS;
+}
+
 PYROWAVE_PUBLIC_API size_t pyrowave_hip_coefficient_count(int width, int height, pyrowave_chroma_subsampling chroma)
 {
 	Geometry G;

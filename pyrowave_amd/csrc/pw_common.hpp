@@ -190,6 +190,60 @@ __host__ __device__ inline int xcd_swizzle(int b, int n, int nxcd)
 // Number of XCDs of the current device (1 where the runtime cannot tell).
 int xcd_count();
 
+// One frame of a device packetize launch (reference
+// pyrowave_encoder.cpp:596-623, 1048-1115): the sequence header goes in the
+// two words before the coded blocks, so header + blocks form one contiguous
+// byte stream; the packet table splits it at the boundary.
+struct PacketizeJob
+{
+	const uint32_t *meta; // [n32][2] {offset words, words}, block-index order
+	uint32_t *stream;     // header (2 words), then the coded blocks
+	uint32_t *table;      // [n32 + 1][2] {offset, size} in bytes
+	uint32_t *info;       // [0] stream bytes, [1] packets
+	uint32_t n32;
+	uint32_t h0;          // header word 0 (size, sequence, header bit)
+	uint32_t h1;          // header word 1 without the coded-block count
+	uint32_t boundary;    // packet boundary in bytes
+};
+constexpr int kPacketizeJobs = 16;
+struct PacketizeBlock
+{
+	PacketizeJob j[kPacketizeJobs];
+};
+
+// One frame of a device parse launch: the packets of one frame, in push order,
+// parsed into the block offset table the dequant kernel reads (the state
+// machine of reference pyrowave_decoder.cpp:126-262 for packets that all carry
+// one sequence number).
+struct ParseJob
+{
+	const uint32_t *data;    // packets back to back (4-byte aligned)
+	const uint32_t *packets; // [count][2] {offset, size} in bytes
+	const uint32_t *count;   // device word: number of packets
+	uint32_t *offsets;       // [n32] word offset of each block in data (UINT32_MAX: missing)
+	uint32_t *info;          // [0] blocks present, [1] blocks announced, [2] error bits, [3] sequence
+	uint32_t data_bytes;
+	uint32_t max_packets;
+	uint32_t n32;
+	uint32_t h0_size;        // expected header word 0, bits 0..27 (width - 1, height - 1)
+	uint32_t chroma;         // 1 for 4:4:4
+};
+constexpr int kParseJobs = 16;
+struct ParseBlock
+{
+	ParseJob j[kParseJobs];
+};
+
+// Error bits of ParseJob::info[2].
+enum ParseError : uint32_t
+{
+	kParseBadTable = 1u << 0,    // packet outside data, misaligned, or too many packets
+	kParseBadHeader = 1u << 1,   // sequence header for another size or chroma, reserved bits set
+	kParseBadBlock = 1u << 2,    // block index out of range, block shorter than 8 bytes or past its packet
+	kParseTrailing = 1u << 3,    // bytes left in a packet that hold no whole unit
+	kParseSequence = 1u << 4     // units of more than one sequence number
+};
+
 // Launch wrappers (defined in the .hip files).
 // u8_only: every job of the launch reads a u8 plane (smaller prefetch state).
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -209,5 +263,7 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
                     hipStream_t stream);
 void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream);
+void launch_packetize(const PacketizeJob *jobs, int count, hipStream_t stream);
+void launch_parse(const ParseJob *jobs, int count, hipStream_t stream);
 void upload_constants();
 } // namespace pw

@@ -315,7 +315,10 @@ SlotLayout slot_layout(const Geometry &G, bool encoder)
 		L.cls = take(kFlat);
 		L.required = take(kFlat * sizeof(int32_t));
 		L.rc = take(kRcCount * sizeof(uint32_t));
-		L.bitstream = take(size_t(G.capacity_words()) * sizeof(uint32_t));
+		// Two words in front of the blocks hold the sequence header of a
+		// device packetize.
+		L.bitstream = take((size_t(G.capacity_words()) + 2) * sizeof(uint32_t));
+		L.packets = take((2 * (n32 + 1) + 2) * sizeof(uint32_t));
 	}
 	return L;
 }
@@ -350,7 +353,9 @@ bool Slot::init(Device *dev_, int w, int h, int chroma, int precision, bool enco
 		fs.cls = base + L.cls;
 		fs.required = reinterpret_cast<int32_t *>(base + L.required);
 		fs.rc = reinterpret_cast<uint32_t *>(base + L.rc);
-		fs.bitstream = reinterpret_cast<uint32_t *>(base + L.bitstream);
+		fs.bitstream = reinterpret_cast<uint32_t *>(base + L.bitstream) + 2;
+		dtable = reinterpret_cast<uint32_t *>(base + L.packets);
+		dpinfo = dtable + 2 * (size_t(G.count32) + 1);
 		fs.capacity_words = G.capacity_words();
 	}
 	fs.n32 = G.count32;

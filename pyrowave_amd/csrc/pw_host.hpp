@@ -108,7 +108,7 @@ struct Plan;
 struct SlotLayout
 {
 	size_t bands, words, bucket, partial_quant, quant, excl, group_sum, meta, totals, prefix, cls, required, rc,
-	    bitstream, total;
+	    bitstream, packets, total;
 };
 SlotLayout slot_layout(const Geometry &G, bool encoder);
 
@@ -134,6 +134,12 @@ struct Slot
 	HostBuffer hmeta;        // n32 * 2 words + rc words
 	HostBuffer hbits;        // bitstream words
 	uint32_t hbits_words = 0;
+	// Device packetize output: the sequence header sits in the two words
+	// before fs.bitstream; table [n32 + 1][2] {offset, size}, then 2 info
+	// words {stream bytes, packets}.
+	uint32_t *dtable = nullptr;
+	uint32_t *dpinfo = nullptr;
+	bool device_packets = false; // a device packetize has been queued
 	hipEvent_t result_ready = nullptr;
 	bool have_result = false;
 	bool result_on_host = false;
@@ -147,6 +153,8 @@ struct Slot
 	int total_blocks = 0;
 	uint32_t last_seq = UINT32_MAX;
 	bool decoded_frame = false;
+	DeviceBuffer dparse;     // device parse: offsets [n32], then 4 info words
+	bool parsed_on_device = false;
 	DeviceBuffer dpayload;   // device payload (+16 B padding)
 	DeviceBuffer doffsets;
 	HostBuffer hpayload;     // pinned staging for payload + offsets
