@@ -401,6 +401,38 @@ def main():
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
                 "launches_timed": launches}
 
+    # Launch-bound case: one 1920x1080 4:2:0 frame per call (crop of the
+    # resident sequence), FP16, eager launches vs a replayed hipGraph.
+    def single_1080p(graphs, steps=60, warmup=5):
+        d2 = pw.Device()
+        d2.set_graphs(graphs)
+        d2.set_timing(False)
+        e = pw.Encoder(d2, 1920, 1080, pw.CHROMA_420, precision=pw.PRECISION_FP16)
+        dd = pw.Decoder(d2, 1920, 1080, pw.CHROMA_420, precision=pw.PRECISION_FP16)
+        out = [torch.empty((1080, 1920), dtype=torch.uint8, device="cuda"),
+               torch.empty((540, 960), dtype=torch.uint8, device="cuda"),
+               torch.empty((540, 960), dtype=torch.uint8, device="cuda")]
+        tg = [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in out])]
+        frs = [pw.hip_frame([(f[0].data_ptr(), f[0].stride(0), 1), (f[1].data_ptr(), f[1].stride(0), 1),
+                             (f[2].data_ptr(), f[2].stride(0), 1)]) for f in dev_frames[:8]]
+        for k in range(warmup):
+            pw.encode_batch([e], [frs[k % 8]], [400000], stream=sptr)
+            pw.decode_encoded_batch([dd], [e], tg, stream=sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(steps):
+            pw.encode_batch([e], [frs[k % 8]], [400000], stream=sptr)
+            pw.decode_encoded_batch([dd], [e], tg, stream=sptr)
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t) / steps * 1e6
+        e.close()
+        dd.close()
+        d2.close()
+        return round(us, 2)
+
+    single = {"eager": single_1080p(False), "graph": single_1080p(True)}
+    print(f"[bench] single-frame graphs done", file=sys.stderr, flush=True)
+
     serving = None
     if args.serving_seconds > 0:
         serving = serving_leg(pw, torch, device, dev_frames, args.serving_seconds)
@@ -453,6 +485,7 @@ def main():
         "stage_us_per_frame_fp32_single": stage_us32,
         "roofline": roofline("iDWT", "k_dwt_inverse"),
         "roofline_forward": roofline("DWT", "k_dwt_forward"),
+        "single_1080p_fp16_us_per_frame": single,
         "serving": serving,
         "cpu_baseline": cpu,
     }

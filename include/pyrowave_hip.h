@@ -130,6 +130,18 @@ PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_decode_encoded_batch(pyrowave_decoder *decoders, pyrowave_encoder *encoders,
                                   const pyrowave_hip_decode_target *targets, uint32_t count, void *hip_stream);
 
+/* Graph mode for the batch entry points (pyrowave_hip_encode_batch,
+ * pyrowave_hip_decode_batch, pyrowave_hip_decode_encoded_batch,
+ * pyrowave_hip_decode_packets_batch).  The first call for a batch composition
+ * (the same encoders or decoders in the same order) captures its launch
+ * sequence into a hipGraph on the caller's stream; later calls rewrite the
+ * graph's per-call parameter nodes (planes, budgets, sequence numbers) and
+ * launch it: one launch per batch instead of one per stage.  Stage timing is
+ * not recorded for graph launches.  Off by default.  No counterpart in the
+ * reference (Vulkan command buffers are re-recorded per call). */
+PYROWAVE_PUBLIC_API void
+pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
+
 /* ---- Packets on the device ------------------------------------------------
  *
  * The reference packetizes on the host after a fence wait
@@ -188,6 +200,27 @@ pyrowave_hip_decode_packets_batch(pyrowave_decoder *decoders, const pyrowave_hip
  * error, PYROWAVE_ERROR_GENERIC if there was no device parse. */
 PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_decoder_parse_result(pyrowave_decoder decoder, uint32_t *blocks_present, uint32_t *blocks_announced);
+
+/* ---- PSNR-HVS-M-H quality metric ----------------------------------------
+ *
+ * Reference psnr.cpp:126-264 with shaders/psnr_hvs_m.comp: 8x8 DCT blocks at a
+ * stride of 4 over a luma plane, masking effect, CSF tables for 16 viewing
+ * height factors 1 + i/8.  One launch covers all 16 factors (the reference
+ * dispatches once per factor). */
+
+/* Adds, per height factor, the frame's 24-bit fixed-point block errors to
+ * device_totals[16] (device memory, caller-cleared) and the frame's pixel
+ * count (blocks x 64) to *total_pixels.  Planes are u8 in device memory.
+ * Texels past the image edge read as 0.  Asynchronous. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_psnr_hvs_m_accumulate(pyrowave_device device, const void *reference, size_t reference_stride,
+                                   const void *distorted, size_t distorted_stride, int width, int height,
+                                   uint64_t *device_totals, uint64_t *total_pixels, void *hip_stream);
+
+/* PSNR-HVS-M-H in dB per height factor from totals copied to the host
+ * (full_range false: peak 223/255, studio swing). */
+PYROWAVE_PUBLIC_API void
+pyrowave_hip_psnr_hvs_m_db(const uint64_t *totals, uint64_t total_pixels, bool full_range, double *db);
 
 /* ---- Introspection (synchronous; for parity tests and tools) ---- */
 

@@ -100,6 +100,16 @@ def lib():
         L.or_dequant.argtypes = [C.POINTER(Layout), P, C.c_long, P, P]
         L.or_idwt_frame.argtypes = [C.POINTER(Layout), P, C.c_int, P, P]
         L.or_idwt_frame_f64.argtypes = [C.POINTER(Layout), P, P, P]
+        L.or_dct8x8.argtypes = [P, P]
+        L.or_maskeff.restype = C.c_float
+        L.or_maskeff.argtypes = [P, P, P]
+        L.or_psnr_hvs_m_tables.argtypes = [C.c_int, C.c_float, P, P, P]
+        L.or_psnr_height_factor.restype = C.c_float
+        L.or_psnr_height_factor.argtypes = [C.c_int]
+        L.or_psnr_hvs_m_totals.restype = C.c_uint64
+        L.or_psnr_hvs_m_totals.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_int, P]
+        L.or_psnr_hvs_m_db.restype = C.c_double
+        L.or_psnr_hvs_m_db.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
         _lib = L
     return _lib
 
@@ -325,6 +335,48 @@ def idwt_level(bands, f64=True):
 
 
 # --- the reference's own bitstream layer (oracle/_ref), for golden fixtures ---
+
+def dct8x8(sig):
+    """8x8 butterfly DCT of shaders/psnr_hvs_m.comp (float32)."""
+    sig = np.ascontiguousarray(sig, np.float32).reshape(64)
+    out = np.zeros(64, np.float32)
+    lib().or_dct8x8(ptr(sig), ptr(out))
+    return out.reshape(8, 8)
+
+
+def maskeff(sig, dct, mask):
+    sig = np.ascontiguousarray(sig, np.float32).reshape(64)
+    dct = np.ascontiguousarray(dct, np.float32).reshape(64)
+    mask = np.ascontiguousarray(mask, np.float32).reshape(64)
+    return float(lib().or_maskeff(ptr(sig), ptr(dct), ptr(mask)))
+
+
+def psnr_hvs_m_tables(height, factor):
+    """(csf, mask, inv_mask) 8x8 float32 tables of one height factor."""
+    t = [np.zeros(64, np.float32) for _ in range(3)]
+    lib().or_psnr_hvs_m_tables(height, factor, *[ptr(x) for x in t])
+    return [x.reshape(8, 8) for x in t]
+
+
+def psnr_height_factor(i):
+    return float(lib().or_psnr_height_factor(i))
+
+
+def psnr_hvs_m_totals(a, b, nfactors=16):
+    """Per height factor, the summed 24-bit fixed-point block errors of luma
+    planes a (reference) and b; and the total pixel count."""
+    a = np.ascontiguousarray(a, np.uint8)
+    b = np.ascontiguousarray(b, np.uint8)
+    assert a.shape == b.shape
+    h, w = a.shape
+    totals = np.zeros(16, np.uint64)
+    tp = lib().or_psnr_hvs_m_totals(ptr(a), w, ptr(b), w, w, h, nfactors, ptr(totals))
+    return totals[:nfactors], int(tp)
+
+
+def psnr_hvs_m_db(total, total_pixels, full_range=True):
+    return float(lib().or_psnr_hvs_m_db(int(total), int(total_pixels), 1 if full_range else 0))
+
 
 _ref = None
 

@@ -180,6 +180,15 @@ int or_idwt_frame(const or_layout *L, const float *coeffs, int precision,
                   uint8_t *const planes[3], const int pitches[3]);
 int or_idwt_frame_f64(const or_layout *L, const double *coeffs, double *const planes[3], const int pitches[3]);
 
+/* PSNR-HVS-M-H (psnr_hvs_m.c; reference shaders/psnr_hvs_m.comp, psnr.cpp). */
+void or_dct8x8(const float sig[64], float dct[64]);
+float or_maskeff(const float sig[64], const float dct[64], const float mask_coeffs[64]);
+void or_psnr_hvs_m_tables(int height, float height_factor, float csf[64], float mask[64], float inv_mask[64]);
+float or_psnr_height_factor(int index);
+uint64_t or_psnr_hvs_m_totals(const uint8_t *a, int pitch_a, const uint8_t *b, int pitch_b, int w, int h,
+                              int nfactors, uint64_t *totals);
+double or_psnr_hvs_m_db(uint64_t total, uint64_t total_pixels, int full_range);
+
 #ifdef __cplusplus
 }
 #endif

@@ -146,7 +146,8 @@ EXPORTS = (
     "pyrowave_hip_decoder_read_coefficients", "pyrowave_hip_device_query_stage_times",
     "pyrowave_hip_device_set_timing", "pyrowave_hip_device_set_timing_stages",
     "pyrowave_hip_encoder_packetize_batch", "pyrowave_hip_encoder_get_packets", "pyrowave_hip_decode_packets_batch",
-    "pyrowave_hip_decoder_parse_result", "pyrowave_hip_encoder_read_packets",
+    "pyrowave_hip_decoder_parse_result", "pyrowave_hip_encoder_read_packets", "pyrowave_hip_device_set_graphs",
+    "pyrowave_hip_psnr_hvs_m_accumulate", "pyrowave_hip_psnr_hvs_m_db",
     "pyrowave_hip_server_create", "pyrowave_hip_server_destroy", "pyrowave_hip_server_session_footprint",
     "pyrowave_hip_server_open_session", "pyrowave_hip_server_close_session", "pyrowave_hip_server_submit",
     "pyrowave_hip_server_poll", "pyrowave_hip_server_take_bitstream",
@@ -210,6 +211,9 @@ def load():
         "pyrowave_hip_device_query_stage_times": (I, [P, P, P, C.c_bool]),
         "pyrowave_hip_device_set_timing": (None, [P, C.c_bool]),
         "pyrowave_hip_device_set_timing_stages": (None, [P, C.c_uint32]),
+        "pyrowave_hip_device_set_graphs": (None, [P, C.c_bool]),
+        "pyrowave_hip_psnr_hvs_m_accumulate": (I, [P, P, SZ, P, SZ, I, I, P, C.POINTER(C.c_uint64), P]),
+        "pyrowave_hip_psnr_hvs_m_db": (None, [P, C.c_uint64, C.c_bool, P]),
         "pyrowave_hip_encoder_packetize_batch": (I, [P, U32, SZ, P]),
         "pyrowave_hip_encoder_get_packets": (I, [P, C.POINTER(PacketStream), C.POINTER(P)]),
         "pyrowave_hip_decode_packets_batch": (I, [P, P, P, U32, P]),
@@ -289,6 +293,10 @@ class Device:
 
     def set_timing(self, enable):
         load().pyrowave_hip_device_set_timing(self.handle, enable)
+
+    def set_graphs(self, enable):
+        """Batch entry points replay a captured hipGraph per batch composition."""
+        load().pyrowave_hip_device_set_graphs(self.handle, enable)
 
     def set_timing_stages(self, stages):
         """Times only the named stages (names as in STAGES)."""
@@ -523,6 +531,26 @@ def parse_result(decoder):
     a, b = C.c_uint32(), C.c_uint32()
     r = load().pyrowave_hip_decoder_parse_result(decoder.handle, C.byref(a), C.byref(b))
     return r, a.value, b.value
+
+
+def psnr_hvs_m_accumulate(device, ref_ptr, ref_stride, dist_ptr, dist_stride, width, height, totals_ptr,
+                          stream=None):
+    """Queues one frame's PSNR-HVS-M-H errors into 16 device uint64 totals;
+    returns the frame's pixel count."""
+    tp = C.c_uint64(0)
+    _check(load().pyrowave_hip_psnr_hvs_m_accumulate(device.handle, ref_ptr, ref_stride, dist_ptr, dist_stride,
+                                                     width, height, totals_ptr, C.byref(tp), stream),
+           "psnr_hvs_m_accumulate")
+    return tp.value
+
+
+def psnr_hvs_m_db(totals, total_pixels, full_range=True):
+    """dB per height factor from 16 host uint64 totals."""
+    import numpy as np
+    t = np.ascontiguousarray(totals, np.uint64)
+    out = np.zeros(16, np.float64)
+    load().pyrowave_hip_psnr_hvs_m_db(t.ctypes.data, total_pixels, full_range, out.ctypes.data)
+    return out
 
 
 class Server:

@@ -6,6 +6,8 @@
 // report PYROWAVE_ERROR_NOT_IMPLEMENTED.
 #include "pw_host.hpp"
 
+#include <cmath>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -1013,7 +1015,13 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encode_batch(pyrowave_encoder *
 		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
 	for (uint32_t i = 0; i < count; i++)
 		fill_encode_call(P->call(int(i)), *slots[i], frames[i], rate_controls[i].maximum_bitstream_size);
-	run_encode(*P, stream);
+	if (D.graphs)
+	{
+		if (!run_graph(*P, stream))
+			return PYROWAVE_ERROR_GENERIC;
+	}
+	else
+		run_encode(*P, stream);
 	for (uint32_t i = 0; i < count; i++)
 	{
 		(void)hipEventRecord(slots[i]->result_ready, stream);
@@ -1079,7 +1087,13 @@ static pyrowave_result decode_batch_common(pyrowave_decoder *decoders, pyrowave_
 		if (!encoders && !upload_packets(*slots[i], stream, F))
 			return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
 	}
-	run_decode(*P, stream);
+	if (D.graphs)
+	{
+		if (!run_graph(*P, stream))
+			return PYROWAVE_ERROR_GENERIC;
+	}
+	else
+		run_decode(*P, stream);
 	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
 }
 
@@ -1254,7 +1268,13 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_batch(pyrowave_d
 		F.dec_payload = jobs[i].data;
 		F.dec_payload_bytes = jobs[i].data_bytes;
 	}
-	run_decode(*Pl, stream);
+	if (D.graphs)
+	{
+		if (!run_graph(*Pl, stream))
+			return PYROWAVE_ERROR_GENERIC;
+	}
+	else
+		run_decode(*Pl, stream);
 	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
 }
 
@@ -1278,6 +1298,100 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decoder_parse_result(pyrowave_d
 	if (blocks_announced)
 		*blocks_announced = info[1];
 	return info[2] ? PYROWAVE_ERROR_INVALID_ARGUMENT : PYROWAVE_SUCCESS;
+}
+
+PYROWAVE_PUBLIC_API void pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable)
+{
+	if (!device)
+		return;
+	std::lock_guard<std::recursive_mutex> g(device->dev.lock);
+	device->dev.graphs = enable;
+}
+
+} // extern "C"
+
+namespace
+{
+// reference psnr.cpp:25-28
+float contrast_sensitivity(float cpd)
+{
+	return 2.6f * (0.0192f + 0.114f * cpd) * std::exp(-std::pow(0.114f * cpd, 1.1f));
+}
+
+// reference psnr.cpp:154-204, 321-324: csf, mask and inverse mask tables of
+// every height factor for an image `height` rows tall.
+void psnr_tables(int height, float *out)
+{
+	for (int f = 0; f < kPsnrFactors; f++)
+	{
+		float *csf = out + f * 192, *mask = csf + 64, *inv = csf + 128;
+		const float height_factor = 1.0f + float(f) / 8.0f;
+		const float nyquist_cpd = float(height) * height_factor * 3.14159265358979323846f / 360.0f;
+		for (int y = 0; y < 8; y++)
+			for (int x = 0; x < 8; x++)
+			{
+				const float H = (float(x) + 0.5f) / 8.0f, V = (float(y) + 0.5f) / 8.0f;
+				const float cpd = std::sqrt(H * H + V * V);
+				csf[8 * y + x] = 2.6f * contrast_sensitivity(cpd * nyquist_cpd);
+			}
+		float max_csf = 0.0f;
+		for (int i = 0; i < 64; i++)
+			max_csf = std::max(max_csf, csf[i]);
+		float norm = 1.0f / max_csf;
+		norm *= norm;
+		for (int i = 0; i < 64; i++)
+			mask[i] = csf[i] * csf[i] * norm;
+		mask[0] = 0.0f;
+		for (int i = 0; i < 64; i++)
+			inv[i] = i ? 1.0f / mask[i] : 0.0f;
+	}
+}
+} // namespace
+
+extern "C" {
+
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_psnr_hvs_m_accumulate(pyrowave_device device, const void *reference,
+                                                                       size_t reference_stride, const void *distorted,
+                                                                       size_t distorted_stride, int width, int height,
+                                                                       uint64_t *device_totals, uint64_t *total_pixels,
+                                                                       void *hip_stream)
+{
+	if (!device || !reference || !distorted || !device_totals || width < 1 || height < 1 || width > 16384 ||
+	    height > 16384 || reference_stride < size_t(width) || distorted_stride < size_t(width) ||
+	    reference_stride > UINT32_MAX || distorted_stride > UINT32_MAX)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	Device &D = device->dev;
+	std::lock_guard<std::recursive_mutex> g(D.lock);
+	(void)hipSetDevice(D.hip_device);
+	auto it = D.psnr_tables.find(height);
+	if (it == D.psnr_tables.end())
+	{
+		std::vector<float> t(size_t(kPsnrFactors) * 192);
+		psnr_tables(height, t.data());
+		std::unique_ptr<DeviceBuffer> buf(new DeviceBuffer);
+		if (!buf->alloc(t.size() * sizeof(float)))
+			return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
+		if (hipMemcpy(buf->ptr, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+			return PYROWAVE_ERROR_GENERIC;
+		it = D.psnr_tables.emplace(height, std::move(buf)).first;
+	}
+	launch_psnr_hvs_m(static_cast<const uint8_t *>(reference), uint32_t(reference_stride),
+	                  static_cast<const uint8_t *>(distorted), uint32_t(distorted_stride), width, height,
+	                  static_cast<const float *>(it->second->ptr), device_totals, pick_stream(D, hip_stream));
+	if (total_pixels)
+		*total_pixels += uint64_t((width - 1) / 4 + 1) * uint64_t((height - 1) / 4 + 1) * 64u;
+	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
+}
+
+PYROWAVE_PUBLIC_API void pyrowave_hip_psnr_hvs_m_db(const uint64_t *totals, uint64_t total_pixels, bool full_range,
+                                                    double *db)
+{
+	if (!totals || !db)
+		return;
+	// reference psnr.cpp:247-264
+	const double peak = full_range ? 1.0f * 1.0f : (223.0f * 223.0f) / (255.0f * 255.0f);
+	for (int i = 0; i < kPsnrFactors; i++)
+		db[i] = 10.0 * std::log10(double(total_pixels) * peak / std::ldexp(double(totals[i]), -24));
 }
 
 PYROWAVE_PUBLIC_API size_t pyrowave_hip_coefficient_count(int width, int height, pyrowave_chroma_subsampling chroma)

@@ -1090,6 +1090,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 }
 } // namespace
 
+const void *set_params_kernel()
+{
+	return reinterpret_cast<const void *>(&k_set_params);
+}
+
 void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream)
 {
 	for (int first = 0; first < nframes; first += kParamsPerLaunch)

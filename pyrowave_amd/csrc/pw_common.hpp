@@ -244,6 +244,10 @@ enum ParseError : uint32_t
 	kParseSequence = 1u << 4     // units of more than one sequence number
 };
 
+// PSNR-HVS-M-H: height factors 1 + i / 8, i < kPsnrFactors (reference
+// psnr.cpp:247, 321-324); tables per factor: csf, mask, inv_mask (64 each).
+constexpr int kPsnrFactors = 16;
+
 // Launch wrappers (defined in the .hip files).
 // u8_only: every job of the launch reads a u8 plane (smaller prefetch state).
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -263,7 +267,12 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
                     hipStream_t stream);
 void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream);
+// The per-call parameter kernel (identifies its nodes in a captured graph).
+const void *set_params_kernel();
 void launch_packetize(const PacketizeJob *jobs, int count, hipStream_t stream);
 void launch_parse(const ParseJob *jobs, int count, hipStream_t stream);
+// Adds each height factor's fixed-point block errors to totals[kPsnrFactors].
+void launch_psnr_hvs_m(const uint8_t *a, uint32_t pitch_a, const uint8_t *b, uint32_t pitch_b, int w, int h,
+                       const float *tables, uint64_t *totals, hipStream_t stream);
 void upload_constants();
 } // namespace pw

@@ -720,12 +720,15 @@ FrameState *Plan::stage_frames(hipStream_t stream)
 // Launch sequences
 // ---------------------------------------------------------------------------
 
-void run_encode(Plan &P, hipStream_t stream)
+void run_encode(Plan &P, hipStream_t stream, bool timed)
 {
 	Device &D = *P.slots[0]->dev;
 	const int n = int(P.slots.size());
 	FrameState *frames = P.stage_frames(stream);
 	hipEvent_t ev;
+	const uint32_t saved_timing = D.timing;
+	if (!timed)
+		D.timing = 0;
 
 	launch_rc_reset(frames, n, P.max_n32, stream);
 
@@ -747,14 +750,18 @@ void run_encode(Plan &P, hipStream_t stream)
 	D.begin_stage(stream, kStagePacking, n, &ev);
 	launch_pack(P.d_bands, P.d_blk_band, P.nblocks, frames, stream);
 	D.end_stage(stream, kStagePacking, n, ev);
+	D.timing = saved_timing;
 }
 
-void run_decode(Plan &P, hipStream_t stream)
+void run_decode(Plan &P, hipStream_t stream, bool timed)
 {
 	Device &D = *P.slots[0]->dev;
 	const int n = int(P.slots.size());
 	FrameState *frames = P.stage_frames(stream);
 	hipEvent_t ev;
+	const uint32_t saved_timing = D.timing;
+	if (!timed)
+		D.timing = 0;
 
 	D.begin_stage(stream, kStageDequant, n, &ev);
 	launch_dequant(P.d_bands, P.d_blk_band, P.nblocks, frames, stream);
@@ -764,5 +771,95 @@ void run_decode(Plan &P, hipStream_t stream)
 	for (int l = kLevels - 1; l >= 0; l--)
 		launch_dwt_inverse(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.any_fp16, stream);
 	D.end_stage(stream, kStageIDWT, n, ev);
+	D.timing = saved_timing;
+}
+
+Plan::~Plan()
+{
+	if (graph_exec)
+		(void)hipGraphExecDestroy(graph_exec);
+	if (graph)
+		(void)hipGraphDestroy(graph);
+}
+
+// Graph mode.  The first call captures the plan's launch sequence on the
+// caller's stream (no stage timing events, no memset nodes: every buffer the
+// sequence clears is cleared by a kernel) and keeps the k_set_params nodes;
+// each call then rewrites those nodes' arguments with the call's parameters
+// and launches the graph.
+bool run_graph(Plan &P, hipStream_t stream)
+{
+	if (!P.graph_exec)
+	{
+		if (hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+			return false;
+		if (P.encoder)
+			run_encode(P, stream, false);
+		else
+			run_decode(P, stream, false);
+		hipGraph_t g = nullptr;
+		if (hipStreamEndCapture(stream, &g) != hipSuccess || !g)
+			return false;
+		hipGraphExec_t e = nullptr;
+		if (hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess)
+		{
+			(void)hipGraphDestroy(g);
+			return false;
+		}
+		size_t n = 0;
+		(void)hipGraphGetNodes(g, nullptr, &n);
+		std::vector<hipGraphNode_t> nodes(n);
+		(void)hipGraphGetNodes(g, nodes.data(), &n);
+		P.param_nodes.clear();
+		P.param_node_base.clear();
+		P.param_first.clear();
+		P.param_count.clear();
+		for (hipGraphNode_t node : nodes)
+		{
+			hipGraphNodeType t;
+			if (hipGraphNodeGetType(node, &t) != hipSuccess || t != hipGraphNodeTypeKernel)
+				continue;
+			hipKernelNodeParams kp = {};
+			if (hipGraphKernelNodeGetParams(node, &kp) != hipSuccess || kp.func != set_params_kernel())
+				continue;
+			if (!kp.kernelParams)
+			{
+				(void)hipGraphExecDestroy(e);
+				(void)hipGraphDestroy(g);
+				return false;
+			}
+			P.param_nodes.push_back(node);
+			P.param_node_base.push_back(kp);
+			P.param_first.push_back(*static_cast<int *>(kp.kernelParams[2]));
+			P.param_count.push_back(*static_cast<int *>(kp.kernelParams[3]));
+		}
+		int covered = 0;
+		for (int c : P.param_count)
+			covered += c;
+		if (covered != int(P.params.size()))
+		{
+			(void)hipGraphExecDestroy(e);
+			(void)hipGraphDestroy(g);
+			return false;
+		}
+		P.graph = g;
+		P.graph_exec = e;
+	}
+	for (size_t k = 0; k < P.param_nodes.size(); k++)
+	{
+		ParamBlock block;
+		const int first = P.param_first[k], count = P.param_count[k];
+		for (int i = 0; i < count; i++)
+			block.p[i] = P.params[first + i];
+		FrameState *frames = P.d_frames;
+		int a_first = first, a_count = count;
+		void *args[4] = {&frames, &block, &a_first, &a_count};
+		hipKernelNodeParams kp = P.param_node_base[k];
+		kp.kernelParams = args;
+		kp.extra = nullptr;
+		if (hipGraphExecKernelNodeSetParams(P.graph_exec, P.param_nodes[k], &kp) != hipSuccess)
+			return false;
+	}
+	return hipGraphLaunch(P.graph_exec, stream) == hipSuccess;
 }
 } // namespace pw

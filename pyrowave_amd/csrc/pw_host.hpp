@@ -94,6 +94,9 @@ struct Device
 	std::vector<TimingRecord> pending;
 	std::vector<hipEvent_t> event_pool;
 	std::atomic<int> live_objects{0};
+	bool graphs = false; // batch entry points replay a captured graph per batch composition
+	// PSNR-HVS-M-H tables per image height (device copies).
+	std::map<int, std::unique_ptr<DeviceBuffer>> psnr_tables;
 
 	hipEvent_t get_event();
 	void begin_stage(hipStream_t s, int stage, int frames, hipEvent_t *start);
@@ -205,6 +208,15 @@ struct Plan
 	// storage they go into it, uploaded on `stream`.
 	bool build(const std::vector<Slot *> &slots, bool encoder, PlanStorage *storage = nullptr,
 	           hipStream_t stream = nullptr);
+
+	// Graph mode: the plan's launch sequence, captured once and replayed with
+	// the per-call parameter nodes updated.
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t graph_exec = nullptr;
+	std::vector<hipGraphNode_t> param_nodes;
+	std::vector<hipKernelNodeParams> param_node_base;
+	std::vector<int> param_first, param_count;
+	~Plan();
 	// Writes the per-call fields (call(i)) into the device FrameState array.
 	FrameState *stage_frames(hipStream_t stream);
 	CallParams &call(int i) { return params[i]; }
@@ -217,8 +229,11 @@ float rdo_distortion_scale(int chroma, int level, int component, int band);
 uint8_t encode_quant(float decoder_q_scale);
 float decode_quant(uint8_t code);
 
-void run_encode(Plan &plan, hipStream_t stream);
-void run_decode(Plan &plan, hipStream_t stream);
+// `timed`: record the device's per-stage timing events.
+void run_encode(Plan &plan, hipStream_t stream, bool timed = true);
+void run_decode(Plan &plan, hipStream_t stream, bool timed = true);
+// Replays the plan's captured launch sequence (capturing it on first use).
+bool run_graph(Plan &plan, hipStream_t stream);
 
 // Budget in bytes -> words of coded blocks (the sequence header comes off).
 int target_words_of(size_t target_bytes);

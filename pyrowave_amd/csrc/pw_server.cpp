@@ -53,6 +53,25 @@ struct Session
 	bool busy = false; // a frame of this session is on a lane
 };
 
+// Pinned host memory one frame's result is downloaded into: the encoder's
+// rate-control words (error bits), the packetize info words, the stream
+// (sequence header + blocks) and the packet table.
+struct ResultBuffer
+{
+	HostBuffer mem;
+	size_t stream_cap = 0; // stream bytes downloaded
+	size_t table_off = 0;
+	uint32_t max_packets = 0;
+	const uint32_t *rc() const { return static_cast<const uint32_t *>(mem.ptr); }
+	const uint32_t *info() const { return rc() + kRcCount; }
+	const uint8_t *stream() const { return static_cast<const uint8_t *>(mem.ptr) + kStreamOff; }
+	const uint32_t *table() const
+	{
+		return reinterpret_cast<const uint32_t *>(static_cast<const uint8_t *>(mem.ptr) + table_off);
+	}
+	static constexpr size_t kStreamOff = 64;
+};
+
 struct Lane
 {
 	hipStream_t stream = nullptr;
@@ -61,14 +80,15 @@ struct Lane
 	bool failed = false; // a table build or launch of the batch failed
 	std::vector<Request> batch;
 	std::vector<Session *> sessions;
+	std::vector<std::unique_ptr<ResultBuffer>> results;
 	PlanStorage enc_store, dec_store;
 	Plan enc_plan, dec_plan;
 };
 
 struct Finished
 {
-	std::vector<uint8_t> bytes;
-	std::vector<pyrowave_packet> packets;
+	std::unique_ptr<ResultBuffer> buf;
+	uint32_t bytes = 0, packets = 0;
 };
 
 bool frame_fits(const Geometry &G, const pyrowave_hip_frame &f)
@@ -122,10 +142,36 @@ struct pyrowave_hip_server_opaque
 	std::vector<std::unique_ptr<Lane>> lanes;
 	std::thread worker;
 
+	// Result buffers not in use (shared by the worker and take_bitstream,
+	// under `m`).
+	std::vector<std::unique_ptr<ResultBuffer>> pool;
+	std::unique_ptr<ResultBuffer> acquire(size_t bytes);
+
 	void run();
 	void launch(Lane &L);
 	void retire(Lane &L);
 };
+
+std::unique_ptr<ResultBuffer> pyrowave_hip_server_opaque::acquire(size_t bytes)
+{
+	{
+		std::lock_guard<std::mutex> g(m);
+		size_t best = pool.size();
+		for (size_t i = 0; i < pool.size(); i++)
+			if (pool[i]->mem.size >= bytes && (best == pool.size() || pool[i]->mem.size < pool[best]->mem.size))
+				best = i;
+		if (best != pool.size())
+		{
+			std::unique_ptr<ResultBuffer> b = std::move(pool[best]);
+			pool.erase(pool.begin() + long(best));
+			return b;
+		}
+	}
+	std::unique_ptr<ResultBuffer> b(new ResultBuffer);
+	if (!b->mem.alloc(bytes + bytes / 4 + 4096))
+		return nullptr;
+	return b;
+}
 
 // Worker: retire finished lanes, fill free lanes, sleep briefly while the
 // device works.  Only this thread touches lanes; sessions are shared with the
@@ -207,13 +253,60 @@ void pyrowave_hip_server_opaque::launch(Lane &L)
 		for (size_t i = 0; i < n; i++)
 			fill_encode_call(L.enc_plan.call(int(i)), *enc_slots[i], L.batch[i].frame, L.batch[i].budget);
 		run_encode(L.enc_plan, L.stream);
-		for (size_t i = 0; i < n; i++)
+		// Packetize on the device, then one download per part straight into
+		// the frame's result buffer: no host walk or copy of the blocks.
+		std::vector<PacketizeJob> jobs(n);
+		L.results.clear();
+		L.results.resize(n);
+		for (size_t i = 0; i < n && ok; i++)
 		{
 			Slot &S = *enc_slots[i];
-			(void)hipEventRecord(S.result_ready, L.stream);
+			const Geometry &G = S.geom;
 			S.have_result = true;
 			S.result_on_host = false;
-			ok = ok && queue_result_download(S, L.stream, download_words(L.batch[i].budget));
+			PacketizeJob &J = jobs[i];
+			J.meta = S.fs.meta;
+			J.stream = S.fs.bitstream - 2;
+			J.table = S.dtable;
+			J.info = S.dpinfo;
+			J.n32 = uint32_t(G.count32);
+			J.h0 = uint32_t(G.width - 1) | (uint32_t(G.height - 1) << 14) | ((S.sequence & 7u) << 28) | (1u << 31);
+			J.h1 = uint32_t(G.chroma == PYROWAVE_CHROMA_SUBSAMPLING_444) << 26;
+			J.boundary = uint32_t(std::min<size_t>(boundary, UINT32_MAX));
+			// Stream bytes are at most 8 + the budget's words; two adjacent
+			// packets together exceed the boundary, so the count is bounded.
+			const size_t stream_cap = std::min(size_t(download_words(L.batch[i].budget)) * 4 + 8,
+			                                   (size_t(S.fs.capacity_words) + 2) * 4);
+			const uint64_t by_size = 2 * uint64_t(stream_cap) / std::max<uint64_t>(J.boundary, 1) + 3;
+			const uint32_t maxpk = uint32_t(std::min<uint64_t>(by_size, uint64_t(G.count32) + 1));
+			const size_t table_off = ResultBuffer::kStreamOff + ((stream_cap + 63) & ~size_t(63));
+			std::unique_ptr<ResultBuffer> rb = acquire(table_off + size_t(maxpk) * 8);
+			if (!rb)
+			{
+				ok = false;
+				break;
+			}
+			rb->stream_cap = stream_cap;
+			rb->table_off = table_off;
+			rb->max_packets = maxpk;
+			L.results[i] = std::move(rb);
+		}
+		if (ok)
+		{
+			launch_packetize(jobs.data(), int(n), L.stream);
+			for (size_t i = 0; i < n; i++)
+			{
+				Slot &S = *enc_slots[i];
+				ResultBuffer &rb = *L.results[i];
+				uint8_t *h = static_cast<uint8_t *>(rb.mem.ptr);
+				(void)hipEventRecord(S.result_ready, L.stream);
+				(void)hipMemcpyAsync(h, S.fs.rc, kRcCount * 4, hipMemcpyDeviceToHost, L.stream);
+				(void)hipMemcpyAsync(h + kRcCount * 4, S.dpinfo, 8, hipMemcpyDeviceToHost, L.stream);
+				(void)hipMemcpyAsync(h + ResultBuffer::kStreamOff, S.fs.bitstream - 2, rb.stream_cap,
+				                     hipMemcpyDeviceToHost, L.stream);
+				(void)hipMemcpyAsync(h + rb.table_off, S.dtable, size_t(rb.max_packets) * 8, hipMemcpyDeviceToHost,
+				                     L.stream);
+			}
 		}
 	}
 	if (ok && !dec_slots.empty())
@@ -242,39 +335,35 @@ void pyrowave_hip_server_opaque::retire(Lane &L)
 	const uint64_t t1 = now_ns();
 	std::vector<pyrowave_hip_server_completion> done(L.batch.size());
 	std::vector<Finished> out(L.batch.size());
+	for (size_t i = 0; i < L.batch.size(); i++)
+	{
+		const Request &R = L.batch[i];
+		pyrowave_hip_server_completion &c = done[i];
+		c.ticket = R.ticket;
+		c.session = R.session;
+		c.submit_ns = R.submit_ns;
+		c.start_ns = R.start_ns;
+		c.done_ns = t1;
+		c.status = device_ok && i < L.results.size() && L.results[i] ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
+		if (c.status != PYROWAVE_SUCCESS)
+			continue;
+		ResultBuffer &rb = *L.results[i];
+		const uint32_t bytes = rb.info()[0], packets = rb.info()[1];
+		// Device error bits, or a result the download did not cover.
+		if (rb.rc()[kRcError] != 0 || bytes > rb.stream_cap || packets == 0 ||
+		    packets > rb.max_packets)
+		{
+			c.status = PYROWAVE_ERROR_GENERIC;
+			continue;
+		}
+		c.bytes = bytes;
+		c.packets = packets;
+		out[i].bytes = bytes;
+		out[i].packets = packets;
+		out[i].buf = std::move(L.results[i]);
+	}
 	{
 		std::lock_guard<std::recursive_mutex> g(dev->lock);
-		for (size_t i = 0; i < L.batch.size(); i++)
-		{
-			const Request &R = L.batch[i];
-			pyrowave_encoder enc = L.sessions[i]->enc;
-			enc->slot.result_on_host = device_ok;
-			pyrowave_hip_server_completion &c = done[i];
-			c.ticket = R.ticket;
-			c.session = R.session;
-			c.submit_ns = R.submit_ns;
-			c.start_ns = R.start_ns;
-			c.done_ns = t1;
-			c.status = device_ok ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
-			size_t np = 0;
-			if (c.status == PYROWAVE_SUCCESS)
-				c.status = pyrowave_encoder_compute_num_packets(enc, boundary, &np);
-			if (c.status == PYROWAVE_SUCCESS)
-			{
-				Finished &f = out[i];
-				f.packets.resize(np);
-				f.bytes.resize(size_t(download_words(R.budget)) * 4 + 8);
-				c.status = pyrowave_encoder_packetize(enc, f.packets.data(), boundary, &np, f.bytes.data(),
-				                                      f.bytes.size());
-				size_t total = 0;
-				for (size_t k = 0; k < np; k++)
-					total += f.packets[k].size;
-				f.bytes.resize(total);
-				f.packets.resize(np);
-				c.bytes = total;
-				c.packets = uint32_t(np);
-			}
-		}
 		dev->resolve_timings(false);
 	}
 	{
@@ -283,6 +372,8 @@ void pyrowave_hip_server_opaque::retire(Lane &L)
 		{
 			if (done[i].status == PYROWAVE_SUCCESS)
 				results.emplace(done[i].ticket, std::move(out[i]));
+			else if (i < L.results.size() && L.results[i])
+				pool.push_back(std::move(L.results[i]));
 			completions.push_back(done[i]);
 			L.sessions[i]->busy = false;
 		}
@@ -291,6 +382,7 @@ void pyrowave_hip_server_opaque::retire(Lane &L)
 	L.busy = false;
 	L.batch.clear();
 	L.sessions.clear();
+	L.results.clear();
 	done_cv.notify_all();
 }
 
@@ -503,13 +595,17 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_server_take_bitstream(pyrowave_
 	auto it = server->results.find(ticket);
 	if (it == server->results.end())
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	const Finished &f = it->second;
-	if (size < f.bytes.size() || (packets && max_packets < f.packets.size()))
+	Finished &f = it->second;
+	if (size < f.bytes || (packets && max_packets < f.packets))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	if (!f.bytes.empty())
-		memcpy(data, f.bytes.data(), f.bytes.size());
-	if (packets && !f.packets.empty())
-		memcpy(packets, f.packets.data(), f.packets.size() * sizeof(pyrowave_packet));
+	memcpy(data, f.buf->stream(), f.bytes);
+	if (packets)
+	{
+		const uint32_t *t = f.buf->table();
+		for (uint32_t i = 0; i < f.packets; i++)
+			packets[i] = {t[2 * i], t[2 * i + 1]};
+	}
+	server->pool.push_back(std::move(f.buf));
 	server->results.erase(it);
 	return PYROWAVE_SUCCESS;
 }
