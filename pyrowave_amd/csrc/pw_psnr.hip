@@ -16,7 +16,8 @@
 //
 // Arithmetic fixed here where the reference leaves it to the driver: texels
 // outside the image read as 0, texel = u8 / 255 correctly rounded, no FMA
-// contraction, correctly rounded sqrt.
+// contraction, correctly rounded sqrt (sqrtf: this HIP maps __fsqrt_rn to the
+// approximate native v_sqrt_f32).
 #include "pw_common.hpp"
 
 #pragma clang fp contract(off)
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(kPsnrThreads) void k_psnr_hvs_m(const uint8_t *__re
 				ma += sqr(da[i]) * mask[i];
 				mb += sqr(db[i]) * mask[i];
 			}
-			ma = __fsqrt_rn(ma * pop_a) / 32.0f;
-			mb = __fsqrt_rn(mb * pop_b) / 32.0f;
+			ma = sqrtf(ma * pop_a) / 32.0f;
+			mb = sqrtf(mb * pop_b) / 32.0f;
 			const float m = ma > mb ? ma : mb;
 			// shaders/psnr_hvs_m.comp:177-199
 			float err = 0.0f;
