@@ -3,9 +3,12 @@
 
 Workload (the headline configuration): a seeded 60-frame 3840x2160 4:2:0
 sequence (3 px/frame pan, 5 % of rectangles redrawn per frame), 1,600,000
-bytes per frame.  A step encodes and decodes one batch of frames that are
-already resident in HBM: pyrowave_hip_encode_batch, then
-pyrowave_hip_decode_encoded_batch straight from the encoders' device results.
+bytes per frame.  A step encodes and decodes three batches of 16 frames that are
+already resident in HBM, each on its own stream so the two overlap on the
+device: pyrowave_hip_encode_batch, then pyrowave_hip_decode_encoded_batch
+straight from the encoders' device results.  The transform rooflines and the
+per-stage times come from a separate pass with one batch on one stream, so
+kernel durations are not stretched by the overlap.
 
 Reported: µs per frame of the FP16-storage batched path (value), the plain
 FP32 single-frame path beside it, ΔPSNR between the two, the byte budget
@@ -39,6 +42,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--streams", type=int, default=3, help="batches in flight, one stream each")
     ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the oracle CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
@@ -92,7 +96,7 @@ def dwt_algorithmic_bytes(fp16):
     return total
 
 
-def trace_summary(path, top=40):
+def trace_summary(path, top=80):
     """Per-(kernel, workgroup count) dispatch statistics of a rocprofv3
     kernel-trace CSV.  The workgroup count separates launches of one kernel
     over different batch sizes and transform levels."""
@@ -136,6 +140,23 @@ def counter_table(paths):
         v = agg[key]
         print(f"{key[0]:20s} {key[1]:10d} " + " ".join(
             f"{sum(v[n]) / len(v[n]):22.4g}" if v[n] else f"{'-':>22s}" for n in names))
+    # Per-wave figures of each kernel's largest launch.
+    print()
+    largest = {}
+    for k, wgs in agg:
+        largest[k] = max(largest.get(k, 0), wgs)
+    for k in sorted(largest):
+        v = agg[(k, largest[k])]
+        waves = sum(v["SQ_WAVES"]) / len(v["SQ_WAVES"]) if v.get("SQ_WAVES") else 0
+        if not waves:
+            continue
+        parts = []
+        for n, label in (("SQ_INSTS_VALU", "VALU"), ("SQ_INSTS_LDS", "LDS"), ("SQ_INSTS_SALU", "SALU"),
+                         ("SQ_INSTS_VMEM", "VMEM"), ("SQ_LDS_BANK_CONFLICT", "LDS bank-conflict cycles"),
+                         ("SQ_WAIT_INST_LDS", "LDS wait cycles")):
+            if v.get(n):
+                parts.append(f"{sum(v[n]) / len(v[n]) / waves:.0f} {label}")
+        print(f"{k} ({largest[k]} workgroups): " + ", ".join(parts) + " per wave")
 
 
 def _counter_rows(path):
@@ -317,51 +338,66 @@ def main():
     def frame_desc(i):
         return pw.hip_frame([(t.data_ptr(), t.stride(0), 1) for t in dev_frames[i % FRAMES]])
 
-    def run_steps(encs, decs, tgts, steps, start, latencies=None):
-        B = len(encs)
+    def run_steps(groups, steps, start, latencies=None):
+        """Each step: every group (B encoders + decoders on its own stream)
+        encodes and decodes its next B frames."""
         for s in range(steps):
-            fr = [frame_desc(start + s * B + i) for i in range(B)]
-            if latencies is not None:
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record(stream)
-            pw.encode_batch(encs, fr, [BUDGET] * B, stream=sptr)
-            pw.decode_encoded_batch(decs, encs, tgts, stream=sptr)
-            if latencies is not None:
-                e1.record(stream)
-                latencies.append((e0, e1))
+            for g, (encs, decs, outs, tgts, st) in enumerate(groups):
+                B = len(encs)
+                fr = [frame_desc(start + (s * len(groups) + g) * B + i) for i in range(B)]
+                if latencies is not None:
+                    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    e0.record(st)
+                pw.encode_batch(encs, fr, [BUDGET] * B, stream=st.cuda_stream)
+                pw.decode_encoded_batch(decs, encs, tgts, stream=st.cuda_stream)
+                if latencies is not None:
+                    e1.record(st)
+                    latencies.append((e0, e1))
 
-    def timed(prec, B, steps, warmup):
-        encs, decs, outs, tgts = make(prec, B)
+    def timed(prec, B, steps, warmup, nstreams):
+        """Throughput with `nstreams` batches of B frames in flight on their own
+        streams; then, on one stream alone, the transform stage events for the
+        rooflines (one-stream throughput) and the per-stage times."""
+        groups = [make(prec, B) + ((stream if g == 0 else torch.cuda.Stream()),) for g in range(nstreams)]
         device.set_timing(False)
-        run_steps(encs, decs, tgts, warmup, 0)
+        run_steps(groups, warmup, 0)
         torch.cuda.synchronize()
-        # Timed region: HIP events on the launch stream around the forward and
-        # inverse transform stages only (four event records per step).
-        device.set_timing_stages(ROOF_STAGES)
-        device.stage_times(reset=True)
         lat = []
         t = time.perf_counter()
-        run_steps(encs, decs, tgts, steps, warmup * B, lat)
+        run_steps(groups, steps, warmup * B * nstreams, lat)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t
-        roof = device.stage_times(reset=True)
         lat_ms = np.array([a.elapsed_time(b) for a, b in lat])
+        # One stream: HIP events on the launch stream around the forward and
+        # inverse transform stages only (four event records per step).
+        one = groups[:1]
+        device.set_timing_stages(ROOF_STAGES)
+        device.stage_times(reset=True)
+        t = time.perf_counter()
+        run_steps(one, steps, 0)
+        torch.cuda.synchronize()
+        wall_one = time.perf_counter() - t
+        roof = device.stage_times(reset=True)
         # Stage times (events on the launch stream) over a separate pass.
         device.set_timing(True)
         device.stage_times(reset=True)
-        run_steps(encs, decs, tgts, max(2, steps // 3), 0)
+        run_steps(one, max(2, steps // 3), 0)
         torch.cuda.synchronize()
         stages = device.stage_times(reset=True)
         device.set_timing(False)
-        return dict(encs=encs, decs=decs, outs=outs, wall=wall, lat_ms=lat_ms, stages=stages, roof=roof, B=B)
+        encs, decs, outs = groups[0][0], groups[0][1], groups[0][2]
+        extra = [x for g in groups[1:] for x in g[0] + g[1]]
+        return dict(encs=encs, decs=decs, outs=outs, extra=extra, wall=wall, wall_one=wall_one, lat_ms=lat_ms,
+                    stages=stages, roof=roof, B=B, nstreams=nstreams)
 
     B = args.batch
-    fp16 = timed(pw.PRECISION_FP16, B, args.steps, args.warmup)
+    fp16 = timed(pw.PRECISION_FP16, B, args.steps, args.warmup, args.streams)
     print(f"[bench] fp16 batched done", file=sys.stderr, flush=True)
-    fp32 = timed(pw.PRECISION_FP32, 1, max(args.steps, 8), args.warmup)
+    fp32 = timed(pw.PRECISION_FP32, 1, max(args.steps, 8), args.warmup, 1)
     print(f"[bench] fp32 single done", file=sys.stderr, flush=True)
 
-    us_fp16 = fp16["wall"] / (args.steps * B) * 1e6
+    us_fp16 = fp16["wall"] / (args.steps * B * args.streams) * 1e6
+    us_fp16_one = fp16["wall_one"] / (args.steps * B) * 1e6
     us_fp32 = fp32["wall"] / (max(args.steps, 8) * 1) * 1e6
 
     # Quality and budget on one batch of frames: both paths decode frames 0..B-1.
@@ -464,15 +500,17 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(fp16["wall"] / args.steps * 1e3, 3),
+        "fp16_one_stream_us_per_frame": round(us_fp16_one, 2),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"3840x2160 4:2:0 60-frame seeded sequence, {BUDGET} B/frame, "
-                               f"batch {B} frames/step, encode+decode device-resident",
+                               f"{args.streams} batches of {B} frames in flight per step (one stream each), "
+                               f"encode+decode device-resident",
                    "precision": "fp16 band storage (levels 0-1 high bands), f32 lifting",
-                   "batch": B, "budget_bytes": BUDGET},
+                   "batch": B, "streams": args.streams, "budget_bytes": BUDGET},
         "fp32_single_us_per_frame": round(us_fp32, 2),
         "speedup_vs_fp32_single": round(us_fp32 / us_fp16, 3),
         "psnr_y_db": {"fp16": round(psnr["fp16"], 4), "fp32": round(psnr["fp32"], 4)},
@@ -491,10 +529,8 @@ def main():
     }
     print(json.dumps(line))
     for r in (fp16, fp32):
-        for e in r["encs"]:
-            e.close()
-        for d in r["decs"]:
-            d.close()
+        for x in r["encs"] + r["decs"] + r["extra"]:
+            x.close()
     device.close()
 
 
