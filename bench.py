@@ -360,6 +360,11 @@ def main():
         rooflines (one-stream throughput) and the per-stage times."""
         groups = [make(prec, B) + ((stream if g == 0 else torch.cuda.Stream()),) for g in range(nstreams)]
         device.set_timing(False)
+        # Transform tile shapes per level, timed on this batch (the other
+        # groups' plans start from the same choice: same level shapes).
+        encs0, decs0, _, tgts0, st0 = groups[0]
+        tuned = pw.autotune_batch(encs0, [frame_desc(i) for i in range(B)], [BUDGET] * B, decs0, tgts0,
+                                  repetitions=3, stream=st0.cuda_stream)
         run_steps(groups, warmup, 0)
         torch.cuda.synchronize()
         lat = []
@@ -388,7 +393,7 @@ def main():
         encs, decs, outs = groups[0][0], groups[0][1], groups[0][2]
         extra = [x for g in groups[1:] for x in g[0] + g[1]]
         return dict(encs=encs, decs=decs, outs=outs, extra=extra, wall=wall, wall_one=wall_one, lat_ms=lat_ms,
-                    stages=stages, roof=roof, B=B, nstreams=nstreams)
+                    stages=stages, roof=roof, B=B, nstreams=nstreams, tuned=tuned)
 
     B = args.batch
     fp16 = timed(pw.PRECISION_FP16, B, args.steps, args.warmup, args.streams)
@@ -501,6 +506,8 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": round(fp16["wall"] / args.steps * 1e3, 3),
         "fp16_one_stream_us_per_frame": round(us_fp16_one, 2),
+        "autotuned_tile_shapes": {"forward_levels_0_4": fp16["tuned"][0], "inverse_levels_0_4": fp16["tuned"][1],
+                                  "shapes": "forward 0: 128x56, 1: 64x56; inverse 0: 64x64, 1: 32x32"},
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

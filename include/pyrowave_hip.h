@@ -142,6 +142,28 @@ pyrowave_hip_decode_encoded_batch(pyrowave_decoder *decoders, pyrowave_encoder *
 PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
 
+/* Autotuner for the transform tile shapes (two per direction: forward 128x56
+ * or 64x56 samples per workgroup, inverse 64x64 or 32x32).  Encodes the batch
+ * (and, with decoders, decodes it from the encoders' results) like
+ * pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch, timing every
+ * shape of every transform level `repetitions` times (HIP events on the
+ * stream; synchronous) and keeping the fastest per level for this batch
+ * composition and for later batches of the same level shapes on the device.
+ * All shapes give the same coefficients bit for bit, so the call's results
+ * are those of a plain encode / decode.  The chosen shape index per level
+ * (0 = the larger tile) is written to forward_variants[5] / inverse_variants[5]
+ * when they are not null.  No counterpart in the reference. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decoders, const pyrowave_hip_frame *frames,
+                            const pyrowave_rate_control *rate_controls, const pyrowave_hip_decode_target *targets,
+                            uint32_t count, uint32_t repetitions, void *hip_stream, int *forward_variants,
+                            int *inverse_variants);
+
+/* Forces one transform tile shape (0 or 1) for every level on the device;
+ * -1 returns to the autotuned / default shapes.  For tests and comparisons. */
+PYROWAVE_PUBLIC_API void
+pyrowave_hip_device_force_tile_variant(pyrowave_device device, int variant);
+
 /* ---- Packets on the device ------------------------------------------------
  *
  * The reference packetizes on the host after a fence wait

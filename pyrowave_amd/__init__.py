@@ -147,7 +147,8 @@ EXPORTS = (
     "pyrowave_hip_device_set_timing", "pyrowave_hip_device_set_timing_stages",
     "pyrowave_hip_encoder_packetize_batch", "pyrowave_hip_encoder_get_packets", "pyrowave_hip_decode_packets_batch",
     "pyrowave_hip_decoder_parse_result", "pyrowave_hip_encoder_read_packets", "pyrowave_hip_device_set_graphs",
-    "pyrowave_hip_psnr_hvs_m_accumulate", "pyrowave_hip_psnr_hvs_m_db",
+    "pyrowave_hip_psnr_hvs_m_accumulate", "pyrowave_hip_psnr_hvs_m_db", "pyrowave_hip_autotune_batch",
+    "pyrowave_hip_device_force_tile_variant",
     "pyrowave_hip_server_create", "pyrowave_hip_server_destroy", "pyrowave_hip_server_session_footprint",
     "pyrowave_hip_server_open_session", "pyrowave_hip_server_close_session", "pyrowave_hip_server_submit",
     "pyrowave_hip_server_poll", "pyrowave_hip_server_take_bitstream",
@@ -212,6 +213,8 @@ def load():
         "pyrowave_hip_device_set_timing": (None, [P, C.c_bool]),
         "pyrowave_hip_device_set_timing_stages": (None, [P, C.c_uint32]),
         "pyrowave_hip_device_set_graphs": (None, [P, C.c_bool]),
+        "pyrowave_hip_autotune_batch": (I, [P, P, P, P, P, U32, U32, P, P, P]),
+        "pyrowave_hip_device_force_tile_variant": (None, [P, I]),
         "pyrowave_hip_psnr_hvs_m_accumulate": (I, [P, P, SZ, P, SZ, I, I, P, C.POINTER(C.c_uint64), P]),
         "pyrowave_hip_psnr_hvs_m_db": (None, [P, C.c_uint64, C.c_bool, P]),
         "pyrowave_hip_encoder_packetize_batch": (I, [P, U32, SZ, P]),
@@ -297,6 +300,10 @@ class Device:
     def set_graphs(self, enable):
         """Batch entry points replay a captured hipGraph per batch composition."""
         load().pyrowave_hip_device_set_graphs(self.handle, enable)
+
+    def force_tile_variant(self, variant):
+        """Every transform level uses tile shape `variant` (0 or 1); -1 = autotuned."""
+        load().pyrowave_hip_device_force_tile_variant(self.handle, variant)
 
     def set_timing_stages(self, stages):
         """Times only the named stages (names as in STAGES)."""
@@ -476,6 +483,19 @@ def decode_batch(decoders, targets, stream=None):
     n = len(decoders)
     ta = (HipDecodeTarget * n)(*targets)
     _check(load().pyrowave_hip_decode_batch(_handles(decoders), ta, n, stream), "decode_batch")
+
+
+def autotune_batch(encoders, frames, budgets, decoders=None, targets=None, repetitions=3, stream=None):
+    """Encodes (and decodes) the batch while timing every transform tile shape
+    per level; returns (forward variants, inverse variants) per level."""
+    n = len(encoders)
+    fa = (HipFrame * n)(*frames)
+    ra = (RateControl * n)(*[RateControl(b) for b in budgets])
+    ta = (HipDecodeTarget * n)(*targets) if targets is not None else None
+    fv, iv = (C.c_int * 5)(), (C.c_int * 5)()
+    _check(load().pyrowave_hip_autotune_batch(_handles(encoders), _handles(decoders) if decoders else None, fa, ra,
+                                              ta, n, repetitions, stream, fv, iv), "autotune_batch")
+    return list(fv), list(iv)
 
 
 def decode_encoded_batch(decoders, encoders, targets, stream=None):

@@ -1300,6 +1300,52 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decoder_parse_result(pyrowave_d
 	return info[2] ? PYROWAVE_ERROR_INVALID_ARGUMENT : PYROWAVE_SUCCESS;
 }
 
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decoders,
+                                                                const pyrowave_hip_frame *frames,
+                                                                const pyrowave_rate_control *rate_controls,
+                                                                const pyrowave_hip_decode_target *targets,
+                                                                uint32_t count, uint32_t repetitions, void *hip_stream,
+                                                                int *forward_variants, int *inverse_variants)
+{
+	if (!encoders || count == 0 || !encoders[0] || (decoders && !targets))
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	Device &D = encoders[0]->device->dev;
+	std::lock_guard<std::recursive_mutex> g(D.lock);
+	const bool graphs = D.graphs;
+	D.graphs = false;
+	D.tune_reps = int(std::max<uint32_t>(1u, std::min<uint32_t>(repetitions, 64u)));
+	pyrowave_result r = pyrowave_hip_encode_batch(encoders, frames, rate_controls, count, hip_stream);
+	if (r == PYROWAVE_SUCCESS && decoders)
+		r = pyrowave_hip_decode_encoded_batch(decoders, encoders, targets, count, hip_stream);
+	D.tune_reps = 0;
+	D.graphs = graphs;
+	if (r != PYROWAVE_SUCCESS)
+		return r;
+	std::vector<Slot *> enc_slots(count), dec_slots;
+	for (uint32_t i = 0; i < count; i++)
+		enc_slots[i] = &encoders[i]->slot;
+	if (Plan *P = get_plan(&D, enc_slots, true))
+		for (int l = 0; l < kLevels && forward_variants; l++)
+			forward_variants[l] = P->variant[l];
+	if (decoders)
+	{
+		for (uint32_t i = 0; i < count; i++)
+			dec_slots.push_back(&decoders[i]->slot);
+		if (Plan *P = get_plan(&D, dec_slots, false))
+			for (int l = 0; l < kLevels && inverse_variants; l++)
+				inverse_variants[l] = P->variant[l];
+	}
+	return PYROWAVE_SUCCESS;
+}
+
+PYROWAVE_PUBLIC_API void pyrowave_hip_device_force_tile_variant(pyrowave_device device, int variant)
+{
+	if (!device)
+		return;
+	std::lock_guard<std::recursive_mutex> g(device->dev.lock);
+	device->dev.forced_variant = variant >= 0 && variant < kDwtVariants ? variant : -1;
+}
+
 PYROWAVE_PUBLIC_API void pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable)
 {
 	if (!device)

@@ -20,17 +20,20 @@ constexpr int kSubdiv = 16;
 constexpr int kBuckets = 128;
 constexpr int kFlat = kBuckets * kSubdiv;
 
-// DWT tiles, one 256-thread workgroup each, with a 4-sample apron on every
-// side.  Forward: 128 x 56 samples of the level domain (136 x 64 staged), so
-// the horizontal pass is 64 rows x 4 windows of 40 and the vertical pass 128
-// columns x 2 windows of 36.  Inverse: 64 x 64 output samples (72 x 72
-// staged) in 16-sample windows.
+// DWT tiles, one workgroup each, with a 4-sample apron on every side.  Two
+// tile shapes per direction; the autotuner picks one per level (both give
+// the same coefficients, bit for bit).
+// Forward: W x 56 samples of the level domain ((W + 8) x 64 staged), so the
+// horizontal pass is 64 rows x W/32 windows of 40 and the vertical pass W
+// columns x 2 windows of 36: 2W threads.  Variant 0: W = 128 (256 threads),
+// variant 1: W = 64 (128 threads).
+// Inverse: T x T output samples ((T + 8)^2 staged) in 16-sample windows.
+// Variant 0: T = 64 (256 threads), variant 1: T = 32 (128 threads).
 constexpr int kApron = 4;
-constexpr int kFwdTileW = 128;
+constexpr int kDwtVariants = 2;
 constexpr int kFwdTileH = 56;
-constexpr int kInvTileW = 64;
-constexpr int kInvTileH = 64;
-constexpr int kDwtThreads = 256;
+constexpr int kFwdVariantTileW[kDwtVariants] = {128, 64};
+constexpr int kInvVariantTile[kDwtVariants] = {64, 32};
 
 // Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
 // sign bits, 8 B header and 3 B per 8x8 block, rounded up to words.
@@ -249,11 +252,10 @@ enum ParseError : uint32_t
 constexpr int kPsnrFactors = 16;
 
 // Launch wrappers (defined in the .hip files).
-// u8_only: every job of the launch reads a u8 plane (smaller prefetch state).
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                        bool u8_only, hipStream_t stream);
+                        int variant, hipStream_t stream);
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                        bool fp16, hipStream_t stream);
+                        int variant, hipStream_t stream);
 void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream);
 // blk_band: two words per block (band job index, bx | by << 16).
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,

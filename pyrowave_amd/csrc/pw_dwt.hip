@@ -7,16 +7,19 @@
 // shaders/idwt.comp:95-108).  A window of n samples yields exact outputs at
 // positions 4 .. n-5, so tiles reproduce the whole-line transform bit for bit.
 //
-// Forward: 128 x 56 domain samples per tile (136 x 64 staged).  Coalesced
-// 4-sample vector loads, then each thread lifts one long window in registers
-// (horizontal: 64 rows x 4 windows of 40 -> 32 outputs; vertical: 128 columns
-// x 2 windows of 36 -> 28 outputs), so window overlap costs 25-29 % extra
-// arithmetic; band stores are coalesced rows.
+// Forward: W x 56 domain samples per tile ((W + 8) x 64 staged), W = 128
+// (256 threads) or 64 (128 threads).  Coalesced 4-sample vector loads, then
+// each thread lifts one long window in registers (horizontal: 64 rows x W/32
+// windows of 40 -> 32 outputs; vertical: W columns x 2 windows of 36 -> 28
+// outputs), so window overlap costs 25-29 % extra arithmetic; band stores
+// are coalesced rows.
 //
-// Inverse: 64 x 64 output samples per tile (72 x 72 staged), 16-sample
-// windows.  The small tile keeps LDS (21.6 KB) and registers low, for seven
-// workgroups per CU: the inverse stages four bands per tile and needs the
-// extra waves in flight to cover its load latency.
+// Inverse: T x T output samples per tile ((T + 8)^2 staged), T = 64 (256
+// threads) or 32 (128 threads), 16-sample windows.  Small tiles keep LDS
+// (21.6 KB at T = 64) and registers low, for seven workgroups per CU: the
+// inverse stages four bands per tile and needs the extra waves in flight to
+// cover its load latency.  Both shapes of either direction give the same
+// coefficients bit for bit; the autotuner picks one per level.
 #include "pw_common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -35,25 +38,40 @@ constexpr float kDelta = float(0.443506852043971);
 constexpr float kK = float(1.230174104914001);
 constexpr float kInvK = float(1.0 / 1.230174104914001);
 
-// Forward tile.
-constexpr int kFwdInW = kFwdTileW + 2 * kApron;               // 136
-constexpr int kFwdInH = kFwdTileH + 2 * kApron;               // 64
-constexpr int kFwdPitch = kFwdInW + 1;                        // odd: row-per-lane reads are conflict-free
-constexpr int kFwdGroups = kFwdInW / 4;                       // 34 groups of 4 samples per row
-constexpr int kFwdSlots = kFwdInH * kFwdGroups;               // 2176
-constexpr int kFwdSlotIters = (kFwdSlots + kDwtThreads - 1) / kDwtThreads;
-constexpr int kFwdHWin = 40, kFwdHOut = 32;                   // horizontal window / outputs
-constexpr int kFwdVWin = 36, kFwdVOut = 28;                   // vertical window / outputs
-static_assert(kFwdInH * (kFwdTileW / kFwdHOut) == kDwtThreads, "one horizontal window per thread");
-static_assert(kFwdTileW * (kFwdTileH / kFwdVOut) == kDwtThreads, "one vertical window per thread");
+constexpr int kFwdHWin = 40, kFwdHOut = 32; // horizontal window / outputs
+constexpr int kFwdVWin = 36, kFwdVOut = 28; // vertical window / outputs
 
-// Inverse tile.
-static_assert(kInvTileW == kInvTileH, "square inverse tiles");
-constexpr int kInvTile = kInvTileW;                           // 64
-constexpr int kInvIn = kInvTile + 2 * kApron;                 // 72
-constexpr int kInvPitch = kInvIn + 1;                         // 73
-constexpr int kInvPass1Tasks = kInvIn * 8;                    // 72 lines x 8 windows of 16
-constexpr int kInvPass1Rounds = (kInvPass1Tasks + kDwtThreads - 1) / kDwtThreads;
+// Forward tile of W x 56 outputs, (W + 8) x 64 staged, 2W threads.
+template <int W>
+struct FwdTile
+{
+	static constexpr int kW = W, kH = kFwdTileH;
+	static constexpr int kThreads = 2 * W;
+	static constexpr int kInW = W + 2 * kApron, kInH = kH + 2 * kApron;
+	static constexpr int kPitch = kInW + 1; // odd: row-per-lane reads are conflict-free
+	static constexpr int kGroups = kInW / 4; // groups of 4 samples per row
+	static constexpr int kSlots = kInH * kGroups;
+	static constexpr int kSlotIters = (kSlots + kThreads - 1) / kThreads;
+	static_assert(kInH * (W / kFwdHOut) == kThreads, "one horizontal window per thread");
+	static_assert(W * (kH / kFwdVOut) == kThreads, "one vertical window per thread");
+	static_assert(kInH == 64, "one row per lane in the horizontal pass");
+};
+
+// Inverse tile of T x T outputs, (T + 8)^2 staged, 16-sample windows
+// (8 exact outputs each).
+template <int T, int THREADS>
+struct InvTile
+{
+	static constexpr int kTile = T, kThreads = THREADS;
+	static constexpr int kIn = T + 2 * kApron;
+	static constexpr int kPitch = kIn + 1;
+	static constexpr int kSegs = T / 8;                       // windows per line
+	static constexpr int kPass1Tasks = kIn * kSegs;           // vertical windows
+	static constexpr int kPass1Rounds = (kPass1Tasks + THREADS - 1) / THREADS;
+	static constexpr int kPass2Rounds = T * kSegs / THREADS;  // horizontal windows
+	static_assert(T * kSegs % THREADS == 0, "whole horizontal rounds");
+	static_assert(2 * kIn <= THREADS, "one thread per extension-map entry");
+};
 
 __constant__ float c_u8_dc[256]; // u8 / 255 - 0.5 (UNORM8 texel with the level-0 DC shift)
 
@@ -231,24 +249,24 @@ __device__ __forceinline__ void unpack4(const Raw4<T> &r, const float *lut, floa
 	}
 }
 
-// Stages the 136 x 64 source window in LDS.  Slot s covers row s / 34,
-// samples 4 * (s % 34) .. +3.  A group whose four samples lie inside the
-// valid source needs no extension and is one vector load; others gather
-// through the extension maps.  Offsets are 32-bit element offsets from the
-// uniform base (planes and bands stay below 4 GiB).  All loads are issued
+// Stages the (W + 8) x 64 source window in LDS.  Slot s covers row
+// s / groups, samples 4 * (s % groups) .. +3.  A group whose four samples lie
+// inside the valid source needs no extension and is one vector load; others
+// gather through the extension maps.  Offsets are 32-bit element offsets from
+// the uniform base (planes and bands stay below 4 GiB).  All loads are issued
 // before the first LDS store.
-template <typename T>
+template <typename S, typename T>
 __device__ __forceinline__ void fwd_stage(const T *src, uint32_t pitch, uint32_t step, bool vec_ok, int x0, int R_x,
                                           const int *ymap, const int *xmap, const float *lut, int tid, float *sA)
 {
-	Raw4<T> raw[kFwdSlotIters];
+	Raw4<T> raw[S::kSlotIters];
 #pragma unroll
-	for (int it = 0; it < kFwdSlotIters; it++)
+	for (int it = 0; it < S::kSlotIters; it++)
 	{
-		const int slot = tid + it * kDwtThreads;
-		if (it < kFwdSlotIters - 1 || slot < kFwdSlots)
+		const int slot = tid + it * S::kThreads;
+		if (it < S::kSlotIters - 1 || slot < S::kSlots)
 		{
-			const int r = slot / kFwdGroups, k = slot - r * kFwdGroups;
+			const int r = slot / S::kGroups, k = slot - r * S::kGroups;
 			const int gx = x0 - kApron + 4 * k;
 			const uint32_t roff = __umul24((uint32_t)ymap[r], pitch);
 			if (vec_ok && gx >= 0 && gx + 3 < R_x)
@@ -258,15 +276,15 @@ __device__ __forceinline__ void fwd_stage(const T *src, uint32_t pitch, uint32_t
 		}
 	}
 #pragma unroll
-	for (int it = 0; it < kFwdSlotIters; it++)
+	for (int it = 0; it < S::kSlotIters; it++)
 	{
-		const int slot = tid + it * kDwtThreads;
-		if (it < kFwdSlotIters - 1 || slot < kFwdSlots)
+		const int slot = tid + it * S::kThreads;
+		if (it < S::kSlotIters - 1 || slot < S::kSlots)
 		{
-			const int r = slot / kFwdGroups, k = slot - r * kFwdGroups;
+			const int r = slot / S::kGroups, k = slot - r * S::kGroups;
 			float f[4];
 			unpack4(raw[it], lut, f);
-			float *d = sA + r * kFwdPitch + 4 * k;
+			float *d = sA + r * S::kPitch + 4 * k;
 #pragma unroll
 			for (int i = 0; i < 4; i++)
 				d[i] = f[i];
@@ -274,13 +292,15 @@ __device__ __forceinline__ void fwd_stage(const T *src, uint32_t pitch, uint32_t
 	}
 }
 
-__global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__restrict__ jobs,
-                                                             const uint16_t *__restrict__ tile_job,
-                                                             const FrameState *__restrict__ frames, int nxcd)
+template <int W>
+__global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict__ jobs,
+                                                       const uint16_t *__restrict__ tile_job,
+                                                       const FrameState *__restrict__ frames, int nxcd)
 {
-	__shared__ float sA[kFwdInH * kFwdPitch];
-	__shared__ int s_xmap[kFwdInW];
-	__shared__ int s_ymap[kFwdInH];
+	using S = FwdTile<W>;
+	__shared__ float sA[S::kInH * S::kPitch];
+	__shared__ int s_xmap[S::kInW];
+	__shared__ int s_ymap[S::kInH];
 	__shared__ float s_lut[256];
 
 	const int tid = threadIdx.x;
@@ -288,14 +308,18 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 	// The job index is uniform: readfirstlane keeps J in scalar registers.
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
-	const int x0 = (local % J.tiles_x) * kFwdTileW;
-	const int y0 = (local / J.tiles_x) * kFwdTileH;
+	const int x0 = (local % J.tiles_x) * S::kW;
+	const int y0 = (local / J.tiles_x) * S::kH;
 
-	if (tid < kFwdInW)
-		s_xmap[tid] = fwd_ext(x0 - kApron + tid, J.R_x, J.A_x);
-	else if (tid < kFwdInW + kFwdInH)
-		s_ymap[tid - kFwdInW] = fwd_ext(y0 - kApron + tid - kFwdInW, J.R_y, J.A_y);
-	s_lut[tid] = c_u8_dc[tid];
+	for (int i = tid; i < S::kInW + S::kInH; i += S::kThreads)
+	{
+		if (i < S::kInW)
+			s_xmap[i] = fwd_ext(x0 - kApron + i, J.R_x, J.A_x);
+		else
+			s_ymap[i - S::kInW] = fwd_ext(y0 - kApron + i - S::kInW, J.R_y, J.A_y);
+	}
+	for (int i = tid; i < 256; i += S::kThreads)
+		s_lut[i] = c_u8_dc[i];
 	__syncthreads();
 
 	if (J.src_kind == kStoreU8)
@@ -304,14 +328,14 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 		const uint8_t *src = F.src[J.component];
 		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
 		const bool vec_ok = step == 1 && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)pitch) & 3) == 0;
-		fwd_stage(src, pitch, step, vec_ok, x0, J.R_x, s_ymap, s_xmap, s_lut, tid, sA);
+		fwd_stage<S>(src, pitch, step, vec_ok, x0, J.R_x, s_ymap, s_xmap, s_lut, tid, sA);
 	}
 	else if (J.src_kind == kStoreF16)
-		fwd_stage(static_cast<const __half *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
-		          s_lut, tid, sA);
+		fwd_stage<S>(static_cast<const __half *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
+		             s_lut, tid, sA);
 	else
-		fwd_stage(static_cast<const float *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
-		          s_lut, tid, sA);
+		fwd_stage<S>(static_cast<const float *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
+		             s_lut, tid, sA);
 	__syncthreads();
 
 	// Horizontal pass: wave w takes window w (outputs 32w .. 32w+31) of the
@@ -321,35 +345,36 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_forward(const DwtJob *__res
 		float v[kFwdHWin];
 #pragma unroll
 		for (int j = 0; j < kFwdHWin; j++)
-			v[j] = sA[row * kFwdPitch + seg * kFwdHOut + j];
+			v[j] = sA[row * S::kPitch + seg * kFwdHOut + j];
 		lift_fwd(v);
 		__syncthreads();
-		// Low outputs to columns 0..63, high outputs to 64..127.
+		// Low outputs to columns 0..W/2-1, high outputs to W/2..W-1.
 #pragma unroll
 		for (int k = 0; k < kFwdHOut / 2; k++)
 		{
-			sA[row * kFwdPitch + seg * (kFwdHOut / 2) + k] = v[4 + 2 * k] * kInvK;
-			sA[row * kFwdPitch + 64 + seg * (kFwdHOut / 2) + k] = v[5 + 2 * k] * kK;
+			sA[row * S::kPitch + seg * (kFwdHOut / 2) + k] = v[4 + 2 * k] * kInvK;
+			sA[row * S::kPitch + W / 2 + seg * (kFwdHOut / 2) + k] = v[5 + 2 * k] * kK;
 		}
 	}
 	__syncthreads();
 
-	// Vertical pass: lanes are consecutive columns; threads 0..127 take
-	// output rows 0..27, threads 128..255 rows 28..55; each store
-	// instruction writes one band row.
+	// Vertical pass: lanes are consecutive columns; threads 0..W-1 take
+	// output rows 0..27, threads W..2W-1 rows 28..55; each store instruction
+	// writes one band row.
 	{
-		const int col = tid & 127, seg = tid >> 7;
+		const int col = tid & (W - 1), seg = tid / W;
 		float v[kFwdVWin];
 #pragma unroll
 		for (int j = 0; j < kFwdVWin; j++)
-			v[j] = sA[(seg * kFwdVOut + j) * kFwdPitch + col];
+			v[j] = sA[(seg * kFwdVOut + j) * S::kPitch + col];
 		lift_fwd(v);
-		const int ox = (x0 >> 1) + (col & 63);
+		const bool low = col < W / 2;
+		const int ox = (x0 >> 1) + (col & (W / 2 - 1));
 		if (ox < J.bw)
 		{
-			void *lo = col < 64 ? J.band[0] : J.band[1];
-			void *hi = col < 64 ? J.band[2] : J.band[3];
-			const int lo_kind = col < 64 ? J.ll_kind : J.band_kind;
+			void *lo = low ? J.band[0] : J.band[1];
+			void *hi = low ? J.band[2] : J.band[3];
+			const int lo_kind = low ? J.ll_kind : J.band_kind;
 			const int oy0 = (y0 >> 1) + seg * (kFwdVOut / 2);
 #pragma unroll
 			for (int k = 0; k < kFwdVOut / 2; k++)
@@ -376,14 +401,14 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
-// Loads the four bands' 36x36 shares of the interleaved 72x72 tile (band b
-// holds the positions whose (row, column) parity is (b >> 1, b & 1)) into
-// LDS; every global load is issued before the first LDS store.
-template <typename TH>
+// Loads the four bands' (T/2 + 4)^2 shares of the interleaved (T + 8)^2 tile
+// (band b holds the positions whose (row, column) parity is (b >> 1, b & 1))
+// into LDS; every global load is issued before the first LDS store.
+template <typename S, typename TH>
 __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, const int *xmap, int tid, float *sA)
 {
-	constexpr int kQ = kInvIn / 2;
-	constexpr int kIters = (kQ * kQ + kDwtThreads - 1) / kDwtThreads;
+	constexpr int kQ = S::kIn / 2;
+	constexpr int kIters = (kQ * kQ + S::kThreads - 1) / S::kThreads;
 	// 32-bit element offsets from the uniform band bases (bands stay below
 	// 4 GiB), so loads use a scalar base and a 32-bit vector offset.
 	const uint32_t pitch = (uint32_t)J.band_pitch;
@@ -394,7 +419,7 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 #pragma unroll
 	for (int k = 0; k < kIters; k++)
 	{
-		const int i = tid + k * kDwtThreads;
+		const int i = tid + k * S::kThreads;
 #pragma unroll
 		for (int b = 0; b < 4; b++)
 			vals[b][k] = 0.0f;
@@ -422,60 +447,62 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 #pragma unroll
 	for (int k = 0; k < kIters; k++)
 	{
-		const int i = tid + k * kDwtThreads;
+		const int i = tid + k * S::kThreads;
 		if (i < kQ * kQ)
 		{
 			const int rr = i / kQ, cc = i - rr * kQ;
 #pragma unroll
 			for (int b = 0; b < 4; b++)
-				sA[(2 * rr + (b >> 1)) * kInvPitch + 2 * cc + (b & 1)] = vals[b][k];
+				sA[(2 * rr + (b >> 1)) * S::kPitch + 2 * cc + (b & 1)] = vals[b][k];
 		}
 	}
 }
 
-__global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
-                                                             const uint16_t *__restrict__ tile_job,
-                                                             const FrameState *__restrict__ frames, int nxcd)
+template <int T, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
+                                                         const uint16_t *__restrict__ tile_job,
+                                                         const FrameState *__restrict__ frames, int nxcd)
 {
-	__shared__ float sA[kInvIn * kInvPitch];
-	__shared__ int s_xmap[kInvIn];
-	__shared__ int s_ymap[kInvIn];
+	using S = InvTile<T, THREADS>;
+	__shared__ float sA[S::kIn * S::kPitch];
+	__shared__ int s_xmap[S::kIn];
+	__shared__ int s_ymap[S::kIn];
 
 	const int tid = threadIdx.x;
 	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
 	// The job index is uniform: readfirstlane keeps J in scalar registers.
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
-	const int ox0 = (local % J.tiles_x) * kInvTile;
-	const int oy0 = (local / J.tiles_x) * kInvTile;
+	const int ox0 = (local % J.tiles_x) * T;
+	const int oy0 = (local / J.tiles_x) * T;
 
-	if (tid < kInvIn)
+	if (tid < S::kIn)
 		s_xmap[tid] = inv_ext(ox0 - kApron + tid, J.A_x);
-	else if (tid < 2 * kInvIn)
-		s_ymap[tid - kInvIn] = inv_ext(oy0 - kApron + tid - kInvIn, J.A_y);
+	else if (tid < 2 * S::kIn)
+		s_ymap[tid - S::kIn] = inv_ext(oy0 - kApron + tid - S::kIn, J.A_y);
 	__syncthreads();
 
 	// Interleaved position parity selects the band: LL (even, even), HL (x
 	// odd), LH (y odd), HH.  The extension preserves parity.
 	if (J.band_kind == kStoreF16)
-		load_bands<__half>(J, s_ymap, s_xmap, tid, sA);
+		load_bands<S, __half>(J, s_ymap, s_xmap, tid, sA);
 	else
-		load_bands<float>(J, s_ymap, s_xmap, tid, sA);
+		load_bands<S, float>(J, s_ymap, s_xmap, tid, sA);
 	__syncthreads();
 
-	// Vertical pass over all 72 columns; rows of the tile are windows 4..11.
-	float vout[kInvPass1Rounds][8];
+	// Vertical pass over all T + 8 columns; rows of the tile are windows 4..11.
+	float vout[S::kPass1Rounds][8];
 #pragma unroll
-	for (int t = 0; t < kInvPass1Rounds; t++)
+	for (int t = 0; t < S::kPass1Rounds; t++)
 	{
-		const int task = tid + t * kDwtThreads;
-		if (t < kInvPass1Rounds - 1 || task < kInvPass1Tasks)
+		const int task = tid + t * THREADS;
+		if (t < S::kPass1Rounds - 1 || task < S::kPass1Tasks)
 		{
-			const int c = task % kInvIn, seg = task / kInvIn;
+			const int c = task % S::kIn, seg = task / S::kIn;
 			float v[16];
 #pragma unroll
 			for (int j = 0; j < 16; j++)
-				v[j] = sA[(seg * 8 + j) * kInvPitch + c] * ((j & 1) ? kInvK : kK);
+				v[j] = sA[(seg * 8 + j) * S::kPitch + c] * ((j & 1) ? kInvK : kK);
 			lift_inv16(v);
 #pragma unroll
 			for (int j = 0; j < 8; j++)
@@ -484,20 +511,20 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 	}
 	__syncthreads();
 #pragma unroll
-	for (int t = 0; t < kInvPass1Rounds; t++)
+	for (int t = 0; t < S::kPass1Rounds; t++)
 	{
-		const int task = tid + t * kDwtThreads;
-		if (t < kInvPass1Rounds - 1 || task < kInvPass1Tasks)
+		const int task = tid + t * THREADS;
+		if (t < S::kPass1Rounds - 1 || task < S::kPass1Tasks)
 		{
-			const int c = task % kInvIn, seg = task / kInvIn;
+			const int c = task % S::kIn, seg = task / S::kIn;
 #pragma unroll
 			for (int j = 0; j < 8; j++)
-				sA[(seg * 8 + j) * kInvPitch + c] = vout[t][j];
+				sA[(seg * 8 + j) * S::kPitch + c] = vout[t][j];
 		}
 	}
 	__syncthreads();
 
-	// Horizontal pass: 8 lanes per row write 64 consecutive outputs.
+	// Horizontal pass: T/8 lanes per row write T consecutive outputs.
 	uint8_t *plane = nullptr;
 	long dpitch = J.dst_pitch;
 	if (J.dst_kind == kStoreU8)
@@ -506,14 +533,14 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 		dpitch = frames[J.frame].dst_pitch[J.component];
 	}
 #pragma unroll
-	for (int t = 0; t < 2; t++)
+	for (int t = 0; t < S::kPass2Rounds; t++)
 	{
-		const int task = tid + t * kDwtThreads;
-		const int seg = task & 7, row = task >> 3;
+		const int task = tid + t * THREADS;
+		const int seg = task % S::kSegs, row = task / S::kSegs;
 		float v[16];
 #pragma unroll
 		for (int j = 0; j < 16; j++)
-			v[j] = sA[row * kInvPitch + seg * 8 + j] * ((j & 1) ? kInvK : kK);
+			v[j] = sA[row * S::kPitch + seg * 8 + j] * ((j & 1) ? kInvK : kK);
 		lift_inv16(v);
 		const int py = oy0 + row;
 		if (J.dst_kind == kStoreU8)
@@ -570,21 +597,29 @@ __global__ __launch_bounds__(kDwtThreads) void k_dwt_inverse(const DwtJob *__res
 } // namespace
 
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                        bool u8_only, hipStream_t stream)
+                        int variant, hipStream_t stream)
 {
-	(void)u8_only;
-	if (tiles > 0)
-		hipLaunchKernelGGL(k_dwt_forward, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames,
-		                   xcd_count());
+	if (tiles <= 0)
+		return;
+	if (variant == 1)
+		hipLaunchKernelGGL(k_dwt_forward<kFwdVariantTileW[1]>, dim3(tiles), dim3(2 * kFwdVariantTileW[1]), 0, stream,
+		                   jobs, tile_job, frames, xcd_count());
+	else
+		hipLaunchKernelGGL(k_dwt_forward<kFwdVariantTileW[0]>, dim3(tiles), dim3(2 * kFwdVariantTileW[0]), 0, stream,
+		                   jobs, tile_job, frames, xcd_count());
 }
 
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                        bool fp16, hipStream_t stream)
+                        int variant, hipStream_t stream)
 {
-	(void)fp16;
-	if (tiles > 0)
-		hipLaunchKernelGGL(k_dwt_inverse, dim3(tiles), dim3(kDwtThreads), 0, stream, jobs, tile_job, frames,
-		                   xcd_count());
+	if (tiles <= 0)
+		return;
+	if (variant == 1)
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTile[1], 128>), dim3(tiles), dim3(128), 0, stream, jobs, tile_job,
+		                   frames, xcd_count());
+	else
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTile[0], 256>), dim3(tiles), dim3(256), 0, stream, jobs, tile_job,
+		                   frames, xcd_count());
 }
 
 int xcd_count()

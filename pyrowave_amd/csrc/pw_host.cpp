@@ -498,14 +498,13 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	const int n = int(slots.size());
 	std::vector<FrameState> host_frames(n);
 	params.assign(n, CallParams{});
-	std::vector<DwtJob> jobs[kLevels];
-	std::vector<uint16_t> tile_map[kLevels];
+	std::vector<DwtJob> jobs[kLevels][kDwtVariants];
+	std::vector<uint16_t> tile_map[kLevels][kDwtVariants];
 	std::vector<BandJob> band_jobs;
 	std::vector<uint32_t> blk_band;
 	std::vector<int32_t> frame_block_base(n), frame_group_base(n);
 	std::vector<uint32_t> grp_frame;
 	max_n32 = 0;
-	any_fp16 = false;
 
 	for (int f = 0; f < n; f++)
 	{
@@ -513,7 +512,6 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 		const Geometry &G = S->geom;
 		host_frames[f] = S->fs;
 		max_n32 = std::max(max_n32, G.count32);
-		any_fp16 |= G.precision == PYROWAVE_HIP_PRECISION_FP16;
 		auto band_ptr = [&](int c, int l, int b) { return static_cast<void *>(S->bands + G.band[c][l][b].store_off); };
 
 		for (int l = 0; l < kLevels; l++)
@@ -577,13 +575,18 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 						J.R_y = J.A_y;
 					}
 				}
-				const int tw = encoder ? kFwdTileW : kInvTileW, th = encoder ? kFwdTileH : kInvTileH;
-				J.tiles_x = (J.A_x + tw - 1) / tw;
-				int tiles_y = (J.A_y + th - 1) / th;
-				J.tile_base = int(tile_map[l].size());
-				uint16_t idx = uint16_t(jobs[l].size());
-				jobs[l].push_back(J);
-				tile_map[l].insert(tile_map[l].end(), size_t(J.tiles_x) * tiles_y, idx);
+				for (int v = 0; v < kDwtVariants; v++)
+				{
+					const int tw = encoder ? kFwdVariantTileW[v] : kInvVariantTile[v];
+					const int th = encoder ? kFwdTileH : kInvVariantTile[v];
+					DwtJob Jv = J;
+					Jv.tiles_x = (J.A_x + tw - 1) / tw;
+					const int tiles_y = (J.A_y + th - 1) / th;
+					Jv.tile_base = int(tile_map[l][v].size());
+					const uint16_t idx = uint16_t(jobs[l][v].size());
+					jobs[l][v].push_back(Jv);
+					tile_map[l][v].insert(tile_map[l][v].end(), size_t(Jv.tiles_x) * tiles_y, idx);
+				}
 			}
 		}
 
@@ -626,14 +629,19 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	}
 	nblocks = int(blk_band.size() / 2);
 	ngroups = int(grp_frame.size());
+	Device &D = *slots[0]->dev;
 	for (int l = 0; l < kLevels; l++)
 	{
-		u8_only[l] = !jobs[l].empty();
-		for (const DwtJob &J : jobs[l])
-			u8_only[l] = u8_only[l] && J.src_kind == kStoreU8;
-		tiles[l] = int(tile_map[l].size());
-		if (jobs[l].size() > 65535)
-			return false;
+		for (int v = 0; v < kDwtVariants; v++)
+		{
+			tiles[l][v] = int(tile_map[l][v].size());
+			if (jobs[l][v].size() > 65535)
+				return false;
+		}
+		variant[l] = 0;
+		auto it = D.tuned_variant.find(tune_key(encoder, l, tiles[l][0]));
+		if (it != D.tuned_variant.end())
+			variant[l] = it->second;
 	}
 
 	// One device allocation for every table.
@@ -644,12 +652,13 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 		return off;
 	};
 	size_t o_frames = take(sizeof(FrameState) * n);
-	size_t o_jobs[kLevels], o_tiles[kLevels];
+	size_t o_jobs[kLevels][kDwtVariants], o_tiles[kLevels][kDwtVariants];
 	for (int l = 0; l < kLevels; l++)
-	{
-		o_jobs[l] = take(sizeof(DwtJob) * jobs[l].size());
-		o_tiles[l] = take(sizeof(uint16_t) * tile_map[l].size());
-	}
+		for (int v = 0; v < kDwtVariants; v++)
+		{
+			o_jobs[l][v] = take(sizeof(DwtJob) * jobs[l][v].size());
+			o_tiles[l][v] = take(sizeof(uint16_t) * tile_map[l][v].size());
+		}
 	size_t o_bands = take(sizeof(BandJob) * band_jobs.size());
 	size_t o_blk = take(sizeof(uint32_t) * blk_band.size());
 	size_t o_fbb = take(sizeof(int32_t) * n);
@@ -681,12 +690,13 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 			memcpy(host + off, src, bytes);
 	};
 	for (int l = 0; l < kLevels; l++)
-	{
-		put(o_jobs[l], jobs[l].data(), sizeof(DwtJob) * jobs[l].size());
-		put(o_tiles[l], tile_map[l].data(), sizeof(uint16_t) * tile_map[l].size());
-		d_dwt_jobs[l] = reinterpret_cast<DwtJob *>(base + o_jobs[l]);
-		d_tile_job[l] = reinterpret_cast<uint16_t *>(base + o_tiles[l]);
-	}
+		for (int v = 0; v < kDwtVariants; v++)
+		{
+			put(o_jobs[l][v], jobs[l][v].data(), sizeof(DwtJob) * jobs[l][v].size());
+			put(o_tiles[l][v], tile_map[l][v].data(), sizeof(uint16_t) * tile_map[l][v].size());
+			d_dwt_jobs[l][v] = reinterpret_cast<DwtJob *>(base + o_jobs[l][v]);
+			d_tile_job[l][v] = reinterpret_cast<uint16_t *>(base + o_tiles[l][v]);
+		}
 	put(o_bands, band_jobs.data(), sizeof(BandJob) * band_jobs.size());
 	put(o_blk, blk_band.data(), sizeof(uint32_t) * blk_band.size());
 	put(o_fbb, frame_block_base.data(), sizeof(int32_t) * n);
@@ -720,6 +730,73 @@ FrameState *Plan::stage_frames(hipStream_t stream)
 // Launch sequences
 // ---------------------------------------------------------------------------
 
+uint64_t tune_key(bool encoder, int level, int tiles0)
+{
+	return (uint64_t(encoder) << 40) | (uint64_t(level) << 32) | uint32_t(tiles0);
+}
+
+int level_variant(const Plan &P, int l)
+{
+	const int f = P.slots[0]->dev->forced_variant;
+	return f >= 0 && f < kDwtVariants ? f : P.variant[l];
+}
+
+// One transform level, autotuned: every variant `reps` times with HIP events
+// around each launch; the fastest (minimum over its launches) is kept.  The
+// last launch leaves the level's output in place for the next level.
+template <typename Launch>
+void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Launch launch)
+{
+	Device &D = *P.slots[0]->dev;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+	{
+		launch(level_variant(P, l));
+		if (e0)
+			(void)hipEventDestroy(e0);
+		return;
+	}
+	int best = P.variant[l];
+	float best_ms = 0.0f;
+	bool have = false;
+	for (int v = 0; v < kDwtVariants; v++)
+	{
+		if (P.tiles[l][v] == 0)
+			continue;
+		float v_ms = 0.0f;
+		for (int r = 0; r < reps; r++)
+		{
+			(void)hipEventRecord(e0, stream);
+			launch(v);
+			(void)hipEventRecord(e1, stream);
+			float ms = 0.0f;
+			if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+				v_ms = r == 0 ? ms : std::min(v_ms, ms);
+		}
+		if (!have || v_ms < best_ms)
+		{
+			best = v;
+			best_ms = v_ms;
+			have = true;
+		}
+	}
+	// The chosen variant runs last, so its output is the one the next level
+	// reads (all variants write the same values).
+	launch(best);
+	if (P.variant[l] != best && P.graph_exec)
+	{
+		// A captured graph holds the old launches: capture again on next use.
+		(void)hipGraphExecDestroy(P.graph_exec);
+		(void)hipGraphDestroy(P.graph);
+		P.graph_exec = nullptr;
+		P.graph = nullptr;
+	}
+	P.variant[l] = best;
+	D.tuned_variant[tune_key(encoder, l, P.tiles[l][0])] = best;
+	(void)hipEventDestroy(e0);
+	(void)hipEventDestroy(e1);
+}
+
 void run_encode(Plan &P, hipStream_t stream, bool timed)
 {
 	Device &D = *P.slots[0]->dev;
@@ -732,9 +809,18 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 
 	launch_rc_reset(frames, n, P.max_n32, stream);
 
+	const int reps = timed ? D.tune_reps : 0;
 	D.begin_stage(stream, kStageDWT, n, &ev);
 	for (int l = 0; l < kLevels; l++)
-		launch_dwt_forward(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.u8_only[l], stream);
+	{
+		auto launch = [&](int v) {
+			launch_dwt_forward(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
+		};
+		if (reps > 0)
+			tune_level(P, true, l, reps, stream, launch);
+		else
+			launch(level_variant(P, l));
+	}
 	D.end_stage(stream, kStageDWT, n, ev);
 
 	D.begin_stage(stream, kStageQuant, n, &ev);
@@ -767,9 +853,18 @@ void run_decode(Plan &P, hipStream_t stream, bool timed)
 	launch_dequant(P.d_bands, P.d_blk_band, P.nblocks, frames, stream);
 	D.end_stage(stream, kStageDequant, n, ev);
 
+	const int reps = timed ? D.tune_reps : 0;
 	D.begin_stage(stream, kStageIDWT, n, &ev);
 	for (int l = kLevels - 1; l >= 0; l--)
-		launch_dwt_inverse(P.d_dwt_jobs[l], P.d_tile_job[l], P.tiles[l], frames, P.any_fp16, stream);
+	{
+		auto launch = [&](int v) {
+			launch_dwt_inverse(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
+		};
+		if (reps > 0)
+			tune_level(P, false, l, reps, stream, launch);
+		else
+			launch(level_variant(P, l));
+	}
 	D.end_stage(stream, kStageIDWT, n, ev);
 	D.timing = saved_timing;
 }

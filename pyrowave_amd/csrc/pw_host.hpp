@@ -95,6 +95,11 @@ struct Device
 	std::vector<hipEvent_t> event_pool;
 	std::atomic<int> live_objects{0};
 	bool graphs = false; // batch entry points replay a captured graph per batch composition
+	// Autotuned transform tile variants, keyed by (direction, level, tiles of
+	// variant 0); new plans of the same shape start from these.
+	std::map<uint64_t, int> tuned_variant;
+	int forced_variant = -1; // >= 0: every transform level uses this variant
+	int tune_reps = 0;       // > 0: the next eager encode / decode autotunes
 	// PSNR-HVS-M-H tables per image height (device copies).
 	std::map<int, std::unique_ptr<DeviceBuffer>> psnr_tables;
 
@@ -190,9 +195,12 @@ struct Plan
 	DeviceBuffer mem;
 	FrameState *d_frames = nullptr;
 
-	DwtJob *d_dwt_jobs[kLevels] = {};
-	uint16_t *d_tile_job[kLevels] = {};
-	int tiles[kLevels] = {};
+	// Transform launch tables per level and tile variant; variant[l] is the
+	// one the launches use (the autotuner's choice, 0 by default).
+	DwtJob *d_dwt_jobs[kLevels][kDwtVariants] = {};
+	uint16_t *d_tile_job[kLevels][kDwtVariants] = {};
+	int tiles[kLevels][kDwtVariants] = {};
+	int variant[kLevels] = {};
 	BandJob *d_bands = nullptr;
 	uint32_t *d_blk_band = nullptr;
 	int32_t *d_frame_block_base = nullptr;
@@ -201,8 +209,6 @@ struct Plan
 	int nblocks = 0;
 	int ngroups = 0;
 	int max_n32 = 0;
-	bool any_fp16 = false;
-	bool u8_only[kLevels] = {};  // every forward job of the level reads a u8 plane
 
 	// Without storage the plan owns its tables (synchronous upload); with
 	// storage they go into it, uploaded on `stream`.
@@ -234,6 +240,13 @@ void run_encode(Plan &plan, hipStream_t stream, bool timed = true);
 void run_decode(Plan &plan, hipStream_t stream, bool timed = true);
 // Replays the plan's captured launch sequence (capturing it on first use).
 bool run_graph(Plan &plan, hipStream_t stream);
+// Autotuner: while Device::tune_reps > 0, run_encode / run_decode time every
+// tile variant of each transform level (`tune_reps` launches each, HIP
+// events on the launch stream, synchronous) and keep the fastest per level
+// in the plan and in Device::tuned_variant.  A level's launch does not modify
+// its own inputs, so repeating it leaves the call's results valid.
+// Key of Device::tuned_variant.
+uint64_t tune_key(bool encoder, int level, int tiles0);
 
 // Budget in bytes -> words of coded blocks (the sequence header comes off).
 int target_words_of(size_t target_bytes);

@@ -1,0 +1,75 @@
+"""Transform tile shapes and the autotuner (MI355X).
+
+Both tile shapes of each transform direction give the same coefficients bit
+for bit, so a batch of uneven frames (1280x720 4:2:0, 640x480 4:4:4,
+1920x1080 4:2:0, FP16 and FP32) encodes and decodes to the same bitstreams and
+planes with either shape forced on every level; pyrowave_hip_autotune_batch
+returns those same results while it times the shapes, reports a valid shape
+per level, and later calls (which use the tuned shapes) still agree.
+"""
+import numpy as np
+import pytest
+
+import pyrowave_amd as pw
+from pyrowave_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(1280, 720, pw.CHROMA_420), (640, 480, pw.CHROMA_444), (1920, 1080, pw.CHROMA_420)]
+
+
+def _batch(torch):
+    inputs = [[torch.from_numpy(p).cuda() for p in synth.frame(w, h, c == pw.CHROMA_420, seed=90 + i)]
+              for i, (w, h, c) in enumerate(SHAPES)]
+    frames = [pw.hip_frame([(t.data_ptr(), t.stride(0), 1) for t in ts]) for ts in inputs]
+    budgets = [int(w * h * 1.5 / 8) for w, h, _ in SHAPES]
+    return inputs, frames, budgets
+
+
+def _targets(torch):
+    outs = [[torch.zeros(pw.plane_shape(w, h, c, p), dtype=torch.uint8, device="cuda") for p in range(3)]
+            for w, h, c in SHAPES]
+    return outs, [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in oo]) for oo in outs]
+
+
+def _collect(encs, outs, budgets):
+    res = []
+    for e, oo, b in zip(encs, outs, budgets):
+        data, pk = e.packetize(1 << 30, b + 64)
+        res.append((data[:sum(z for _, z in pk)].copy(), [o.cpu().numpy() for o in oo]))
+    return res
+
+
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP32])
+def test_tile_shapes_agree_and_autotune(precision):
+    import torch
+    inputs, frames, budgets = _batch(torch)
+    results = {}
+    dev = pw.Device()
+    try:
+        for mode in ("shape0", "shape1", "autotune", "tuned"):
+            dev.force_tile_variant({"shape0": 0, "shape1": 1}.get(mode, -1))
+            encs = [pw.Encoder(dev, w, h, c, precision=precision) for w, h, c in SHAPES]
+            decs = [pw.Decoder(dev, w, h, c, precision=precision) for w, h, c in SHAPES]
+            try:
+                outs, tg = _targets(torch)
+                if mode == "autotune":
+                    fv, iv = pw.autotune_batch(encs, frames, budgets, decs, tg, repetitions=2)
+                    assert all(v in (0, 1) for v in fv + iv), (fv, iv)
+                    print("autotuned shapes: forward", fv, "inverse", iv)
+                else:
+                    pw.encode_batch(encs, frames, budgets)
+                    pw.decode_encoded_batch(decs, encs, tg)
+                torch.cuda.synchronize()
+                results[mode] = _collect(encs, outs, budgets)
+            finally:
+                for x in encs + decs:
+                    x.close()
+    finally:
+        dev.close()
+    ref = results["shape0"]
+    for mode in ("shape1", "autotune", "tuned"):
+        for i, ((a, pa), (b, pb)) in enumerate(zip(ref, results[mode])):
+            assert np.array_equal(a, b), (mode, i)
+            for p in range(3):
+                assert np.array_equal(pa[p], pb[p]), (mode, i, p)
