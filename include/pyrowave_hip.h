@@ -142,8 +142,8 @@ pyrowave_hip_decode_encoded_batch(pyrowave_decoder *decoders, pyrowave_encoder *
 PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
 
-/* Autotuner for the transform tile shapes (two per direction: forward 128x56
- * or 64x56 samples per workgroup, inverse 64x64 or 32x32).  Encodes the batch
+/* Autotuner for the transform tile shapes (forward 128x56, 64x56, 128x48 or
+ * 128x40 samples per workgroup, inverse 64x64, 32x32 or 64x56).  Encodes the batch
  * (and, with decoders, decodes it from the encoders' results) like
  * pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch, timing every
  * shape of every transform level `repetitions` times (HIP events on the
@@ -159,8 +159,9 @@ pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decode
                             uint32_t count, uint32_t repetitions, void *hip_stream, int *forward_variants,
                             int *inverse_variants);
 
-/* Forces one transform tile shape (0 or 1) for every level on the device;
- * -1 returns to the autotuned / default shapes.  For tests and comparisons. */
+/* Forces one transform tile shape (forward 0-3, inverse 0-2; a shape a
+ * direction does not have leaves that direction as it was) for every level; -1 returns to the autotuned / default shapes.
+ * For tests and comparisons. */
 PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_force_tile_variant(pyrowave_device device, int variant);
 

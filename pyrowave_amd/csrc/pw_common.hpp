@@ -23,17 +23,23 @@ constexpr int kFlat = kBuckets * kSubdiv;
 // DWT tiles, one workgroup each, with a 4-sample apron on every side.  Two
 // tile shapes per direction; the autotuner picks one per level (both give
 // the same coefficients, bit for bit).
-// Forward: W x 56 samples of the level domain ((W + 8) x 64 staged), so the
-// horizontal pass is 64 rows x W/32 windows of 40 and the vertical pass W
-// columns x 2 windows of 36: 2W threads.  Variant 0: W = 128 (256 threads),
-// variant 1: W = 64 (128 threads).
-// Inverse: T x T output samples ((T + 8)^2 staged) in 16-sample windows.
-// Variant 0: T = 64 (256 threads), variant 1: T = 32 (128 threads).
+// Forward: W x H samples of the level domain ((W + 8) x (H + 8) staged); the
+// horizontal pass is H + 8 rows x W/32 windows of 40 and the vertical pass W
+// columns x 2 windows of H/2 + 8: 2W threads.  Variant 0: 128 x 56 (256
+// threads), 1: 64 x 56 (128 threads); 2: 128 x 48 and 3: 128 x 40 (256
+// threads, u8 conversion without an LDS table: 31 / 27 KB of LDS, five / six
+// workgroups per CU instead of four).
+// Inverse: W x H output samples ((W + 8) x (H + 8) staged) in 16-sample
+// windows.  Variant 0: 64 x 64 (256 threads), 1: 32 x 32 (128 threads),
+// 2: 64 x 56 (256 threads, 19 KB of LDS: eight workgroups per CU, not seven).
 constexpr int kApron = 4;
-constexpr int kDwtVariants = 2;
-constexpr int kFwdTileH = 56;
-constexpr int kFwdVariantTileW[kDwtVariants] = {128, 64};
-constexpr int kInvVariantTile[kDwtVariants] = {64, 32};
+constexpr int kFwdVariants = 4;
+constexpr int kInvVariants = 3;
+constexpr int kDwtVariants = 4; // table slots per level (the larger count)
+constexpr int kFwdVariantTileW[kFwdVariants] = {128, 64, 128, 128};
+constexpr int kFwdVariantTileH[kFwdVariants] = {56, 56, 48, 40};
+constexpr int kInvVariantTileW[kInvVariants] = {64, 32, 64};
+constexpr int kInvVariantTileH[kInvVariants] = {64, 32, 56};
 
 // Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
 // sign bits, 8 B header and 3 B per 8x8 block, rounded up to words.

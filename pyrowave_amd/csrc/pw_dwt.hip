@@ -39,38 +39,43 @@ constexpr float kK = float(1.230174104914001);
 constexpr float kInvK = float(1.0 / 1.230174104914001);
 
 constexpr int kFwdHWin = 40, kFwdHOut = 32; // horizontal window / outputs
-constexpr int kFwdVWin = 36, kFwdVOut = 28; // vertical window / outputs
 
-// Forward tile of W x 56 outputs, (W + 8) x 64 staged, 2W threads.
-template <int W>
+// Forward tile of W x H outputs, (W + 8) x (H + 8) staged, 2W threads;
+// vertical windows of H/2 + 8 samples (H/2 exact outputs).  LUT: convert u8
+// samples through an LDS copy of the u/255 - 0.5 table (else arithmetically).
+template <int W, int H, bool LUT>
 struct FwdTile
 {
-	static constexpr int kW = W, kH = kFwdTileH;
+	static constexpr int kW = W, kH = H;
 	static constexpr int kThreads = 2 * W;
-	static constexpr int kInW = W + 2 * kApron, kInH = kH + 2 * kApron;
+	static constexpr int kInW = W + 2 * kApron, kInH = H + 2 * kApron;
 	static constexpr int kPitch = kInW + 1; // odd: row-per-lane reads are conflict-free
 	static constexpr int kGroups = kInW / 4; // groups of 4 samples per row
 	static constexpr int kSlots = kInH * kGroups;
 	static constexpr int kSlotIters = (kSlots + kThreads - 1) / kThreads;
-	static_assert(kInH * (W / kFwdHOut) == kThreads, "one horizontal window per thread");
-	static_assert(W * (kH / kFwdVOut) == kThreads, "one vertical window per thread");
-	static_assert(kInH == 64, "one row per lane in the horizontal pass");
+	static constexpr int kVOut = H / 2, kVWin = kVOut + 2 * kApron;
+	static constexpr bool kLut = LUT;
+	static_assert(kInH <= 64, "one staged row per lane in the horizontal pass");
+	static_assert(kInH * (W / kFwdHOut) <= kThreads, "one horizontal window per thread");
+	static_assert(W * 2 == kThreads && kVOut % 2 == 0, "two vertical windows per column");
 };
 
-// Inverse tile of T x T outputs, (T + 8)^2 staged, 16-sample windows
-// (8 exact outputs each).
-template <int T, int THREADS>
+// Inverse tile of TW x TH outputs, (TW + 8) x (TH + 8) staged, 16-sample
+// windows (8 exact outputs each).
+template <int TW, int TH, int THREADS>
 struct InvTile
 {
-	static constexpr int kTile = T, kThreads = THREADS;
-	static constexpr int kIn = T + 2 * kApron;
-	static constexpr int kPitch = kIn + 1;
-	static constexpr int kSegs = T / 8;                       // windows per line
-	static constexpr int kPass1Tasks = kIn * kSegs;           // vertical windows
+	static constexpr int kTW = TW, kTH = TH, kThreads = THREADS;
+	static constexpr int kInW = TW + 2 * kApron, kInH = TH + 2 * kApron;
+	static constexpr int kPitch = kInW + 1;
+	static constexpr int kVSegs = TH / 8;                    // vertical windows per column
+	static constexpr int kHSegs = TW / 8;                    // horizontal windows per row
+	static constexpr int kPass1Tasks = kInW * kVSegs;
 	static constexpr int kPass1Rounds = (kPass1Tasks + THREADS - 1) / THREADS;
-	static constexpr int kPass2Rounds = T * kSegs / THREADS;  // horizontal windows
-	static_assert(T * kSegs % THREADS == 0, "whole horizontal rounds");
-	static_assert(2 * kIn <= THREADS, "one thread per extension-map entry");
+	static constexpr int kPass2Tasks = TH * kHSegs;
+	static constexpr int kPass2Rounds = (kPass2Tasks + THREADS - 1) / THREADS;
+	static_assert(TW % 8 == 0 && TH % 8 == 0, "whole windows");
+	static_assert(kInW + kInH <= THREADS, "one thread per extension-map entry");
 };
 
 __constant__ float c_u8_dc[256]; // u8 / 255 - 0.5 (UNORM8 texel with the level-0 DC shift)
@@ -224,6 +229,17 @@ __device__ __forceinline__ Raw4<T> load4_gather(const T *row, const int *xmap, i
 	return r;
 }
 
+// u / 255 - 0.5 without a table: fmaf(u, hi, u * lo) with hi + lo = 1/255
+// split into two floats rounds to float(u / 255) for every u in 0..255
+// (checked exhaustively against the correctly rounded quotient), then the
+// DC shift; equal to the table entry.
+__device__ __forceinline__ float u8_dc(uint32_t u)
+{
+	const float fu = (float)u;
+	const float hi = __uint_as_float(0x3b808081u), lo = __uint_as_float(0xaf7efeffu);
+	return fmaf(fu, hi, fu * lo) - 0.5f;
+}
+
 template <typename T>
 __device__ __forceinline__ void unpack4(const Raw4<T> &r, const float *lut, float (&f)[4])
 {
@@ -231,7 +247,7 @@ __device__ __forceinline__ void unpack4(const Raw4<T> &r, const float *lut, floa
 	{
 #pragma unroll
 		for (int i = 0; i < 4; i++)
-			f[i] = lut[(r.w >> (8 * i)) & 0xffu];
+			f[i] = lut ? lut[(r.w >> (8 * i)) & 0xffu] : u8_dc((r.w >> (8 * i)) & 0xffu);
 	}
 	else if constexpr (sizeof(T) == 2)
 	{
@@ -292,16 +308,17 @@ __device__ __forceinline__ void fwd_stage(const T *src, uint32_t pitch, uint32_t
 	}
 }
 
-template <int W>
+template <int W, int H, bool LUT>
 __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict__ jobs,
                                                        const uint16_t *__restrict__ tile_job,
                                                        const FrameState *__restrict__ frames, int nxcd)
 {
-	using S = FwdTile<W>;
+	using S = FwdTile<W, H, LUT>;
 	__shared__ float sA[S::kInH * S::kPitch];
 	__shared__ int s_xmap[S::kInW];
 	__shared__ int s_ymap[S::kInH];
-	__shared__ float s_lut[256];
+	__shared__ float s_lut_mem[LUT ? 256 : 1];
+	float *s_lut = LUT ? s_lut_mem : nullptr;
 
 	const int tid = threadIdx.x;
 	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
@@ -318,8 +335,9 @@ __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict_
 		else
 			s_ymap[i - S::kInW] = fwd_ext(y0 - kApron + i - S::kInW, J.R_y, J.A_y);
 	}
-	for (int i = tid; i < 256; i += S::kThreads)
-		s_lut[i] = c_u8_dc[i];
+	if constexpr (LUT)
+		for (int i = tid; i < 256; i += S::kThreads)
+			s_lut_mem[i] = c_u8_dc[i];
 	__syncthreads();
 
 	if (J.src_kind == kStoreU8)
@@ -339,34 +357,41 @@ __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict_
 	__syncthreads();
 
 	// Horizontal pass: wave w takes window w (outputs 32w .. 32w+31) of the
-	// 64 staged rows, one row per lane.
+	// staged rows, one row per lane (lanes past the last row idle).
 	{
 		const int row = tid & 63, seg = tid >> 6;
+		const bool live = row < S::kInH;
 		float v[kFwdHWin];
+		if (live)
+		{
 #pragma unroll
-		for (int j = 0; j < kFwdHWin; j++)
-			v[j] = sA[row * S::kPitch + seg * kFwdHOut + j];
-		lift_fwd(v);
+			for (int j = 0; j < kFwdHWin; j++)
+				v[j] = sA[row * S::kPitch + seg * kFwdHOut + j];
+			lift_fwd(v);
+		}
 		__syncthreads();
 		// Low outputs to columns 0..W/2-1, high outputs to W/2..W-1.
-#pragma unroll
-		for (int k = 0; k < kFwdHOut / 2; k++)
+		if (live)
 		{
-			sA[row * S::kPitch + seg * (kFwdHOut / 2) + k] = v[4 + 2 * k] * kInvK;
-			sA[row * S::kPitch + W / 2 + seg * (kFwdHOut / 2) + k] = v[5 + 2 * k] * kK;
+#pragma unroll
+			for (int k = 0; k < kFwdHOut / 2; k++)
+			{
+				sA[row * S::kPitch + seg * (kFwdHOut / 2) + k] = v[4 + 2 * k] * kInvK;
+				sA[row * S::kPitch + W / 2 + seg * (kFwdHOut / 2) + k] = v[5 + 2 * k] * kK;
+			}
 		}
 	}
 	__syncthreads();
 
 	// Vertical pass: lanes are consecutive columns; threads 0..W-1 take
-	// output rows 0..27, threads W..2W-1 rows 28..55; each store instruction
+	// output rows 0..H/2-1, threads W..2W-1 the rest; each store instruction
 	// writes one band row.
 	{
 		const int col = tid & (W - 1), seg = tid / W;
-		float v[kFwdVWin];
+		float v[S::kVWin];
 #pragma unroll
-		for (int j = 0; j < kFwdVWin; j++)
-			v[j] = sA[(seg * kFwdVOut + j) * S::kPitch + col];
+		for (int j = 0; j < S::kVWin; j++)
+			v[j] = sA[(seg * S::kVOut + j) * S::kPitch + col];
 		lift_fwd(v);
 		const bool low = col < W / 2;
 		const int ox = (x0 >> 1) + (col & (W / 2 - 1));
@@ -375,9 +400,9 @@ __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict_
 			void *lo = low ? J.band[0] : J.band[1];
 			void *hi = low ? J.band[2] : J.band[3];
 			const int lo_kind = low ? J.ll_kind : J.band_kind;
-			const int oy0 = (y0 >> 1) + seg * (kFwdVOut / 2);
+			const int oy0 = (y0 >> 1) + seg * (S::kVOut / 2);
 #pragma unroll
-			for (int k = 0; k < kFwdVOut / 2; k++)
+			for (int k = 0; k < S::kVOut / 2; k++)
 			{
 				const int oy = oy0 + k;
 				if (oy < J.bh)
@@ -401,14 +426,15 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
-// Loads the four bands' (T/2 + 4)^2 shares of the interleaved (T + 8)^2 tile
+// Loads the four bands' (TW/2 + 4) x (TH/2 + 4) shares of the interleaved
+// (TW + 8) x (TH + 8) tile
 // (band b holds the positions whose (row, column) parity is (b >> 1, b & 1))
 // into LDS; every global load is issued before the first LDS store.
 template <typename S, typename TH>
 __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, const int *xmap, int tid, float *sA)
 {
-	constexpr int kQ = S::kIn / 2;
-	constexpr int kIters = (kQ * kQ + S::kThreads - 1) / S::kThreads;
+	constexpr int kQX = S::kInW / 2, kQY = S::kInH / 2;
+	constexpr int kIters = (kQX * kQY + S::kThreads - 1) / S::kThreads;
 	// 32-bit element offsets from the uniform band bases (bands stay below
 	// 4 GiB), so loads use a scalar base and a 32-bit vector offset.
 	const uint32_t pitch = (uint32_t)J.band_pitch;
@@ -423,9 +449,9 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 #pragma unroll
 		for (int b = 0; b < 4; b++)
 			vals[b][k] = 0.0f;
-		if (i < kQ * kQ)
+		if (i < kQX * kQY)
 		{
-			const int rr = i / kQ, cc = i - rr * kQ;
+			const int rr = i / kQX, cc = i - rr * kQX;
 			const uint32_t row0 = __umul24((uint32_t)(ymap[2 * rr] >> 1), pitch);
 			const uint32_t row1 = __umul24((uint32_t)(ymap[2 * rr + 1] >> 1), pitch);
 			const uint32_t col0 = (uint32_t)(xmap[2 * cc] >> 1), col1 = (uint32_t)(xmap[2 * cc + 1] >> 1);
@@ -448,9 +474,9 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 	for (int k = 0; k < kIters; k++)
 	{
 		const int i = tid + k * S::kThreads;
-		if (i < kQ * kQ)
+		if (i < kQX * kQY)
 		{
-			const int rr = i / kQ, cc = i - rr * kQ;
+			const int rr = i / kQX, cc = i - rr * kQX;
 #pragma unroll
 			for (int b = 0; b < 4; b++)
 				sA[(2 * rr + (b >> 1)) * S::kPitch + 2 * cc + (b & 1)] = vals[b][k];
@@ -458,28 +484,28 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 	}
 }
 
-template <int T, int THREADS>
+template <int TW, int TH, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
                                                          const uint16_t *__restrict__ tile_job,
                                                          const FrameState *__restrict__ frames, int nxcd)
 {
-	using S = InvTile<T, THREADS>;
-	__shared__ float sA[S::kIn * S::kPitch];
-	__shared__ int s_xmap[S::kIn];
-	__shared__ int s_ymap[S::kIn];
+	using S = InvTile<TW, TH, THREADS>;
+	__shared__ float sA[S::kInH * S::kPitch];
+	__shared__ int s_xmap[S::kInW];
+	__shared__ int s_ymap[S::kInH];
 
 	const int tid = threadIdx.x;
 	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
 	// The job index is uniform: readfirstlane keeps J in scalar registers.
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
-	const int ox0 = (local % J.tiles_x) * T;
-	const int oy0 = (local / J.tiles_x) * T;
+	const int ox0 = (local % J.tiles_x) * TW;
+	const int oy0 = (local / J.tiles_x) * TH;
 
-	if (tid < S::kIn)
+	if (tid < S::kInW)
 		s_xmap[tid] = inv_ext(ox0 - kApron + tid, J.A_x);
-	else if (tid < 2 * S::kIn)
-		s_ymap[tid - S::kIn] = inv_ext(oy0 - kApron + tid - S::kIn, J.A_y);
+	else if (tid < S::kInW + S::kInH)
+		s_ymap[tid - S::kInW] = inv_ext(oy0 - kApron + tid - S::kInW, J.A_y);
 	__syncthreads();
 
 	// Interleaved position parity selects the band: LL (even, even), HL (x
@@ -490,7 +516,7 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		load_bands<S, float>(J, s_ymap, s_xmap, tid, sA);
 	__syncthreads();
 
-	// Vertical pass over all T + 8 columns; rows of the tile are windows 4..11.
+	// Vertical pass over all TW + 8 columns; rows of the tile are windows 4..11.
 	float vout[S::kPass1Rounds][8];
 #pragma unroll
 	for (int t = 0; t < S::kPass1Rounds; t++)
@@ -498,7 +524,7 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		const int task = tid + t * THREADS;
 		if (t < S::kPass1Rounds - 1 || task < S::kPass1Tasks)
 		{
-			const int c = task % S::kIn, seg = task / S::kIn;
+			const int c = task % S::kInW, seg = task / S::kInW;
 			float v[16];
 #pragma unroll
 			for (int j = 0; j < 16; j++)
@@ -516,7 +542,7 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		const int task = tid + t * THREADS;
 		if (t < S::kPass1Rounds - 1 || task < S::kPass1Tasks)
 		{
-			const int c = task % S::kIn, seg = task / S::kIn;
+			const int c = task % S::kInW, seg = task / S::kInW;
 #pragma unroll
 			for (int j = 0; j < 8; j++)
 				sA[(seg * 8 + j) * S::kPitch + c] = vout[t][j];
@@ -524,7 +550,7 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 	}
 	__syncthreads();
 
-	// Horizontal pass: T/8 lanes per row write T consecutive outputs.
+	// Horizontal pass: TW/8 lanes per row write TW consecutive outputs.
 	uint8_t *plane = nullptr;
 	long dpitch = J.dst_pitch;
 	if (J.dst_kind == kStoreU8)
@@ -536,7 +562,9 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 	for (int t = 0; t < S::kPass2Rounds; t++)
 	{
 		const int task = tid + t * THREADS;
-		const int seg = task % S::kSegs, row = task / S::kSegs;
+		if (t == S::kPass2Rounds - 1 && task >= S::kPass2Tasks)
+			break;
+		const int seg = task % S::kHSegs, row = task / S::kHSegs;
 		float v[16];
 #pragma unroll
 		for (int j = 0; j < 16; j++)
@@ -602,11 +630,17 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 	if (tiles <= 0)
 		return;
 	if (variant == 1)
-		hipLaunchKernelGGL(k_dwt_forward<kFwdVariantTileW[1]>, dim3(tiles), dim3(2 * kFwdVariantTileW[1]), 0, stream,
-		                   jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[1], kFwdVariantTileH[1], true>), dim3(tiles),
+		                   dim3(2 * kFwdVariantTileW[1]), 0, stream, jobs, tile_job, frames, xcd_count());
+	else if (variant == 2)
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[2], kFwdVariantTileH[2], false>), dim3(tiles),
+		                   dim3(2 * kFwdVariantTileW[2]), 0, stream, jobs, tile_job, frames, xcd_count());
+	else if (variant == 3)
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[3], kFwdVariantTileH[3], false>), dim3(tiles),
+		                   dim3(2 * kFwdVariantTileW[3]), 0, stream, jobs, tile_job, frames, xcd_count());
 	else
-		hipLaunchKernelGGL(k_dwt_forward<kFwdVariantTileW[0]>, dim3(tiles), dim3(2 * kFwdVariantTileW[0]), 0, stream,
-		                   jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[0], kFwdVariantTileH[0], true>), dim3(tiles),
+		                   dim3(2 * kFwdVariantTileW[0]), 0, stream, jobs, tile_job, frames, xcd_count());
 }
 
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -615,11 +649,14 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 	if (tiles <= 0)
 		return;
 	if (variant == 1)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTile[1], 128>), dim3(tiles), dim3(128), 0, stream, jobs, tile_job,
-		                   frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128>), dim3(tiles), dim3(128), 0,
+		                   stream, jobs, tile_job, frames, xcd_count());
+	else if (variant == 2)
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 256>), dim3(tiles), dim3(256), 0,
+		                   stream, jobs, tile_job, frames, xcd_count());
 	else
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTile[0], 256>), dim3(tiles), dim3(256), 0, stream, jobs, tile_job,
-		                   frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[0], kInvVariantTileH[0], 256>), dim3(tiles), dim3(256), 0,
+		                   stream, jobs, tile_job, frames, xcd_count());
 }
 
 int xcd_count()

@@ -575,10 +575,10 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 						J.R_y = J.A_y;
 					}
 				}
-				for (int v = 0; v < kDwtVariants; v++)
+				for (int v = 0; v < (encoder ? kFwdVariants : kInvVariants); v++)
 				{
-					const int tw = encoder ? kFwdVariantTileW[v] : kInvVariantTile[v];
-					const int th = encoder ? kFwdTileH : kInvVariantTile[v];
+					const int tw = encoder ? kFwdVariantTileW[v] : kInvVariantTileW[v];
+					const int th = encoder ? kFwdVariantTileH[v] : kInvVariantTileH[v];
 					DwtJob Jv = J;
 					Jv.tiles_x = (J.A_x + tw - 1) / tw;
 					const int tiles_y = (J.A_y + th - 1) / th;
@@ -737,8 +737,9 @@ uint64_t tune_key(bool encoder, int level, int tiles0)
 
 int level_variant(const Plan &P, int l)
 {
+	// A forced shape this direction does not have (no tiles) falls back.
 	const int f = P.slots[0]->dev->forced_variant;
-	return f >= 0 && f < kDwtVariants ? f : P.variant[l];
+	return f >= 0 && f < kDwtVariants && P.tiles[l][f] > 0 ? f : P.variant[l];
 }
 
 // One transform level, autotuned: every variant `reps` times with HIP events

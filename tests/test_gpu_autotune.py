@@ -1,9 +1,11 @@
 """Transform tile shapes and the autotuner (MI355X).
 
-Both tile shapes of each transform direction give the same coefficients bit
-for bit, so a batch of uneven frames (1280x720 4:2:0, 640x480 4:4:4,
+Every tile shape of each transform direction (forward 128x56, 64x56, and
+128x48 and 128x40 without the LDS table; inverse 64x64, 32x32 and 64x56)
+gives the same
+coefficients bit for bit, so a batch of uneven frames (1280x720 4:2:0, 640x480 4:4:4,
 1920x1080 4:2:0, FP16 and FP32) encodes and decodes to the same bitstreams and
-planes with either shape forced on every level; pyrowave_hip_autotune_batch
+planes with any shape forced on every level; pyrowave_hip_autotune_batch
 returns those same results while it times the shapes, reports a valid shape
 per level, and later calls (which use the tuned shapes) still agree.
 """
@@ -47,15 +49,15 @@ def test_tile_shapes_agree_and_autotune(precision):
     results = {}
     dev = pw.Device()
     try:
-        for mode in ("shape0", "shape1", "autotune", "tuned"):
-            dev.force_tile_variant({"shape0": 0, "shape1": 1}.get(mode, -1))
+        for mode in ("shape0", "shape1", "shape2", "shape3", "autotune", "tuned"):
+            dev.force_tile_variant({"shape0": 0, "shape1": 1, "shape2": 2, "shape3": 3}.get(mode, -1))
             encs = [pw.Encoder(dev, w, h, c, precision=precision) for w, h, c in SHAPES]
             decs = [pw.Decoder(dev, w, h, c, precision=precision) for w, h, c in SHAPES]
             try:
                 outs, tg = _targets(torch)
                 if mode == "autotune":
                     fv, iv = pw.autotune_batch(encs, frames, budgets, decs, tg, repetitions=2)
-                    assert all(v in (0, 1) for v in fv + iv), (fv, iv)
+                    assert all(v in (0, 1, 2, 3) for v in fv) and all(v in (0, 1, 2) for v in iv), (fv, iv)
                     print("autotuned shapes: forward", fv, "inverse", iv)
                 else:
                     pw.encode_batch(encs, frames, budgets)
@@ -68,7 +70,7 @@ def test_tile_shapes_agree_and_autotune(precision):
     finally:
         dev.close()
     ref = results["shape0"]
-    for mode in ("shape1", "autotune", "tuned"):
+    for mode in ("shape1", "shape2", "shape3", "autotune", "tuned"):
         for i, ((a, pa), (b, pb)) in enumerate(zip(ref, results[mode])):
             assert np.array_equal(a, b), (mode, i)
             for p in range(3):
