@@ -60,21 +60,26 @@ struct FwdTile
 	static_assert(W * 2 == kThreads && kVOut % 2 == 0, "two vertical windows per column");
 };
 
-// Inverse tile of TW x TH outputs, (TW + 8) x (TH + 8) staged, 16-sample
-// windows (8 exact outputs each).
-template <int TW, int TH, int THREADS>
+// Inverse tile of TW x TH outputs, (TW + 8) x (TH + 8) staged, WIN-sample
+// windows (WIN - 8 exact outputs each).  PRESCALE: each 1-D pass's K / 1/K
+// scaling is applied once per sample (vertical at staging, horizontal on the
+// vertical outputs) instead of once per window sample; the products are the
+// same, so the results are too.
+template <int TW, int TH, int THREADS, int WIN, bool PRESCALE>
 struct InvTile
 {
 	static constexpr int kTW = TW, kTH = TH, kThreads = THREADS;
+	static constexpr int kWin = WIN, kOut = WIN - 2 * kApron;
+	static constexpr bool kPrescale = PRESCALE;
 	static constexpr int kInW = TW + 2 * kApron, kInH = TH + 2 * kApron;
 	static constexpr int kPitch = kInW + 1;
-	static constexpr int kVSegs = TH / 8;                    // vertical windows per column
-	static constexpr int kHSegs = TW / 8;                    // horizontal windows per row
+	static constexpr int kVSegs = TH / kOut;                 // vertical windows per column
+	static constexpr int kHSegs = TW / kOut;                 // horizontal windows per row
 	static constexpr int kPass1Tasks = kInW * kVSegs;
 	static constexpr int kPass1Rounds = (kPass1Tasks + THREADS - 1) / THREADS;
 	static constexpr int kPass2Tasks = TH * kHSegs;
 	static constexpr int kPass2Rounds = (kPass2Tasks + THREADS - 1) / THREADS;
-	static_assert(TW % 8 == 0 && TH % 8 == 0, "whole windows");
+	static_assert(TW % kOut == 0 && TH % kOut == 0 && kOut % 8 == 0, "whole windows, 8-sample stores");
 	static_assert(kInW + kInH <= THREADS, "one thread per extension-map entry");
 };
 
@@ -146,21 +151,22 @@ __device__ __forceinline__ void lift_fwd(float (&v)[N])
 		v[i] = fmaf(kDelta, v[i - 1] + v[i + 1], v[i]);
 }
 
-// Inverse lifting on a 16-sample window starting at an even position, already
-// scaled by K (even) / 1/K (odd); outputs at window positions 4 .. 11.
-__device__ __forceinline__ void lift_inv16(float (&v)[16])
+// Inverse lifting on an N-sample window starting at an even position, already
+// scaled by K (even) / 1/K (odd); outputs at window positions 4 .. N-5.
+template <int N>
+__device__ __forceinline__ void lift_inv(float (&v)[N])
 {
 #pragma unroll
-	for (int i = 2; i <= 14; i += 2)
+	for (int i = 2; i <= N - 2; i += 2)
 		v[i] = fmaf(-kDelta, v[i - 1] + v[i + 1], v[i]);
 #pragma unroll
-	for (int i = 3; i <= 13; i += 2)
+	for (int i = 3; i <= N - 3; i += 2)
 		v[i] = fmaf(-kGamma, v[i - 1] + v[i + 1], v[i]);
 #pragma unroll
-	for (int i = 4; i <= 12; i += 2)
+	for (int i = 4; i <= N - 4; i += 2)
 		v[i] = fmaf(-kBeta, v[i - 1] + v[i + 1], v[i]);
 #pragma unroll
-	for (int i = 5; i <= 11; i += 2)
+	for (int i = 5; i <= N - 5; i += 2)
 		v[i] = fmaf(-kAlpha, v[i - 1] + v[i + 1], v[i]);
 }
 
@@ -479,17 +485,52 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 			const int rr = i / kQX, cc = i - rr * kQX;
 #pragma unroll
 			for (int b = 0; b < 4; b++)
-				sA[(2 * rr + (b >> 1)) * S::kPitch + 2 * cc + (b & 1)] = vals[b][k];
+			{
+				// With PRESCALE, the vertical pass's scaling once per staged
+				// sample: even rows (LL, HL) by K, odd rows (LH, HH) by 1/K.
+				const float x = S::kPrescale ? vals[b][k] * ((b >> 1) ? kInvK : kK) : vals[b][k];
+				sA[(2 * rr + (b >> 1)) * S::kPitch + 2 * cc + (b & 1)] = x;
+			}
 		}
 	}
 }
 
-template <int TW, int TH, int THREADS>
+template <typename S>
+__device__ __forceinline__ void store_u8_run(uint8_t *dst, int n_valid, const float *v)
+{
+	// v[0 .. kOut) to dst, 8-byte stores where aligned and whole.
+#pragma unroll
+	for (int c = 0; c < S::kOut / 8; c++)
+	{
+		uint8_t *d = dst + 8 * c;
+		if (8 * c + 8 <= n_valid && ((uintptr_t)d & 7) == 0)
+		{
+			uint32_t lo = 0, hi = 0;
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+			{
+				lo |= (uint32_t)unorm8_store(v[8 * c + j]) << (8 * j);
+				hi |= (uint32_t)unorm8_store(v[8 * c + 4 + j]) << (8 * j);
+			}
+			*reinterpret_cast<uint2 *>(d) = make_uint2(lo, hi);
+		}
+		else
+		{
+#pragma unroll
+			for (int j = 0; j < 8; j++)
+				if (8 * c + j < n_valid)
+					d[j] = unorm8_store(v[8 * c + j]);
+		}
+	}
+}
+
+template <int TW, int TH, int THREADS, int WIN, bool PRESCALE>
 __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
                                                          const uint16_t *__restrict__ tile_job,
                                                          const FrameState *__restrict__ frames, int nxcd)
 {
-	using S = InvTile<TW, TH, THREADS>;
+	using S = InvTile<TW, TH, THREADS, WIN, PRESCALE>;
+	constexpr int kOut = S::kOut;
 	__shared__ float sA[S::kInH * S::kPitch];
 	__shared__ int s_xmap[S::kInW];
 	__shared__ int s_ymap[S::kInH];
@@ -516,8 +557,9 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		load_bands<S, float>(J, s_ymap, s_xmap, tid, sA);
 	__syncthreads();
 
-	// Vertical pass over all TW + 8 columns; rows of the tile are windows 4..11.
-	float vout[S::kPass1Rounds][8];
+	// Vertical pass over all TW + 8 columns; window seg covers staged rows
+	// seg * kOut .. + WIN - 1, its outputs are tile rows seg * kOut .. + kOut - 1.
+	float vout[S::kPass1Rounds][kOut];
 #pragma unroll
 	for (int t = 0; t < S::kPass1Rounds; t++)
 	{
@@ -525,14 +567,20 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		if (t < S::kPass1Rounds - 1 || task < S::kPass1Tasks)
 		{
 			const int c = task % S::kInW, seg = task / S::kInW;
-			float v[16];
+			float v[WIN];
 #pragma unroll
-			for (int j = 0; j < 16; j++)
-				v[j] = sA[(seg * 8 + j) * S::kPitch + c] * ((j & 1) ? kInvK : kK);
-			lift_inv16(v);
+			for (int j = 0; j < WIN; j++)
+			{
+				const float x = sA[(seg * kOut + j) * S::kPitch + c];
+				v[j] = PRESCALE ? x : x * ((j & 1) ? kInvK : kK);
+			}
+			lift_inv(v);
+			// With PRESCALE, the horizontal pass's scaling: even columns K,
+			// odd 1/K (tile columns start at an even position).
+			const float sc = (c & 1) ? kInvK : kK;
 #pragma unroll
-			for (int j = 0; j < 8; j++)
-				vout[t][j] = v[4 + j];
+			for (int j = 0; j < kOut; j++)
+				vout[t][j] = PRESCALE ? v[4 + j] * sc : v[4 + j];
 		}
 	}
 	__syncthreads();
@@ -544,13 +592,13 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		{
 			const int c = task % S::kInW, seg = task / S::kInW;
 #pragma unroll
-			for (int j = 0; j < 8; j++)
-				sA[(seg * 8 + j) * S::kPitch + c] = vout[t][j];
+			for (int j = 0; j < kOut; j++)
+				sA[(seg * kOut + j) * S::kPitch + c] = vout[t][j];
 		}
 	}
 	__syncthreads();
 
-	// Horizontal pass: TW/8 lanes per row write TW consecutive outputs.
+	// Horizontal pass: TW / kOut lanes per row write TW consecutive outputs.
 	uint8_t *plane = nullptr;
 	long dpitch = J.dst_pitch;
 	if (J.dst_kind == kStoreU8)
@@ -565,58 +613,39 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		if (t == S::kPass2Rounds - 1 && task >= S::kPass2Tasks)
 			break;
 		const int seg = task % S::kHSegs, row = task / S::kHSegs;
-		float v[16];
+		float v[WIN];
 #pragma unroll
-		for (int j = 0; j < 16; j++)
-			v[j] = sA[row * S::kPitch + seg * 8 + j] * ((j & 1) ? kInvK : kK);
-		lift_inv16(v);
+		for (int j = 0; j < WIN; j++)
+		{
+			const float x = sA[row * S::kPitch + seg * kOut + j];
+			v[j] = PRESCALE ? x : x * ((j & 1) ? kInvK : kK);
+		}
+		lift_inv(v);
 		const int py = oy0 + row;
+		const int px0 = ox0 + seg * kOut;
 		if (J.dst_kind == kStoreU8)
 		{
 			if (py < J.R_y)
-			{
-				const int px0 = ox0 + seg * 8;
-				uint8_t *dst = plane + (long)py * dpitch + px0;
-				if (px0 + 8 <= J.R_x && ((uintptr_t)dst & 7) == 0)
-				{
-					uint32_t lo = 0, hi = 0;
-#pragma unroll
-					for (int j = 0; j < 4; j++)
-					{
-						lo |= (uint32_t)unorm8_store(v[4 + j]) << (8 * j);
-						hi |= (uint32_t)unorm8_store(v[8 + j]) << (8 * j);
-					}
-					*reinterpret_cast<uint2 *>(dst) = make_uint2(lo, hi);
-				}
-				else
-				{
-#pragma unroll
-					for (int j = 4; j < 12; j++)
-					{
-						if (px0 + j - 4 < J.R_x)
-							dst[j - 4] = unorm8_store(v[j]);
-					}
-				}
-			}
+				store_u8_run<S>(plane + (long)py * dpitch + px0, J.R_x - px0, v + 4);
 		}
 		else if (py < J.A_y)
 		{
-			const int px0 = ox0 + seg * 8;
 			const long idx = (long)py * dpitch + px0;
-			if (J.dst_kind == kStoreF32 && px0 + 8 <= J.A_x)
+			if (J.dst_kind == kStoreF32 && px0 + kOut <= J.A_x)
 			{
 				// 32-byte aligned: px0 is a multiple of 8, the pitch of 32.
 				float4 *d = reinterpret_cast<float4 *>(static_cast<float *>(J.dst) + idx);
-				d[0] = make_float4(v[4], v[5], v[6], v[7]);
-				d[1] = make_float4(v[8], v[9], v[10], v[11]);
+#pragma unroll
+				for (int q = 0; q < kOut / 4; q++)
+					d[q] = make_float4(v[4 + 4 * q], v[5 + 4 * q], v[6 + 4 * q], v[7 + 4 * q]);
 			}
 			else
 			{
 #pragma unroll
-				for (int j = 4; j < 12; j++)
+				for (int j = 0; j < kOut; j++)
 				{
-					if (px0 + j - 4 < J.A_x)
-						store_kind(J.dst, J.dst_kind, idx + j - 4, v[j]);
+					if (px0 + j < J.A_x)
+						store_kind(J.dst, J.dst_kind, idx + j, v[4 + j]);
 				}
 			}
 		}
@@ -649,14 +678,17 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 	if (tiles <= 0)
 		return;
 	if (variant == 1)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128>), dim3(tiles), dim3(128), 0,
-		                   stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128, 16, false>), dim3(tiles),
+		                   dim3(128), 0, stream, jobs, tile_job, frames, xcd_count());
 	else if (variant == 2)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 256>), dim3(tiles), dim3(256), 0,
-		                   stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 256, 16, false>), dim3(tiles),
+		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_count());
+	else if (variant == 3)
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[3], kInvVariantTileH[3], 256, 24, true>), dim3(tiles),
+		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_count());
 	else
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[0], kInvVariantTileH[0], 256>), dim3(tiles), dim3(256), 0,
-		                   stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[0], kInvVariantTileH[0], 256, 16, false>), dim3(tiles),
+		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_count());
 }
 
 int xcd_count()

@@ -1,7 +1,8 @@
 """Transform tile shapes and the autotuner (MI355X).
 
 Every tile shape of each transform direction (forward 128x56, 64x56, and
-128x48 and 128x40 without the LDS table; inverse 64x64, 32x32 and 64x56)
+128x48 and 128x40 without the LDS table; inverse 64x64, 32x32, 64x56 and
+64x48 with 24-sample windows and once-per-sample scaling)
 gives the same
 coefficients bit for bit, so a batch of uneven frames (1280x720 4:2:0, 640x480 4:4:4,
 1920x1080 4:2:0, FP16 and FP32) encodes and decodes to the same bitstreams and
@@ -57,7 +58,7 @@ def test_tile_shapes_agree_and_autotune(precision):
                 outs, tg = _targets(torch)
                 if mode == "autotune":
                     fv, iv = pw.autotune_batch(encs, frames, budgets, decs, tg, repetitions=2)
-                    assert all(v in (0, 1, 2, 3) for v in fv) and all(v in (0, 1, 2) for v in iv), (fv, iv)
+                    assert all(v in (0, 1, 2, 3) for v in fv + iv), (fv, iv)
                     print("autotuned shapes: forward", fv, "inverse", iv)
                 else:
                     pw.encode_batch(encs, frames, budgets)
