@@ -143,12 +143,13 @@ PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
 
 /* Autotuner for the transform tile shapes (forward 128x56, 64x56, 128x48 or
- * 128x40 samples per workgroup, inverse 64x64, 32x32 or 64x56).  Encodes the batch
+ * 128x40 samples per workgroup, inverse 64x64, 32x32, 64x56 or 64x48).  Encodes the batch
  * (and, with decoders, decodes it from the encoders' results) like
  * pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch, timing every
  * shape of every transform level `repetitions` times (HIP events on the
- * stream; synchronous) and keeping the fastest per level for this batch
- * composition and for later batches of the same level shapes on the device.
+ * stream; synchronous) and keeping the fastest per level (the default shape 0
+ * unless another is at least 2% faster) for this batch composition and for
+ * later batches of the same level shapes on the device.
  * All shapes give the same coefficients bit for bit, so the call's results
  * are those of a plain encode / decode.  The chosen shape index per level
  * (0 = the larger tile) is written to forward_variants[5] / inverse_variants[5]
@@ -159,7 +160,7 @@ pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decode
                             uint32_t count, uint32_t repetitions, void *hip_stream, int *forward_variants,
                             int *inverse_variants);
 
-/* Forces one transform tile shape (forward 0-3, inverse 0-2; a shape a
+/* Forces one transform tile shape (forward 0-3, inverse 0-3; a shape a
  * direction does not have leaves that direction as it was) for every level; -1 returns to the autotuned / default shapes.
  * For tests and comparisons. */
 PYROWAVE_PUBLIC_API void

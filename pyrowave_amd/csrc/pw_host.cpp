@@ -742,9 +742,15 @@ int level_variant(const Plan &P, int l)
 	return f >= 0 && f < kDwtVariants && P.tiles[l][f] > 0 ? f : P.variant[l];
 }
 
+// A candidate shape replaces the default one (variant 0) only when it is this
+// much faster: shapes closer than that are within launch-to-launch noise, and
+// keeping the default there makes the choice the same from run to run.
+constexpr float kTuneMargin = 0.02f;
+
 // One transform level, autotuned: every variant `reps` times with HIP events
-// around each launch; the fastest (minimum over its launches) is kept.  The
-// last launch leaves the level's output in place for the next level.
+// around each launch; the fastest (minimum over its launches) is kept if it
+// beats the default by kTuneMargin.  The last launch leaves the level's output
+// in place for the next level.
 template <typename Launch>
 void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Launch launch)
 {
@@ -758,7 +764,7 @@ void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Laun
 		return;
 	}
 	int best = P.variant[l];
-	float best_ms = 0.0f;
+	float best_ms = 0.0f, default_ms = -1.0f;
 	bool have = false;
 	for (int v = 0; v < kDwtVariants; v++)
 	{
@@ -774,6 +780,8 @@ void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Laun
 			if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
 				v_ms = r == 0 ? ms : std::min(v_ms, ms);
 		}
+		if (v == 0)
+			default_ms = v_ms;
 		if (!have || v_ms < best_ms)
 		{
 			best = v;
@@ -781,6 +789,8 @@ void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Laun
 			have = true;
 		}
 	}
+	if (best != 0 && default_ms >= 0.0f && best_ms > default_ms * (1.0f - kTuneMargin))
+		best = 0;
 	// The chosen variant runs last, so its output is the one the next level
 	// reads (all variants write the same values).
 	launch(best);

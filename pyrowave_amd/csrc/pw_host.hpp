@@ -243,7 +243,8 @@ bool run_graph(Plan &plan, hipStream_t stream);
 // Autotuner: while Device::tune_reps > 0, run_encode / run_decode time every
 // tile variant of each transform level (`tune_reps` launches each, HIP
 // events on the launch stream, synchronous) and keep the fastest per level
-// in the plan and in Device::tuned_variant.  A level's launch does not modify
+// (the default shape unless another is 2% faster) in the plan and in
+// Device::tuned_variant.  A level's launch does not modify
 // its own inputs, so repeating it leaves the call's results valid.
 // Key of Device::tuned_variant.
 uint64_t tune_key(bool encoder, int level, int tiles0);
