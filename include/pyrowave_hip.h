@@ -146,8 +146,8 @@ pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
  * 128x40 samples per workgroup, inverse 64x64, 32x32, 64x56 or 64x48).  Encodes the batch
  * (and, with decoders, decodes it from the encoders' results) like
  * pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch, timing every
- * shape of every transform level `repetitions` times (HIP events on the
- * stream; synchronous) and keeping the fastest per level (the default shape 0
+ * shape of every transform level over three rounds of `repetitions` launches
+ * back to back (HIP events on the stream; synchronous) and keeping the fastest per level (the default shape 0
  * unless another is at least 2% faster) for this batch composition and for
  * later batches of the same level shapes on the device.
  * All shapes give the same coefficients bit for bit, so the call's results

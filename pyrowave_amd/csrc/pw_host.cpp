@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 
 namespace pw
 {
@@ -747,10 +748,17 @@ int level_variant(const Plan &P, int l)
 // keeping the default there makes the choice the same from run to run.
 constexpr float kTuneMargin = 0.02f;
 
-// One transform level, autotuned: every variant `reps` times with HIP events
-// around each launch; the fastest (minimum over its launches) is kept if it
-// beats the default by kTuneMargin.  The last launch leaves the level's output
-// in place for the next level.
+// Rounds of the autotuner's measurement per level (each round times every
+// variant once, in turn, so clock and cache drift fall on all of them alike).
+constexpr int kTuneRounds = 3;
+
+// One transform level, autotuned: in each round, every variant launches
+// `reps` times back to back between two HIP events (a level's launch reads
+// inputs it does not write, so repeating it is harmless); a variant's time is
+// its best round divided by `reps`, which prices launches as the pipeline
+// issues them rather than one isolated dispatch.  The fastest is kept if it
+// beats the default by kTuneMargin.  The last launch leaves the level's
+// output in place for the next level.
 template <typename Launch>
 void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Launch launch)
 {
@@ -763,32 +771,31 @@ void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Laun
 			(void)hipEventDestroy(e0);
 		return;
 	}
-	int best = P.variant[l];
-	float best_ms = 0.0f, default_ms = -1.0f;
-	bool have = false;
+	float v_ms[kDwtVariants];
 	for (int v = 0; v < kDwtVariants; v++)
-	{
-		if (P.tiles[l][v] == 0)
-			continue;
-		float v_ms = 0.0f;
-		for (int r = 0; r < reps; r++)
+		v_ms[v] = std::numeric_limits<float>::infinity();
+	for (int r = 0; r < kTuneRounds; r++)
+		for (int v = 0; v < kDwtVariants; v++)
 		{
+			if (P.tiles[l][v] == 0)
+				continue;
 			(void)hipEventRecord(e0, stream);
-			launch(v);
+			for (int k = 0; k < reps; k++)
+				launch(v);
 			(void)hipEventRecord(e1, stream);
 			float ms = 0.0f;
 			if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
-				v_ms = r == 0 ? ms : std::min(v_ms, ms);
+				v_ms[v] = std::min(v_ms[v], ms / float(reps));
 		}
-		if (v == 0)
-			default_ms = v_ms;
-		if (!have || v_ms < best_ms)
+	int best = P.variant[l];
+	float best_ms = std::numeric_limits<float>::infinity();
+	for (int v = 0; v < kDwtVariants; v++)
+		if (P.tiles[l][v] > 0 && v_ms[v] < best_ms)
 		{
 			best = v;
-			best_ms = v_ms;
-			have = true;
+			best_ms = v_ms[v];
 		}
-	}
+	const float default_ms = P.tiles[l][0] > 0 ? v_ms[0] : -1.0f;
 	if (best != 0 && default_ms >= 0.0f && best_ms > default_ms * (1.0f - kTuneMargin))
 		best = 0;
 	// The chosen variant runs last, so its output is the one the next level

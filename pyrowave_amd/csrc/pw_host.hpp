@@ -241,8 +241,9 @@ void run_decode(Plan &plan, hipStream_t stream, bool timed = true);
 // Replays the plan's captured launch sequence (capturing it on first use).
 bool run_graph(Plan &plan, hipStream_t stream);
 // Autotuner: while Device::tune_reps > 0, run_encode / run_decode time every
-// tile variant of each transform level (`tune_reps` launches each, HIP
-// events on the launch stream, synchronous) and keep the fastest per level
+// tile variant of each transform level (three rounds of `tune_reps` launches
+// back to back, HIP events on the launch stream, synchronous) and keep the
+// fastest per level
 // (the default shape unless another is 2% faster) in the plan and in
 // Device::tuned_variant.  A level's launch does not modify
 // its own inputs, so repeating it leaves the call's results valid.
