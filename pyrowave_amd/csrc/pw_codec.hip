@@ -85,21 +85,6 @@ __device__ __forceinline__ float f16_bits_to_f32(uint32_t h)
 	return __half2float(__ushort_as_half((unsigned short)h));
 }
 
-__device__ __forceinline__ float load_band(const BandJob &J, long idx)
-{
-	if (J.kind == kStoreF16)
-		return __half2float(static_cast<const __half *>(J.data)[idx]);
-	return static_cast<const float *>(J.data)[idx];
-}
-
-__device__ __forceinline__ void store_band(const BandJob &J, long idx, float v)
-{
-	if (J.kind == kStoreF16)
-		static_cast<__half *>(J.data)[idx] = __float2half_rn(v);
-	else
-		static_cast<float *>(J.data)[idx] = v;
-}
-
 // Cross-lane moves inside 8-lane clusters with DPP: quad_perm [1,0,3,2] and
 // [2,3,0,1] swap neighbours at distance 1 and 2, row_half_mirror pairs lane i
 // with lane 7-i.  After the two quad steps all four lanes of a quad hold the
@@ -965,11 +950,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	if (!missing)
 	{
 		const uint32_t *payload = F.dec_payload + off;
+		// The first kBlockThreads words (one per lane; most blocks are
+		// shorter) load together with the header word that gives the
+		// length, so a typical block arrives in one round trip.
+		const uint32_t first = (uint32_t)tid < payload_words - off ? payload[tid] : 0u;
 		const uint32_t w0 = payload[0];
 		uint32_t nwords = (w0 >> 16) & 0xfffu;
 		nwords = nwords < (uint32_t)kMaxBlockWords ? nwords : (uint32_t)kMaxBlockWords;
 		nwords = nwords < payload_words - off ? nwords : payload_words - off;
-		for (uint32_t i = tid; i < nwords; i += kBlockThreads)
+		if ((uint32_t)tid < nwords)
+			s_pay[tid] = first;
+		for (uint32_t i = tid + kBlockThreads; i < nwords; i += kBlockThreads)
 			s_pay[i] = payload[i];
 		__syncthreads();
 		const BlockBytes byte_at{s_pay, payload, nwords * 4, (payload_words - off) * 4};
@@ -1084,9 +1075,32 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			}
 		}
 	}
+	// Both rows of the 4x2 subblock go out as one vector store each (band
+	// storage is padded to whole 32x32 blocks, so they are in bounds and
+	// aligned): sample p sits at row p & 1, column p >> 1.
+	const long i0 = (long)y * J.pitch + x;
+	if (J.kind == kStoreF16)
+	{
+		__half *b16 = static_cast<__half *>(J.data);
+		uint2 r[2];
 #pragma unroll
-	for (int p = 0; p < 8; p++)
-		store_band(J, (long)(y + (p & 1)) * J.pitch + x + (p >> 1), v[p]);
+		for (int row = 0; row < 2; row++)
+		{
+			const uint32_t h0 = __half_as_ushort(__float2half_rn(v[row]));
+			const uint32_t h1 = __half_as_ushort(__float2half_rn(v[row + 2]));
+			const uint32_t h2 = __half_as_ushort(__float2half_rn(v[row + 4]));
+			const uint32_t h3 = __half_as_ushort(__float2half_rn(v[row + 6]));
+			r[row] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
+		}
+		*reinterpret_cast<uint2 *>(b16 + i0) = r[0];
+		*reinterpret_cast<uint2 *>(b16 + i0 + J.pitch) = r[1];
+	}
+	else
+	{
+		float *b32 = static_cast<float *>(J.data);
+		*reinterpret_cast<float4 *>(b32 + i0) = make_float4(v[0], v[2], v[4], v[6]);
+		*reinterpret_cast<float4 *>(b32 + i0 + J.pitch) = make_float4(v[1], v[3], v[5], v[7]);
+	}
 }
 } // namespace
 
