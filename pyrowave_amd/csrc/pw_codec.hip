@@ -739,8 +739,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	const BandJob &J = C.J;
 	FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
-	const int quant = F.quant[C.b];
-	const uint32_t words_expected = F.words[(long)C.b * 16 + quant];
+	// The block's 16 word counts load with its quant level (one round trip
+	// instead of two dependent ones).
+	const uint4 *const w16 = reinterpret_cast<const uint4 *>(F.words + (long)C.b * 16);
+	const uint4 wlo = w16[0], whi = w16[1];
+	const int quant = __builtin_amdgcn_readfirstlane(F.quant[C.b]);
+	const uint32_t wpairs[8] = {wlo.x, wlo.y, wlo.z, wlo.w, whi.x, whi.y, whi.z, whi.w};
+	const uint32_t words_expected = (wpairs[quant >> 1] >> (16 * (quant & 1))) & 0xffffu;
 
 	if (words_expected == 0)
 	{
@@ -938,11 +943,24 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	const int x = C.bx * 32 + 8 * (k8 & 3) + 4 * (s >> 2);
 	const int y = C.by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
 
-	uint32_t off = F.dec_offsets[(long)C.b * F.dec_offset_stride];
-	bool missing = off == 0xffffffffu;
-	if (F.dec_offset_stride == 2 && F.dec_offsets[2L * C.b + 1] == 0)
-		missing = true;
+	// The frame's decode inputs are read together, and a meta pair (offset,
+	// words) as one 8-byte load: each dependent lookup costs a round trip.
+	const uint32_t *const offsets = F.dec_offsets;
+	const int offset_stride = F.dec_offset_stride;
 	const uint32_t payload_words = F.dec_payload_bytes >> 2;
+	uint32_t off;
+	bool missing;
+	if (offset_stride == 2)
+	{
+		const uint2 e = reinterpret_cast<const uint2 *>(offsets)[C.b];
+		off = e.x;
+		missing = off == 0xffffffffu || e.y == 0;
+	}
+	else
+	{
+		off = offsets[C.b];
+		missing = off == 0xffffffffu;
+	}
 	if (!missing && off >= payload_words)
 		missing = true;
 
