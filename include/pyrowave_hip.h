@@ -25,8 +25,9 @@ extern "C" {
 
 typedef enum pyrowave_hip_precision
 {
-	/* Packed FP16 lifting for levels 0-1 with FP16 band storage there; FP32
-	 * for levels 2-4.  Quantizer policy as the reference's precision >= 1. */
+	/* FP32 lifting; the high bands of levels 0-1 are stored as FP16 (as the
+	 * reference's default precision 1 stores them), everything else as FP32.
+	 * Quantizer policy as the reference's precision >= 1. */
 	PYROWAVE_HIP_PRECISION_FP16 = 1,
 	/* FP32 lifting and storage everywhere (reference precision 2). */
 	PYROWAVE_HIP_PRECISION_FP32 = 2,
@@ -143,13 +144,14 @@ PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
 
 /* Autotuner for the transform tile shapes (forward 128x56, 64x56, 128x48 or
- * 128x40 samples per workgroup, inverse 64x64, 32x32, 64x56 or 64x48).  Encodes the batch
- * (and, with decoders, decodes it from the encoders' results) like
- * pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch, timing every
- * shape of every transform level over three rounds of `repetitions` launches
- * back to back (HIP events on the stream; synchronous) and keeping the fastest per level (the default shape 0
- * unless another is at least 2% faster) for this batch composition and for
- * later batches of the same level shapes on the device.
+ * 128x40 samples per workgroup, inverse 64x64, 32x32, 64x56 or 64x48).
+ * Encodes the batch (and, with decoders, decodes it from the encoders'
+ * results) like pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch,
+ * timing every shape of every transform level over three rounds of
+ * `repetitions` launches back to back (HIP events on the stream; synchronous)
+ * and keeping the fastest per level (the default shape 0 unless another is at
+ * least 2% faster) for this batch composition and for later batches of the
+ * same level shapes on the device.
  * All shapes give the same coefficients bit for bit, so the call's results
  * are those of a plain encode / decode.  The chosen shape index per level
  * (0 = the larger tile) is written to forward_variants[5] / inverse_variants[5]
@@ -161,8 +163,8 @@ pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decode
                             int *inverse_variants);
 
 /* Forces one transform tile shape (forward 0-3, inverse 0-3; a shape a
- * direction does not have leaves that direction as it was) for every level; -1 returns to the autotuned / default shapes.
- * For tests and comparisons. */
+ * direction does not have leaves that direction as it was) for every level;
+ * -1 returns to the autotuned / default shapes.  For tests and comparisons. */
 PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_force_tile_variant(pyrowave_device device, int variant);
 
@@ -375,6 +377,96 @@ pyrowave_hip_server_poll(pyrowave_hip_server server, pyrowave_hip_server_complet
 PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_server_take_bitstream(pyrowave_hip_server server, uint64_t ticket, void *data, size_t size,
                                    pyrowave_packet *packets, uint32_t max_packets);
+
+/* ---- External memory: zero-copy frames shared with other APIs -----------
+ *
+ * The reference imports memory of other APIs as a Vulkan image from an
+ * external handle and takes plane views of it (pyrowave_image_create,
+ * pyrowave_image_get_image_view, pyrowave_image_destroy: pyrowave.h:269-352,
+ * pyrowave_c.cpp:524-792).  Here shared memory is a linear device buffer
+ * passed as a POSIX file descriptor; planes inside it are described by a
+ * layout and become an encoder source (pyrowave_hip_frame) or a decoder
+ * target (pyrowave_hip_decode_target) without a copy. */
+
+typedef struct pyrowave_hip_external_memory_opaque *pyrowave_hip_external_memory;
+
+typedef enum pyrowave_hip_external_handle_type
+{
+	/* A POSIX fd of an exported device allocation (HIP or Vulkan opaque fd, or
+	 * a dma-buf). */
+	PYROWAVE_HIP_EXTERNAL_HANDLE_FD = 1,
+	PYROWAVE_HIP_EXTERNAL_HANDLE_INT_MAX = 0x7fffffff
+} pyrowave_hip_external_handle_type;
+
+typedef struct pyrowave_hip_external_memory_import_info
+{
+	pyrowave_device device;
+	/* Not consumed: the library imports a duplicate, the caller still owns
+	 * (and may close) this descriptor. */
+	int fd;
+	pyrowave_hip_external_handle_type handle_type;
+	/* Bytes of the exported allocation. */
+	uint64_t size;
+} pyrowave_hip_external_memory_import_info;
+
+typedef enum pyrowave_hip_plane_layout
+{
+	/* Y, Cb, Cr planes, one byte per sample (I420 / I444). */
+	PYROWAVE_HIP_PLANES_THREE = 0,
+	/* Y plane, then one plane of interleaved Cb, Cr bytes (NV12 / NV24);
+	 * encoder input only. */
+	PYROWAVE_HIP_PLANES_INTERLEAVED_CHROMA = 1,
+	PYROWAVE_HIP_PLANES_INT_MAX = 0x7fffffff
+} pyrowave_hip_plane_layout;
+
+typedef struct pyrowave_hip_external_image_layout
+{
+	/* Luma size (even under 4:2:0). */
+	uint32_t width;
+	uint32_t height;
+	pyrowave_chroma_subsampling chroma;
+	pyrowave_hip_plane_layout layout;
+	/* Byte offset of each plane from the start of the memory and its row
+	 * pitch in bytes (interleaved chroma: planes 0 and 1 only). */
+	uint64_t plane_offset[3];
+	uint64_t row_pitch[3];
+} pyrowave_hip_external_image_layout;
+
+/* Maps memory exported elsewhere (hipImportExternalMemory).  Replaces
+ * pyrowave_image_create (reference pyrowave.h:330-331) for this backend.
+ * PYROWAVE_ERROR_FAILED_EXTERNAL_HANDLE when the runtime refuses the handle. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_external_memory_import(const pyrowave_hip_external_memory_import_info *info,
+                                    pyrowave_hip_external_memory *memory);
+
+/* Allocates device memory that other processes or APIs can import and
+ * returns a POSIX fd for it (owned by the caller; close it when it has been
+ * handed over).  The size is rounded up to the allocation granularity
+ * (pyrowave_hip_external_memory_size).  No counterpart in the reference, which
+ * only imports; a producer that cannot allocate shareable memory itself
+ * writes into this one. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_external_memory_allocate(pyrowave_device device, uint64_t size, pyrowave_hip_external_memory *memory,
+                                      int *exported_fd);
+
+/* Device address and size of the mapping (for the caller's own copies). */
+PYROWAVE_PUBLIC_API void *pyrowave_hip_external_memory_device_address(pyrowave_hip_external_memory memory);
+PYROWAVE_PUBLIC_API uint64_t pyrowave_hip_external_memory_size(pyrowave_hip_external_memory memory);
+
+/* Encoder source / decoder target over planes in the memory; every plane must
+ * lie inside it (PYROWAVE_ERROR_INVALID_ARGUMENT otherwise).  Replaces
+ * pyrowave_image_get_image_view (reference pyrowave.h:347-349). */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_external_memory_frame(pyrowave_hip_external_memory memory, const pyrowave_hip_external_image_layout *layout,
+                                   pyrowave_hip_frame *frame);
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_external_memory_target(pyrowave_hip_external_memory memory,
+                                    const pyrowave_hip_external_image_layout *layout,
+                                    pyrowave_hip_decode_target *target);
+
+/* Unmaps and releases the memory (replaces pyrowave_image_destroy, reference
+ * pyrowave.h:351).  Encodes and decodes that use it must have completed. */
+PYROWAVE_PUBLIC_API void pyrowave_hip_external_memory_destroy(pyrowave_hip_external_memory memory);
 
 #ifdef __cplusplus
 }

@@ -121,6 +121,15 @@ class ServerCompletion(C.Structure):
                 ("done_ns", C.c_uint64)]
 
 
+class ExternalMemoryImportInfo(C.Structure):
+    _fields_ = [("device", C.c_void_p), ("fd", C.c_int), ("handle_type", C.c_int), ("size", C.c_uint64)]
+
+
+class ExternalImageLayout(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("chroma", C.c_int), ("layout", C.c_int),
+                ("plane_offset", C.c_uint64 * 3), ("row_pitch", C.c_uint64 * 3)]
+
+
 MESSAGE_CB = C.CFUNCTYPE(None, C.c_void_p, C.c_char_p)
 
 _lib = None
@@ -152,6 +161,10 @@ EXPORTS = (
     "pyrowave_hip_server_create", "pyrowave_hip_server_destroy", "pyrowave_hip_server_session_footprint",
     "pyrowave_hip_server_open_session", "pyrowave_hip_server_close_session", "pyrowave_hip_server_submit",
     "pyrowave_hip_server_poll", "pyrowave_hip_server_take_bitstream",
+    "pyrowave_hip_external_memory_import", "pyrowave_hip_external_memory_allocate",
+    "pyrowave_hip_external_memory_device_address", "pyrowave_hip_external_memory_size",
+    "pyrowave_hip_external_memory_frame", "pyrowave_hip_external_memory_target",
+    "pyrowave_hip_external_memory_destroy",
 )
 
 
@@ -232,6 +245,14 @@ def load():
                                            C.POINTER(C.c_uint64)]),
         "pyrowave_hip_server_poll": (I, [P, C.POINTER(ServerCompletion), U32, C.POINTER(U32), U32]),
         "pyrowave_hip_server_take_bitstream": (I, [P, C.c_uint64, P, SZ, C.POINTER(Packet), U32]),
+        "pyrowave_hip_external_memory_import": (I, [C.POINTER(ExternalMemoryImportInfo), C.POINTER(P)]),
+        "pyrowave_hip_external_memory_allocate": (I, [P, C.c_uint64, C.POINTER(P), C.POINTER(I)]),
+        "pyrowave_hip_external_memory_device_address": (P, [P]),
+        "pyrowave_hip_external_memory_size": (C.c_uint64, [P]),
+        "pyrowave_hip_external_memory_frame": (I, [P, C.POINTER(ExternalImageLayout), C.POINTER(HipFrame)]),
+        "pyrowave_hip_external_memory_target": (I, [P, C.POINTER(ExternalImageLayout),
+                                                    C.POINTER(HipDecodeTarget)]),
+        "pyrowave_hip_external_memory_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -571,6 +592,68 @@ def psnr_hvs_m_db(totals, total_pixels, full_range=True):
     out = np.zeros(16, np.float64)
     load().pyrowave_hip_psnr_hvs_m_db(t.ctypes.data, total_pixels, full_range, out.ctypes.data)
     return out
+
+
+EXTERNAL_HANDLE_FD = 1
+PLANES_THREE, PLANES_INTERLEAVED_CHROMA = 0, 1
+
+
+def external_layout(width, height, chroma, offsets, pitches, layout=PLANES_THREE):
+    L = ExternalImageLayout()
+    L.width, L.height, L.chroma, L.layout = width, height, chroma, layout
+    for i in range(3):
+        L.plane_offset[i] = offsets[i] if i < len(offsets) else 0
+        L.row_pitch[i] = pitches[i] if i < len(pitches) else 0
+    return L
+
+
+class ExternalMemory:
+    """Device memory shared with other APIs or processes through a POSIX fd
+    (pyrowave_hip_external_memory_*): `allocate` makes exportable memory and
+    returns its fd, `import_fd` maps an fd exported elsewhere."""
+
+    def __init__(self, handle, fd=-1):
+        self.handle = handle
+        self.fd = fd  # exported fd (allocate only); the caller closes it
+
+    @classmethod
+    def allocate(cls, device, size):
+        h, fd = C.c_void_p(), C.c_int(-1)
+        _check(load().pyrowave_hip_external_memory_allocate(device.handle, size, C.byref(h), C.byref(fd)),
+               "external_memory_allocate")
+        return cls(h, fd.value)
+
+    @classmethod
+    def import_fd(cls, device, fd, size):
+        info = ExternalMemoryImportInfo(device.handle, fd, EXTERNAL_HANDLE_FD, size)
+        h = C.c_void_p()
+        _check(load().pyrowave_hip_external_memory_import(C.byref(info), C.byref(h)), "external_memory_import")
+        return cls(h)
+
+    @property
+    def address(self):
+        return load().pyrowave_hip_external_memory_device_address(self.handle)
+
+    @property
+    def size(self):
+        return load().pyrowave_hip_external_memory_size(self.handle)
+
+    def frame(self, layout):
+        f = HipFrame()
+        _check(load().pyrowave_hip_external_memory_frame(self.handle, C.byref(layout), C.byref(f)),
+               "external_memory_frame")
+        return f
+
+    def target(self, layout):
+        t = HipDecodeTarget()
+        _check(load().pyrowave_hip_external_memory_target(self.handle, C.byref(layout), C.byref(t)),
+               "external_memory_target")
+        return t
+
+    def close(self):
+        if self.handle:
+            load().pyrowave_hip_external_memory_destroy(self.handle)
+            self.handle = None
 
 
 class Server:

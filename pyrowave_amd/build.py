@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpyrowave-shared.so")
-SOURCES = ["pw_dwt.hip", "pw_codec.hip", "pw_packets.hip", "pw_psnr.hip", "pw_host.cpp", "pw_capi.cpp", "pw_server.cpp"]
+SOURCES = ["pw_dwt.hip", "pw_codec.hip", "pw_packets.hip", "pw_psnr.hip", "pw_host.cpp", "pw_capi.cpp", "pw_server.cpp", "pw_external.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PYROWAVE_OFFLOAD_ARCH", "gfx950")

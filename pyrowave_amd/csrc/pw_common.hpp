@@ -164,7 +164,6 @@ struct DwtJob
 	int32_t tile_base;
 	int32_t frame;      // frame slot
 	int32_t component;
-	int32_t fp16_math;  // packed FP16 lifting
 };
 
 // One transmitted band of one frame, for the per-32x32-block kernels.

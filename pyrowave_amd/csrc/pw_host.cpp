@@ -531,7 +531,6 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 				J.band_pitch = B0.pitch;
 				J.bw = B0.bw;
 				J.bh = B0.bh;
-				J.fp16_math = J.band_kind == kStoreF16;
 				for (int b = 0; b < kBands; b++)
 					J.band[b] = band_ptr(c, l, b);
 				if (encoder)
