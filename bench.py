@@ -3,7 +3,7 @@
 
 Workload (the headline configuration): a seeded 60-frame 3840x2160 4:2:0
 sequence (3 px/frame pan, 5 % of rectangles redrawn per frame), 1,600,000
-bytes per frame.  A step encodes and decodes three batches of 16 frames that are
+bytes per frame.  A step encodes and decodes two batches of 32 frames that are
 already resident in HBM, each on its own stream so the two overlap on the
 device: pyrowave_hip_encode_batch, then pyrowave_hip_decode_encoded_batch
 straight from the encoders' device results.  The transform rooflines and the
@@ -41,8 +41,8 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--streams", type=int, default=3, help="batches in flight, one stream each")
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--streams", type=int, default=2, help="batches in flight, one stream each")
     ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the oracle CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
@@ -60,14 +60,16 @@ def parse():
     return ap.parse_args()
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "traffic_4k420_fp16_b16.txt")
+def traffic_profile(batch):
+    return os.path.join(ROOT, "profiles", f"traffic_4k420_fp16_b{batch}.txt")
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC
-    summary (a line `traffic_bytes_per_launch <kernel> <bytes>`), or None."""
+def measured_traffic(kernel, batch):
+    """HBM bytes per launch of `kernel` at this batch size from the committed
+    rocprofv3 PMC summary (a line `traffic_bytes_per_launch <kernel> <bytes>`),
+    or None when no profile of that batch size exists."""
     try:
-        with open(TRAFFIC_PROFILE) as f:
+        with open(traffic_profile(batch)) as f:
             for line in f:
                 parts = line.split()
                 if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
@@ -444,7 +446,8 @@ def main():
         us_per_launch = ms * 1e3 / max(launches, 1)
         achieved = bytes_per_launch / (us_per_launch * 1e-6) / 1e9
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(kernel),
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(kernel, args.batch),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
                 "launches_timed": launches}
 
