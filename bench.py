@@ -13,9 +13,11 @@ kernel durations are not stretched by the overlap.
 Reported: µs per frame of the FP16-storage batched path (value), the plain
 FP32 single-frame path beside it, ΔPSNR between the two, the byte budget
 check, per-batch p50/p99 latency, the transform rooflines from HIP events
-recorded on the launch stream, the serving loop (64 mixed sessions: latency
-p50/p99 at cadence and saturated frames/s), and the oracle (single-threaded
-CPU restatement) timed on a bounded sample as the CPU baseline.
+recorded on the launch stream, one batch of 32 uneven frames (720p to 4K,
+4:2:0 and 4:4:4) eager and as a replayed hipGraph, the serving loop (64 mixed
+sessions: latency p50/p99 at cadence and saturated frames/s), and the oracle
+(single-threaded CPU restatement) timed on a bounded sample as the CPU
+baseline.
 """
 import argparse
 import json
@@ -483,6 +485,54 @@ def main():
     single = {"eager": single_1080p(False), "graph": single_1080p(True)}
     print(f"[bench] single-frame graphs done", file=sys.stderr, flush=True)
 
+    # One batch of 32 uneven frames (8 each of 1280x720 4:2:0, 1920x1080 4:2:0,
+    # 2560x1440 4:4:4, 3840x2160 4:2:0; crops of the resident sequence, the
+    # 4:4:4 chroma planes luma crops of neighbouring frames), 1.6 bits/pixel,
+    # FP16, tile shapes autotuned on the batch: eager launches vs a replayed
+    # hipGraph, µs per batch.
+    def uneven_batch(graphs, steps=20, warmup=3):
+        shapes = [(1280, 720, pw.CHROMA_420), (1920, 1080, pw.CHROMA_420), (2560, 1440, pw.CHROMA_444),
+                  (3840, 2160, pw.CHROMA_420)]
+        d2 = pw.Device()
+        d2.set_timing(False)
+        encs, decs, frs, tgs, budgets, keep = [], [], [], [], [], []
+        for si, (w, h, c) in enumerate(shapes):
+            for k in range(8):
+                encs.append(pw.Encoder(d2, w, h, c, precision=pw.PRECISION_FP16))
+                decs.append(pw.Decoder(d2, w, h, c, precision=pw.PRECISION_FP16))
+                f = dev_frames[(8 * si + k) % FRAMES]
+                if c == pw.CHROMA_444:
+                    planes = [dev_frames[(8 * si + k + j) % FRAMES][0][:h, :w] for j in range(3)]
+                else:
+                    planes = [f[0][:h, :w], f[1][:h // 2, :w // 2], f[2][:h // 2, :w // 2]]
+                frs.append(pw.hip_frame([(p.data_ptr(), p.stride(0), 1) for p in planes]))
+                ch, cw = (h, w) if c == pw.CHROMA_444 else (h // 2, w // 2)
+                outs = [torch.empty((h, w), dtype=torch.uint8, device="cuda")] + \
+                       [torch.empty((ch, cw), dtype=torch.uint8, device="cuda") for _ in range(2)]
+                keep.append(outs)
+                tgs.append(pw.hip_target([(o.data_ptr(), o.stride(0)) for o in outs]))
+                budgets.append(int(w * h * 1.6 / 8))
+        pw.autotune_batch(encs, frs, budgets, decs, tgs, repetitions=5, stream=sptr)
+        d2.set_graphs(graphs)
+        for _ in range(warmup):
+            pw.encode_batch(encs, frs, budgets, stream=sptr)
+            pw.decode_encoded_batch(decs, encs, tgs, stream=sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            pw.encode_batch(encs, frs, budgets, stream=sptr)
+            pw.decode_encoded_batch(decs, encs, tgs, stream=sptr)
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t) / steps * 1e6
+        for x in encs + decs:
+            x.close()
+        d2.close()
+        return round(us, 1)
+
+    uneven = {"eager": uneven_batch(False), "graph": uneven_batch(True)}
+    uneven["frames_per_s_eager"] = round(32 / uneven["eager"] * 1e6, 1)
+    print(f"[bench] uneven batch done", file=sys.stderr, flush=True)
+
     serving = None
     if args.serving_seconds > 0:
         serving = serving_leg(pw, torch, device, dev_frames, args.serving_seconds)
@@ -540,6 +590,7 @@ def main():
         "roofline": roofline("iDWT", "k_dwt_inverse"),
         "roofline_forward": roofline("DWT", "k_dwt_forward"),
         "single_1080p_fp16_us_per_frame": single,
+        "uneven_batch_32_fp16_us_per_batch": uneven,
         "serving": serving,
         "cpu_baseline": cpu,
     }
