@@ -651,6 +651,218 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		}
 	}
 }
+
+// --- forward, streamed (shape 4) ------------------------------------------
+//
+// One wave per tile of kStreamTileW x kStreamTileH input samples and no LDS.
+// Lane l holds the even / odd input pair at columns x0 - 4 + 2l and
+// x0 - 3 + 2l; lanes 2 .. 61 produce the tile's outputs, lanes 0-1 and 62-63
+// are the apron.  The wave walks down the kStreamTileH + 8 rows of the tile's
+// window two rows at a time: each row is lifted horizontally across the lanes
+// (the neighbouring lane's sample comes from a wave-wide DPP shift), and the
+// two columns each lane then holds (horizontal low and high) are lifted
+// vertically by a delay line in registers, which completes row pair p when
+// pair p + 2 arrives.  Every sample sees the operations of lift_fwd in the
+// same order, so the coefficients equal those of the LDS-tiled shapes.
+constexpr int kStreamWaves = 4; // independent waves (tiles) per workgroup
+static_assert(kFwdVariantTileW[kFwdStreamVariant] == 2 * 64 - 2 * kApron, "one input pair per lane");
+static_assert(kFwdVariantTileH[kFwdStreamVariant] % 2 == 0, "whole row pairs");
+
+constexpr int kDppWaveShl1 = 0x130; // lane i reads lane i + 1
+constexpr int kDppWaveShr1 = 0x138; // lane i reads lane i - 1
+
+template <int Ctrl>
+__device__ __forceinline__ float dpp_wave(float v)
+{
+	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xF, 0xF, false));
+}
+
+// Horizontal lifting of one row (e, o: this lane's even and odd sample), then
+// the low / high scaling.
+__device__ __forceinline__ void hlift_row(float &e, float &o)
+{
+	o = fmaf(kAlpha, e + dpp_wave<kDppWaveShl1>(e), o);
+	e = fmaf(kBeta, dpp_wave<kDppWaveShr1>(o) + o, e);
+	o = fmaf(kGamma, e + dpp_wave<kDppWaveShl1>(e), o);
+	e = fmaf(kDelta, dpp_wave<kDppWaveShr1>(o) + o, e);
+	e = e * kInvK;
+	o = o * kK;
+}
+
+// Vertical lifting of one column as a delay line.  Before pair n arrives, re
+// and ro hold pair n-1 as input, po and pe pair n-2 after the first two
+// lifting steps (o'(n-2), e'(n-2)), and qo the odd sample of pair n-3 after
+// the third (o''(n-3)).
+struct VLine
+{
+	float re = 0.0f, ro = 0.0f, po = 0.0f, pe = 0.0f, qo = 0.0f;
+
+	// Takes pair n (e, o); gives the finished pair n-2 scaled.
+	__device__ __forceinline__ void push(float e, float o, float &low, float &high)
+	{
+		const float o1 = fmaf(kAlpha, re + e, ro); // o'(n-1)
+		const float e1 = fmaf(kBeta, po + o1, re); // e'(n-1)
+		const float o2 = fmaf(kGamma, pe + e1, po); // o''(n-2)
+		const float e2 = fmaf(kDelta, qo + o2, pe); // e''(n-2)
+		low = e2 * kInvK;
+		high = o2 * kK;
+		re = e;
+		ro = o;
+		po = o1;
+		pe = e1;
+		qo = o2;
+	}
+};
+
+// Raw loads through global (not generic) pointers: a generic load also
+// counts on the LDS counter, and any wait for the oldest prefetched row would
+// then wait for every later one too.
+typedef const __attribute__((address_space(1))) uint8_t *GU8;
+typedef const __attribute__((address_space(1))) uint16_t *GU16;
+typedef const __attribute__((address_space(1))) uint32_t *GU32;
+
+// Every format is fetched as the aligned 32-bit word holding the sample (a
+// byte or half is extracted when it is converted): a narrower load's
+// zero-extension would be placed right after it by the compiler and wait for
+// the data.  An aligned word never crosses a page, so it stays inside the
+// allocation that holds the sample.
+template <int KIND>
+__device__ __forceinline__ uint32_t stream_raw(const void *base, uint32_t off)
+{
+	if constexpr (KIND == kStoreU8)
+		return ((GU32)base)[off >> 2];
+	else if constexpr (KIND == kStoreF16)
+		return ((GU32)base)[off >> 1];
+	else
+		return ((GU32)base)[off];
+}
+
+// The sample at element offset `off` out of its word, as a float (u8 with
+// the DC shift).
+template <int KIND>
+__device__ __forceinline__ float stream_value(uint32_t raw, uint32_t off)
+{
+	if constexpr (KIND == kStoreU8)
+		return u8_dc((raw >> (8 * (off & 3))) & 0xffu);
+	else if constexpr (KIND == kStoreF16)
+		return __half2float(__ushort_as_half((unsigned short)(raw >> (16 * (off & 1)))));
+	else
+		return __uint_as_float(raw);
+}
+
+__device__ __forceinline__ void stream_store(void *base, int kind, long idx, float v)
+{
+	if (kind == kStoreF16)
+		((__attribute__((address_space(1))) uint16_t *)base)[idx] = __half_as_ushort(__float2half_rn(v));
+	else
+		((__attribute__((address_space(1))) float *)base)[idx] = v;
+}
+
+// One tile of the streamed forward transform with source samples of format
+// KIND (fixed per wave, so the loads need no branches).
+template <int KIND>
+__device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void *src, uint32_t pitch, uint32_t step,
+                                                    int x0, int y0, int lane)
+{
+	constexpr int TH = kFwdVariantTileH[kFwdStreamVariant];
+	constexpr int kPairs = TH / 2 + kApron;
+	static_assert(kPairs % 2 == 0, "the row-pair loop is unrolled by two");
+	const uint32_t xe = __umul24((uint32_t)fwd_ext(x0 - kApron + 2 * lane, J.R_x, J.A_x), step);
+	const uint32_t xo = __umul24((uint32_t)fwd_ext(x0 - kApron + 2 * lane + 1, J.R_x, J.A_x), step);
+	const int ox = (x0 >> 1) - kApron / 2 + lane;
+	const bool store_x = lane >= kApron / 2 && lane < 64 - kApron / 2 && ox < J.bw;
+
+	// Raw words of a row pair: even / odd column of the even row, then of the
+	// odd row, and their element offsets (for the extraction).
+	auto offsets = [&](int n, uint32_t (&o)[4]) {
+		const int y = y0 - kApron + 2 * n;
+		const uint32_t r0 = __umul24((uint32_t)fwd_ext(y, J.R_y, J.A_y), pitch);
+		const uint32_t r1 = __umul24((uint32_t)fwd_ext(y + 1, J.R_y, J.A_y), pitch);
+		o[0] = r0 + xe;
+		o[1] = r0 + xo;
+		o[2] = r1 + xe;
+		o[3] = r1 + xo;
+	};
+	auto load_pair = [&](const uint32_t (&o)[4], uint32_t (&r)[4]) {
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			r[i] = stream_raw<KIND>(src, o[i]);
+	};
+	VLine lo, hi; // columns of horizontal low / high outputs
+	auto lift_pair = [&](int n, float e0, float o0, float e1, float o1) {
+		hlift_row(e0, o0);
+		hlift_row(e1, o1);
+		float ll, lh, hl, hh;
+		lo.push(e0, e1, ll, lh);
+		hi.push(o0, o1, hl, hh);
+		const int oy = (y0 >> 1) + n - kApron;
+		if (n >= kApron && store_x && oy < J.bh)
+		{
+			const long idx = (long)oy * J.band_pitch + ox;
+			stream_store(J.band[0], J.ll_kind, idx, ll);
+			stream_store(J.band[1], J.band_kind, idx, hl);
+			stream_store(J.band[2], J.band_kind, idx, lh);
+			stream_store(J.band[3], J.band_kind, idx, hh);
+		}
+	};
+	// Both buffers are converted (their loads were issued an iteration ago),
+	// refilled with the next two pairs, then lifted and stored: the loads of
+	// one iteration are in flight during the whole of the previous one's
+	// lifting, and the only wait is for loads that old.
+	uint32_t oa[4], ob[4], ra[4], rb[4];
+	offsets(0, oa);
+	offsets(1, ob);
+	load_pair(oa, ra);
+	load_pair(ob, rb);
+#pragma unroll 1
+	for (int n = 0; n < kPairs; n += 2)
+	{
+		float a[4], b[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+		{
+			a[i] = stream_value<KIND>(ra[i], oa[i]);
+			b[i] = stream_value<KIND>(rb[i], ob[i]);
+		}
+		if (n + 2 < kPairs)
+		{
+			offsets(n + 2, oa);
+			offsets(n + 3, ob);
+			load_pair(oa, ra);
+			load_pair(ob, rb);
+		}
+		// The scheduler would otherwise sink the loads towards their use.
+		__builtin_amdgcn_sched_barrier(0);
+		lift_pair(n, a[0], a[1], a[2], a[3]);
+		lift_pair(n + 1, b[0], b[1], b[2], b[3]);
+	}
+}
+
+__global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const DwtJob *__restrict__ jobs,
+                                                                          const uint16_t *__restrict__ tile_job,
+                                                                          const FrameState *__restrict__ frames,
+                                                                          int tiles, int nxcd)
+{
+	constexpr int TW = kFwdVariantTileW[kFwdStreamVariant], TH = kFwdVariantTileH[kFwdStreamVariant];
+	const int lane = threadIdx.x & 63;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	if (tile >= tiles)
+		return; // whole waves: no wave-level or workgroup barrier follows
+	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
+	const int local = tile - J.tile_base;
+	const int x0 = (local % J.tiles_x) * TW;
+	const int y0 = (local / J.tiles_x) * TH;
+	if (J.src_kind == kStoreU8)
+	{
+		const FrameState &F = frames[J.frame];
+		forward_stream_tile<kStoreU8>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
+		                              (uint32_t)F.src_step[J.component], x0, y0, lane);
+	}
+	else if (J.src_kind == kStoreF16)
+		forward_stream_tile<kStoreF16>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+	else
+		forward_stream_tile<kStoreF32>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+}
 } // namespace
 
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -658,7 +870,10 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	if (tiles <= 0)
 		return;
-	if (variant == 1)
+	if (variant == kFwdStreamVariant)
+		hipLaunchKernelGGL(k_dwt_forward_stream, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
+		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+	else if (variant == 1)
 		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[1], kFwdVariantTileH[1], true>), dim3(tiles),
 		                   dim3(2 * kFwdVariantTileW[1]), 0, stream, jobs, tile_job, frames, xcd_count());
 	else if (variant == 2)
