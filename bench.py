@@ -566,7 +566,8 @@ def main():
         "ms_per_step": round(fp16["wall"] / args.steps * 1e3, 3),
         "fp16_one_stream_us_per_frame": round(us_fp16_one, 2),
         "autotuned_tile_shapes": {"forward_levels_0_4": fp16["tuned"][0], "inverse_levels_0_4": fp16["tuned"][1],
-                                  "shapes": "forward 0: 128x56, 1: 64x56, 2: 128x48, 3: 128x40, 4: 120x64 streamed; inverse 0: 64x64, 1: 32x32, 2: 64x56, 3: 64x48"},
+                                  "shapes": "forward 0: 128x56, 1: 64x56, 2: 128x48, 3: 128x40; inverse 0: 64x64, "
+                                            "1: 32x32, 2: 64x56, 3: 64x48; both 4: 120x32 streamed"},
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

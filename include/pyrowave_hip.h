@@ -144,8 +144,9 @@ PYROWAVE_PUBLIC_API void
 pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
 
 /* Autotuner for the transform tile shapes (forward 128x56, 64x56, 128x48 or
- * 128x40 samples per workgroup, or 120x64 per wave streamed down its rows with
- * the lifting in registers; inverse 64x64, 32x32, 64x56 or 64x48).
+ * 128x40 samples per workgroup, inverse 64x64, 32x32, 64x56 or 64x48, and in
+ * both directions 120x32 per wave streamed down its rows with the lifting in
+ * registers).
  * Encodes the batch (and, with decoders, decodes it from the encoders'
  * results) like pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch,
  * timing every shape of every transform level over three rounds of
@@ -163,7 +164,7 @@ pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decode
                             uint32_t count, uint32_t repetitions, void *hip_stream, int *forward_variants,
                             int *inverse_variants);
 
-/* Forces one transform tile shape (forward 0-4, inverse 0-3; a shape a
+/* Forces one transform tile shape (forward 0-4, inverse 0-4; a shape a
  * direction does not have leaves that direction as it was) for every level;
  * -1 returns to the autotuned / default shapes.  For tests and comparisons. */
 PYROWAVE_PUBLIC_API void

@@ -323,8 +323,8 @@ class Device:
         load().pyrowave_hip_device_set_graphs(self.handle, enable)
 
     def force_tile_variant(self, variant):
-        """Every transform level uses tile shape `variant` (forward 0-4, inverse 0-3; a
-        shape one direction lacks leaves it as it was); -1 = autotuned."""
+        """Every transform level uses tile shape `variant` (0-4 in both
+        directions); -1 = autotuned."""
         load().pyrowave_hip_device_force_tile_variant(self.handle, variant)
 
     def set_timing_stages(self, stages):

@@ -34,17 +34,18 @@ constexpr int kFlat = kBuckets * kSubdiv;
 // 19 KB of LDS: eight workgroups per CU, not seven), all with 16-sample
 // windows; 3: 64 x 48 (256 threads) with 24-sample windows and each pass's
 // scaling applied once per sample (a third fewer lifting instructions).
-// Forward variant 4 streams instead: one wave per 120 x 64 tile walks down
-// its rows with lifting in registers (no LDS).
+// Variant 4 of each direction streams instead: one wave per 120 x 32 tile
+// walks down its rows with the lifting in registers (no LDS).
 constexpr int kApron = 4;
 constexpr int kFwdVariants = 5;
-constexpr int kInvVariants = 4;
+constexpr int kInvVariants = 5;
 constexpr int kDwtVariants = 5; // table slots per level (the larger count)
 constexpr int kFwdStreamVariant = 4;
+constexpr int kInvStreamVariant = 4;
 constexpr int kFwdVariantTileW[kFwdVariants] = {128, 64, 128, 128, 120};
-constexpr int kFwdVariantTileH[kFwdVariants] = {56, 56, 48, 40, 64};
-constexpr int kInvVariantTileW[kInvVariants] = {64, 32, 64, 64};
-constexpr int kInvVariantTileH[kInvVariants] = {64, 32, 56, 48};
+constexpr int kFwdVariantTileH[kFwdVariants] = {56, 56, 48, 40, 32};
+constexpr int kInvVariantTileW[kInvVariants] = {64, 32, 64, 64, 120};
+constexpr int kInvVariantTileH[kInvVariants] = {64, 32, 56, 48, 32};
 
 // Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
 // sign bits, 8 B header and 3 B per 8x8 block, rounded up to words.

@@ -772,21 +772,25 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 	const int ox = (x0 >> 1) - kApron / 2 + lane;
 	const bool store_x = lane >= kApron / 2 && lane < 64 - kApron / 2 && ox < J.bw;
 
-	// Raw words of a row pair: even / odd column of the even row, then of the
-	// odd row, and their element offsets (for the extraction).
-	auto offsets = [&](int n, uint32_t (&o)[4]) {
+	// A row pair is addressed by its two row bases (uniform: scalar registers)
+	// plus this lane's even / odd column offsets; its raw words are, in order,
+	// even and odd column of the even row, then of the odd row.
+	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
 		const int y = y0 - kApron + 2 * n;
-		const uint32_t r0 = __umul24((uint32_t)fwd_ext(y, J.R_y, J.A_y), pitch);
-		const uint32_t r1 = __umul24((uint32_t)fwd_ext(y + 1, J.R_y, J.A_y), pitch);
-		o[0] = r0 + xe;
-		o[1] = r0 + xo;
-		o[2] = r1 + xe;
-		o[3] = r1 + xo;
+		r0 = __umul24((uint32_t)fwd_ext(y, J.R_y, J.A_y), pitch);
+		r1 = __umul24((uint32_t)fwd_ext(y + 1, J.R_y, J.A_y), pitch);
 	};
-	auto load_pair = [&](const uint32_t (&o)[4], uint32_t (&r)[4]) {
-#pragma unroll
-		for (int i = 0; i < 4; i++)
-			r[i] = stream_raw<KIND>(src, o[i]);
+	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
+		r[0] = stream_raw<KIND>(src, r0 + xe);
+		r[1] = stream_raw<KIND>(src, r0 + xo);
+		r[2] = stream_raw<KIND>(src, r1 + xe);
+		r[3] = stream_raw<KIND>(src, r1 + xo);
+	};
+	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
+		v[0] = stream_value<KIND>(r[0], r0 + xe);
+		v[1] = stream_value<KIND>(r[1], r0 + xo);
+		v[2] = stream_value<KIND>(r[2], r1 + xe);
+		v[3] = stream_value<KIND>(r[3], r1 + xo);
 	};
 	VLine lo, hi; // columns of horizontal low / high outputs
 	auto lift_pair = [&](int n, float e0, float o0, float e1, float o1) {
@@ -809,27 +813,23 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 	// refilled with the next two pairs, then lifted and stored: the loads of
 	// one iteration are in flight during the whole of the previous one's
 	// lifting, and the only wait is for loads that old.
-	uint32_t oa[4], ob[4], ra[4], rb[4];
-	offsets(0, oa);
-	offsets(1, ob);
-	load_pair(oa, ra);
-	load_pair(ob, rb);
+	uint32_t a0, a1, b0, b1, ra[4], rb[4];
+	rows(0, a0, a1);
+	rows(1, b0, b1);
+	load_pair(a0, a1, ra);
+	load_pair(b0, b1, rb);
 #pragma unroll 1
 	for (int n = 0; n < kPairs; n += 2)
 	{
 		float a[4], b[4];
-#pragma unroll
-		for (int i = 0; i < 4; i++)
-		{
-			a[i] = stream_value<KIND>(ra[i], oa[i]);
-			b[i] = stream_value<KIND>(rb[i], ob[i]);
-		}
+		values(a0, a1, ra, a);
+		values(b0, b1, rb, b);
 		if (n + 2 < kPairs)
 		{
-			offsets(n + 2, oa);
-			offsets(n + 3, ob);
-			load_pair(oa, ra);
-			load_pair(ob, rb);
+			rows(n + 2, a0, a1);
+			rows(n + 3, b0, b1);
+			load_pair(a0, a1, ra);
+			load_pair(b0, b1, rb);
 		}
 		// The scheduler would otherwise sink the loads towards their use.
 		__builtin_amdgcn_sched_barrier(0);
@@ -863,6 +863,179 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const 
 	else
 		forward_stream_tile<kStoreF32>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 }
+
+// --- inverse, streamed (shape 4) ------------------------------------------
+//
+// The inverse counterpart of k_dwt_forward_stream: one wave per tile of
+// kInvVariantTileW x kInvVariantTileH output samples, no LDS.  Lane l holds
+// the interleaved column pair ox0 - 4 + 2l, +1 (lanes 2 .. 61 produce the
+// tile).  The wave walks down the interleaved rows two at a time: each row
+// pair brings LL and HL samples (even row) and LH and HH samples (odd row),
+// found through the decoder's symmetric extension; the two columns each lane
+// holds go through the vertical inverse lifting as a delay line (row pair p
+// completes when pair p + 2 arrives), and each completed row through the
+// horizontal inverse lifting across the lanes.  The operations and their
+// order are those of the LDS-tiled shapes (lift_inv, K / 1/K scaling before
+// each pass), so the output is the same.
+
+// Vertical inverse lifting of one column as a delay line.  Before pair n
+// arrives: ro = odd sample of pair n-1 as input, e1 = even of pair n-1 after
+// the first step, o2 = odd of pair n-2 after the second, e3 = even of pair
+// n-2 after the third.
+struct VLineInv
+{
+	float ro = 0.0f, e1 = 0.0f, o2 = 0.0f, e3 = 0.0f;
+
+	// Takes pair n (scaled input); gives the finished pair n-2.
+	__device__ __forceinline__ void push(float e, float o, float &even, float &odd)
+	{
+		const float ne1 = fmaf(-kDelta, ro + o, e);   // e1(n)
+		const float no2 = fmaf(-kGamma, e1 + ne1, ro); // o2(n-1)
+		const float ne3 = fmaf(-kBeta, o2 + no2, e1);  // e3(n-1)
+		odd = fmaf(-kAlpha, e3 + ne3, o2);             // o4(n-2)
+		even = e3;                                     // e3(n-2)
+		ro = o;
+		e1 = ne1;
+		o2 = no2;
+		e3 = ne3;
+	}
+};
+
+// Horizontal inverse lifting of one row (e, o: this lane's even and odd
+// column) after the column scaling.
+__device__ __forceinline__ void hlift_inv_row(float &e, float &o)
+{
+	e = e * kK;
+	o = o * kInvK;
+	e = fmaf(-kDelta, dpp_wave<kDppWaveShr1>(o) + o, e);
+	o = fmaf(-kGamma, e + dpp_wave<kDppWaveShl1>(e), o);
+	e = fmaf(-kBeta, dpp_wave<kDppWaveShr1>(o) + o, e);
+	o = fmaf(-kAlpha, e + dpp_wave<kDppWaveShl1>(e), o);
+}
+
+// One tile with high bands of format HKIND (the LL band is FP32).
+template <int HKIND>
+__device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const FrameState *frames, int ox0, int oy0,
+                                                    int lane)
+{
+	constexpr int TH = kInvVariantTileH[kInvStreamVariant];
+	constexpr int kPairs = TH / 2 + kApron;
+	static_assert(kPairs % 2 == 0, "the row-pair loop is unrolled by two");
+	const uint32_t pitch = (uint32_t)J.band_pitch;
+	const int xe = inv_ext(ox0 - kApron + 2 * lane, J.A_x), xo = inv_ext(ox0 - kApron + 2 * lane + 1, J.A_x);
+	// Parity is preserved: xe is even (LL / LH column), xo odd (HL / HH).
+	const uint32_t ce = (uint32_t)(xe >> 1), co = (uint32_t)(xo >> 1);
+	const int px = ox0 - kApron + 2 * lane;
+	const bool lane_out = lane >= kApron / 2 && lane < 64 - kApron / 2;
+
+	uint8_t *plane = nullptr;
+	long dpitch = J.dst_pitch;
+	if (J.dst_kind == kStoreU8)
+	{
+		plane = frames[J.frame].dst[J.component];
+		dpitch = frames[J.frame].dst_pitch[J.component];
+	}
+
+	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
+		const int y = oy0 - kApron + 2 * n;
+		r0 = __umul24((uint32_t)(inv_ext(y, J.A_y) >> 1), pitch);
+		r1 = __umul24((uint32_t)(inv_ext(y + 1, J.A_y) >> 1), pitch);
+	};
+	// Raw words: LL and HL of the even row, LH and HH of the odd row.
+	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
+		r[0] = stream_raw<kStoreF32>(J.band[0], r0 + ce);
+		r[1] = stream_raw<HKIND>(J.band[1], r0 + co);
+		r[2] = stream_raw<HKIND>(J.band[2], r1 + ce);
+		r[3] = stream_raw<HKIND>(J.band[3], r1 + co);
+	};
+	// Scaled for the vertical pass: even rows by K, odd rows by 1/K.
+	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
+		v[0] = stream_value<kStoreF32>(r[0], r0 + ce) * kK;
+		v[1] = stream_value<HKIND>(r[1], r0 + co) * kK;
+		v[2] = stream_value<HKIND>(r[2], r1 + ce) * kInvK;
+		v[3] = stream_value<HKIND>(r[3], r1 + co) * kInvK;
+	};
+	auto store_row = [&](int py, float e, float o) {
+		if (!lane_out)
+			return;
+		if (J.dst_kind == kStoreU8)
+		{
+			if (py < J.R_y)
+			{
+				uint8_t *d = plane + (long)py * dpitch + px;
+				if (px < J.R_x)
+					d[0] = unorm8_store(e);
+				if (px + 1 < J.R_x)
+					d[1] = unorm8_store(o);
+			}
+		}
+		else if (py < J.A_y)
+		{
+			const long idx = (long)py * dpitch + px;
+			if (px < J.A_x)
+				stream_store(J.dst, J.dst_kind, idx, e);
+			if (px + 1 < J.A_x)
+				stream_store(J.dst, J.dst_kind, idx + 1, o);
+		}
+	};
+	VLineInv ce_line, co_line; // the even and the odd column
+	auto lift_pair = [&](int n, const float (&v)[4]) {
+		float e0, e1, o0, o1; // rows 2p, 2p+1 of the finished pair p = n - 2; e: even column, o: odd
+		ce_line.push(v[0], v[2], e0, e1);
+		co_line.push(v[1], v[3], o0, o1);
+		if (n < kApron)
+			return;
+		hlift_inv_row(e0, o0);
+		hlift_inv_row(e1, o1);
+		const int py = oy0 - kApron + 2 * (n - 2);
+		store_row(py, e0, o0);
+		store_row(py + 1, e1, o1);
+	};
+	uint32_t a0, a1, b0, b1, ra[4], rb[4];
+	rows(0, a0, a1);
+	rows(1, b0, b1);
+	load_pair(a0, a1, ra);
+	load_pair(b0, b1, rb);
+#pragma unroll 1
+	for (int n = 0; n < kPairs; n += 2)
+	{
+		float a[4], b[4];
+		values(a0, a1, ra, a);
+		values(b0, b1, rb, b);
+		if (n + 2 < kPairs)
+		{
+			rows(n + 2, a0, a1);
+			rows(n + 3, b0, b1);
+			load_pair(a0, a1, ra);
+			load_pair(b0, b1, rb);
+		}
+		// The scheduler would otherwise sink the loads towards their use.
+		__builtin_amdgcn_sched_barrier(0);
+		lift_pair(n, a);
+		lift_pair(n + 1, b);
+	}
+}
+
+__global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const DwtJob *__restrict__ jobs,
+                                                                          const uint16_t *__restrict__ tile_job,
+                                                                          const FrameState *__restrict__ frames,
+                                                                          int tiles, int nxcd)
+{
+	constexpr int TW = kInvVariantTileW[kInvStreamVariant], TH = kInvVariantTileH[kInvStreamVariant];
+	static_assert(TW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
+	const int lane = threadIdx.x & 63;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	if (tile >= tiles)
+		return; // whole waves: no wave-level or workgroup barrier follows
+	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
+	const int local = tile - J.tile_base;
+	const int ox0 = (local % J.tiles_x) * TW;
+	const int oy0 = (local / J.tiles_x) * TH;
+	if (J.band_kind == kStoreF16)
+		inverse_stream_tile<kStoreF16>(J, frames, ox0, oy0, lane);
+	else
+		inverse_stream_tile<kStoreF32>(J, frames, ox0, oy0, lane);
+}
 } // namespace
 
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -892,7 +1065,10 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	if (tiles <= 0)
 		return;
-	if (variant == 1)
+	if (variant == kInvStreamVariant)
+		hipLaunchKernelGGL(k_dwt_inverse_stream, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
+		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+	else if (variant == 1)
 		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128, 16, false>), dim3(tiles),
 		                   dim3(128), 0, stream, jobs, tile_job, frames, xcd_count());
 	else if (variant == 2)

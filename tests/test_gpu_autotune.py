@@ -1,10 +1,10 @@
 """Transform tile shapes and the autotuner (MI355X).
 
 Every tile shape of each transform direction (forward 128x56, 64x56, and
-128x48 and 128x40 without the LDS table, and the streamed 120x64 shape that
-lifts in registers; inverse 64x64, 32x32, 64x56 and 64x48 with 24-sample
-windows and once-per-sample scaling) gives the same coefficients bit for bit
-(compared directly), so a batch of uneven frames (1280x720 4:2:0, 640x480
+128x48 and 128x40 without the LDS table; inverse 64x64, 32x32, 64x56 and
+64x48 with 24-sample windows and once-per-sample scaling; and in both
+directions the streamed 120x32 shape that lifts in registers) gives the same
+coefficients bit for bit (compared directly), so a batch of uneven frames (1280x720 4:2:0, 640x480
 4:4:4, 1920x1080 4:2:0, FP16 and FP32) encodes and decodes to the same
 bitstreams and planes with any shape forced on every level; pyrowave_hip_autotune_batch
 returns those same results while it times the shapes, reports a valid shape
@@ -62,7 +62,7 @@ def test_tile_shapes_agree_and_autotune(precision):
                 outs, tg = _targets(torch)
                 if mode == "autotune":
                     fv, iv = pw.autotune_batch(encs, frames, budgets, decs, tg, repetitions=2)
-                    assert all(v in (0, 1, 2, 3, 4) for v in fv) and all(v in (0, 1, 2, 3) for v in iv), (fv, iv)
+                    assert all(v in (0, 1, 2, 3, 4) for v in fv + iv), (fv, iv)
                     print("autotuned shapes: forward", fv, "inverse", iv)
                 else:
                     pw.encode_batch(encs, frames, budgets)
