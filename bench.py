@@ -440,6 +440,10 @@ def main():
     # One batch runs LEVELS launches of each kernel; a launch's algorithmic
     # bytes are the batch's compulsory bytes / LEVELS, its duration the
     # stage time / LEVELS, so achieved = compulsory bytes / stage time.
+    def kernels_of(base, shapes):
+        """The kernels one transform stage launches: shape 4 is the streamed kernel."""
+        return "+".join(sorted({base + "_stream" if v == 4 else base for v in shapes}))
+
     def roofline(stage, kernel):
         ms, frames = fp16["roof"][stage]
         launches = args.steps * LEVELS
@@ -588,8 +592,8 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 1)},
         "stage_us_per_frame_fp16": stage_us,
         "stage_us_per_frame_fp32_single": stage_us32,
-        "roofline": roofline("iDWT", "k_dwt_inverse"),
-        "roofline_forward": roofline("DWT", "k_dwt_forward"),
+        "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1])),
+        "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0])),
         "single_1080p_fp16_us_per_frame": single,
         "uneven_batch_32_fp16_us_per_batch": uneven,
         "serving": serving,

@@ -151,12 +151,13 @@ pyrowave_hip_device_set_graphs(pyrowave_device device, bool enable);
  * results) like pyrowave_hip_encode_batch / pyrowave_hip_decode_encoded_batch,
  * timing every shape of every transform level over three rounds of
  * `repetitions` launches back to back (HIP events on the stream; synchronous)
- * and keeping the fastest per level (the default shape 0 unless another is at
- * least 2% faster) for this batch composition and for later batches of the
+ * and keeping the fastest per level (the default, the streamed shape 4 on
+ * levels of at least 2048 of its tiles and shape 0 on smaller ones, unless
+ * another is at least 2% faster) for this batch composition and for later batches of the
  * same level shapes on the device.
  * All shapes give the same coefficients bit for bit, so the call's results
  * are those of a plain encode / decode.  The chosen shape index per level
- * (0 = the larger tile) is written to forward_variants[5] / inverse_variants[5]
+ * is written to forward_variants[5] / inverse_variants[5]
  * when they are not null.  No counterpart in the reference. */
 PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_autotune_batch(pyrowave_encoder *encoders, pyrowave_decoder *decoders, const pyrowave_hip_frame *frames,

@@ -42,6 +42,18 @@ constexpr int kInvVariants = 5;
 constexpr int kDwtVariants = 5; // table slots per level (the larger count)
 constexpr int kFwdStreamVariant = 4;
 constexpr int kInvStreamVariant = 4;
+// Shape of a level that has not been autotuned: the streamed one when the
+// level has at least kStreamMinTiles of its tiles (eight waves per CU; it is
+// the fastest on the large levels of a batch), else the tiled shape 0 (a
+// streamed wave walks its rows one after another, which a level with few
+// tiles cannot hide: one 1080p frame took 176 instead of 129 us).  The
+// autotuner keeps the default unless another shape is 2% faster.
+constexpr int kStreamMinTiles = 2048;
+static_assert(kFwdStreamVariant == kInvStreamVariant, "one streamed shape index");
+constexpr int default_dwt_variant(int stream_tiles)
+{
+	return stream_tiles >= kStreamMinTiles ? kFwdStreamVariant : 0;
+}
 constexpr int kFwdVariantTileW[kFwdVariants] = {128, 64, 128, 128, 120};
 constexpr int kFwdVariantTileH[kFwdVariants] = {56, 56, 48, 40, 32};
 constexpr int kInvVariantTileW[kInvVariants] = {64, 32, 64, 64, 120};

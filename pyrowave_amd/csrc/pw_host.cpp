@@ -638,7 +638,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 			if (jobs[l][v].size() > 65535)
 				return false;
 		}
-		variant[l] = 0;
+		variant[l] = default_dwt_variant(tiles[l][kFwdStreamVariant]);
 		auto it = D.tuned_variant.find(tune_key(encoder, l, tiles[l][0]));
 		if (it != D.tuned_variant.end())
 			variant[l] = it->second;
@@ -742,9 +742,10 @@ int level_variant(const Plan &P, int l)
 	return f >= 0 && f < kDwtVariants && P.tiles[l][f] > 0 ? f : P.variant[l];
 }
 
-// A candidate shape replaces the default one (variant 0) only when it is this
-// much faster: shapes closer than that are within launch-to-launch noise, and
-// keeping the default there makes the choice the same from run to run.
+// A candidate shape replaces the level's default one (default_dwt_variant)
+// only when it is this much faster: shapes closer than that are within launch-to-launch
+// noise, and keeping the default there makes the choice the same from run to
+// run.
 constexpr float kTuneMargin = 0.02f;
 
 // Rounds of the autotuner's measurement per level (each round times every
@@ -794,9 +795,10 @@ void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Laun
 			best = v;
 			best_ms = v_ms[v];
 		}
-	const float default_ms = P.tiles[l][0] > 0 ? v_ms[0] : -1.0f;
-	if (best != 0 && default_ms >= 0.0f && best_ms > default_ms * (1.0f - kTuneMargin))
-		best = 0;
+	const int dflt = default_dwt_variant(P.tiles[l][kFwdStreamVariant]);
+	const float default_ms = P.tiles[l][dflt] > 0 ? v_ms[dflt] : -1.0f;
+	if (best != dflt && default_ms >= 0.0f && best_ms > default_ms * (1.0f - kTuneMargin))
+		best = dflt;
 	// The chosen variant runs last, so its output is the one the next level
 	// reads (all variants write the same values).
 	launch(best);
