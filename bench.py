@@ -111,7 +111,7 @@ def short_kernel_name(full, fallback=48):
     return m.group(1) + (m.group(2) or "").replace(" ", "")
 
 
-def trace_summary(path, top=80):
+def trace_summary(path, top=250):
     """Per-(kernel, workgroup count) dispatch statistics of a rocprofv3
     kernel-trace CSV.  The workgroup count separates launches of one kernel
     over different batch sizes and transform levels."""
