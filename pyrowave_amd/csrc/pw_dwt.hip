@@ -750,7 +750,7 @@ __device__ __forceinline__ float stream_value(uint32_t raw, uint32_t off)
 		return __uint_as_float(raw);
 }
 
-__device__ __forceinline__ void stream_store(void *base, int kind, long idx, float v)
+__device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx, float v)
 {
 	if (kind == kStoreF16)
 		((__attribute__((address_space(1))) uint16_t *)base)[idx] = __half_as_ushort(__float2half_rn(v));
@@ -775,10 +775,15 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 	// A row pair is addressed by its two row bases (uniform: scalar registers)
 	// plus this lane's even / odd column offsets; its raw words are, in order,
 	// even and odd column of the even row, then of the odd row.
+	// Row offsets of the window's 2 * kPairs rows, one row per lane, computed
+	// once (the extension's modulo would otherwise run per row pair); a row
+	// pair reads its two entries with readlane.
+	static_assert(2 * kPairs <= 64, "one window row per lane");
+	const uint32_t row_off =
+	    lane < 2 * kPairs ? __umul24((uint32_t)fwd_ext(y0 - kApron + lane, J.R_y, J.A_y), pitch) : 0u;
 	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
-		const int y = y0 - kApron + 2 * n;
-		r0 = __umul24((uint32_t)fwd_ext(y, J.R_y, J.A_y), pitch);
-		r1 = __umul24((uint32_t)fwd_ext(y + 1, J.R_y, J.A_y), pitch);
+		r0 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n);
+		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
 	};
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
 		r[0] = stream_raw<KIND>(src, r0 + xe);
@@ -802,7 +807,9 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 		const int oy = (y0 >> 1) + n - kApron;
 		if (n >= kApron && store_x && oy < J.bh)
 		{
-			const long idx = (long)oy * J.band_pitch + ox;
+			// 32-bit element offsets from the uniform band bases (bands stay
+			// below 4 GiB): scalar base, one vector offset.
+			const uint32_t idx = __umul24((uint32_t)oy, (uint32_t)J.band_pitch) + (uint32_t)ox;
 			stream_store(J.band[0], J.ll_kind, idx, ll);
 			stream_store(J.band[1], J.band_kind, idx, hl);
 			stream_store(J.band[2], J.band_kind, idx, lh);
@@ -936,10 +943,14 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 		dpitch = frames[J.frame].dst_pitch[J.component];
 	}
 
+	// Band-row offsets of the window's 2 * kPairs interleaved rows, one row
+	// per lane, computed once; a row pair reads its two entries with readlane.
+	static_assert(2 * kPairs <= 64, "one window row per lane");
+	const uint32_t row_off =
+	    lane < 2 * kPairs ? __umul24((uint32_t)(inv_ext(oy0 - kApron + lane, J.A_y) >> 1), pitch) : 0u;
 	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
-		const int y = oy0 - kApron + 2 * n;
-		r0 = __umul24((uint32_t)(inv_ext(y, J.A_y) >> 1), pitch);
-		r1 = __umul24((uint32_t)(inv_ext(y + 1, J.A_y) >> 1), pitch);
+		r0 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n);
+		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
 	};
 	// Raw words: LL and HL of the even row, LH and HH of the odd row.
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
@@ -971,7 +982,7 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 		}
 		else if (py < J.A_y)
 		{
-			const long idx = (long)py * dpitch + px;
+			const uint32_t idx = __umul24((uint32_t)py, (uint32_t)dpitch) + (uint32_t)px;
 			if (px < J.A_x)
 				stream_store(J.dst, J.dst_kind, idx, e);
 			if (px + 1 < J.A_x)
