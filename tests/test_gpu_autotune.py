@@ -81,3 +81,43 @@ def test_tile_shapes_agree_and_autotune(precision):
             assert np.array_equal(a, b), (mode, i)
             for p in range(3):
                 assert np.array_equal(pa[p], pb[p]), (mode, i, p)
+
+
+ODD = [(200, 130, pw.CHROMA_444), (258, 146, pw.CHROMA_420), (130, 66, pw.CHROMA_420), (1000, 562, pw.CHROMA_420)]
+
+
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP32])
+def test_streamed_shape_on_odd_sizes(precision):
+    """The streamed shape (4) on sizes that are not multiples of the tiles, of
+    32 or of the 128 minimum (edge extension, partial tiles, tiny levels):
+    coefficients, bitstreams and decoded planes equal those of shape 0."""
+    import torch
+    inputs = [[torch.from_numpy(p).cuda() for p in synth.frame(w, h, c == pw.CHROMA_420, seed=70 + i)]
+              for i, (w, h, c) in enumerate(ODD)]
+    frames = [pw.hip_frame([(t.data_ptr(), t.stride(0), 1) for t in ts]) for ts in inputs]
+    budgets = [int(w * h * 2.0 / 8) for w, h, _ in ODD]
+    results = {}
+    dev = pw.Device()
+    try:
+        for shape in (0, 4):
+            dev.force_tile_variant(shape)
+            encs = [pw.Encoder(dev, w, h, c, precision=precision) for w, h, c in ODD]
+            decs = [pw.Decoder(dev, w, h, c, precision=precision) for w, h, c in ODD]
+            try:
+                outs = [[torch.zeros(pw.plane_shape(w, h, c, p), dtype=torch.uint8, device="cuda")
+                         for p in range(3)] for w, h, c in ODD]
+                tg = [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in oo]) for oo in outs]
+                pw.encode_batch(encs, frames, budgets)
+                pw.decode_encoded_batch(decs, encs, tg)
+                torch.cuda.synchronize()
+                results[shape] = _collect(encs, outs, budgets)
+            finally:
+                for x in encs + decs:
+                    x.close()
+    finally:
+        dev.close()
+    for i, ((a, pa, ca), (b, pb, cb)) in enumerate(zip(results[0], results[4])):
+        assert np.array_equal(ca, cb), (ODD[i], int(np.count_nonzero(ca != cb)))
+        assert np.array_equal(a, b), ODD[i]
+        for p in range(3):
+            assert np.array_equal(pa[p], pb[p]), (ODD[i], p)
