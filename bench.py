@@ -66,19 +66,25 @@ def traffic_profile(batch):
     return os.path.join(ROOT, "profiles", f"traffic_4k420_fp16_b{batch}.txt")
 
 
-def measured_traffic(kernel, batch):
-    """HBM bytes per launch of `kernel` at this batch size from the committed
-    rocprofv3 PMC summary (a line `traffic_bytes_per_launch <kernel> <bytes>`),
-    or None when no profile of that batch size exists."""
+def measured_traffic(direction, shapes, batch):
+    """HBM bytes per launch of one transform direction ("forward" or
+    "inverse") with the given tile shape per level, at this batch size: the
+    mean over the five levels of the committed rocprofv3 PMC figures (lines
+    `traffic_level <direction> <level> <shape> <bytes>`), or None when the
+    profile of that batch size, or a level's figure, is missing."""
+    table = {}
     try:
         with open(traffic_profile(batch)) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                    return int(parts[2])
+                if len(parts) == 5 and parts[0] == "traffic_level" and parts[1] == direction:
+                    table[(int(parts[2]), int(parts[3]))] = int(parts[4])
     except OSError:
-        pass
-    return None
+        return None
+    per_level = [table.get((level, shape)) for level, shape in enumerate(shapes)]
+    if not per_level or None in per_level:
+        return None
+    return int(round(sum(per_level) / len(per_level)))
 
 
 def dwt_algorithmic_bytes(fp16):
@@ -444,7 +450,7 @@ def main():
         """The kernels one transform stage launches: shape 4 is the streamed kernel."""
         return "+".join(sorted({base + "_stream" if v == 4 else base for v in shapes}))
 
-    def roofline(stage, kernel):
+    def roofline(stage, kernel, direction, shapes):
         ms, frames = fp16["roof"][stage]
         launches = args.steps * LEVELS
         per_frame = dwt_algorithmic_bytes(True)
@@ -453,7 +459,7 @@ def main():
         achieved = bytes_per_launch / (us_per_launch * 1e-6) / 1e9
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(kernel, args.batch),
+                "traffic": measured_traffic(direction, shapes, args.batch),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
                 "launches_timed": launches}
 
@@ -592,8 +598,9 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 1)},
         "stage_us_per_frame_fp16": stage_us,
         "stage_us_per_frame_fp32_single": stage_us32,
-        "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1])),
-        "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0])),
+        "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1]), "inverse", fp16["tuned"][1]),
+        "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0]), "forward",
+                                     fp16["tuned"][0]),
         "single_1080p_fp16_us_per_frame": single,
         "uneven_batch_32_fp16_us_per_batch": uneven,
         "serving": serving,
