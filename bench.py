@@ -13,11 +13,11 @@ kernel durations are not stretched by the overlap.
 Reported: µs per frame of the FP16-storage batched path (value), the plain
 FP32 single-frame path beside it, ΔPSNR between the two, the byte budget
 check, per-batch p50/p99 latency, the transform rooflines from HIP events
-recorded on the launch stream, one batch of 32 uneven frames (720p to 4K,
-4:2:0 and 4:4:4) eager and as a replayed hipGraph, the serving loop (64 mixed
-sessions: latency p50/p99 at cadence and saturated frames/s), and the oracle
-(single-threaded CPU restatement) timed on a bounded sample as the CPU
-baseline.
+recorded on the launch stream, 1080p frames batched in FP16 and one at a time
+in FP32, one batch of 32 uneven frames (720p to 4K, 4:2:0 and 4:4:4) eager
+and as a replayed hipGraph, the serving loop (64 mixed sessions: latency
+p50/p99 at cadence and saturated frames/s), and the oracle (single-threaded
+CPU restatement) timed on a bounded sample as the CPU baseline.
 """
 import argparse
 import json
@@ -106,7 +106,7 @@ def dwt_algorithmic_bytes(fp16):
     return total
 
 
-def short_kernel_name(full, fallback=48):
+def short_kernel_name(full):
     """`k_name<template args>` out of a demangled kernel name: the template
     arguments name the transform tile shape, so two shapes whose launches
     happen to have the same workgroup count stay apart."""
@@ -495,6 +495,48 @@ def main():
     single = {"eager": single_1080p(False), "graph": single_1080p(True)}
     print(f"[bench] single-frame graphs done", file=sys.stderr, flush=True)
 
+    # 1920x1080 4:2:0 (crops of the resident sequence) at 400,000 B per frame:
+    # FP16 batched like the headline (batches of B frames on `streams`
+    # streams) and the plain FP32 single-frame path, µs per frame.
+    def frames_1080p(prec, n, nstreams, steps=20, warmup=3):
+        d2 = pw.Device()
+        d2.set_timing(False)
+        groups = []
+        for g in range(nstreams):
+            encs = [pw.Encoder(d2, 1920, 1080, pw.CHROMA_420, precision=prec) for _ in range(n)]
+            decs = [pw.Decoder(d2, 1920, 1080, pw.CHROMA_420, precision=prec) for _ in range(n)]
+            outs = [[torch.empty((1080, 1920), dtype=torch.uint8, device="cuda"),
+                     torch.empty((540, 960), dtype=torch.uint8, device="cuda"),
+                     torch.empty((540, 960), dtype=torch.uint8, device="cuda")] for _ in range(n)]
+            tgs = [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in out]) for out in outs]
+            groups.append((encs, decs, outs, tgs, stream if g == 0 else torch.cuda.Stream()))
+        frs = [pw.hip_frame([(f[0].data_ptr(), f[0].stride(0), 1), (f[1].data_ptr(), f[1].stride(0), 1),
+                             (f[2].data_ptr(), f[2].stride(0), 1)]) for f in dev_frames]
+
+        def run(k0, steps_):
+            for s in range(steps_):
+                for g, (encs, decs, _, tgs, st) in enumerate(groups):
+                    fr = [frs[(k0 + (s * nstreams + g) * n + i) % FRAMES] for i in range(n)]
+                    pw.encode_batch(encs, fr, [400000] * n, stream=st.cuda_stream)
+                    pw.decode_encoded_batch(decs, encs, tgs, stream=st.cuda_stream)
+
+        run(0, warmup)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        run(warmup * n * nstreams, steps)
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t) / (steps * n * nstreams) * 1e6
+        for encs, decs, _, _, _ in groups:
+            for x in encs + decs:
+                x.close()
+        d2.close()
+        return round(us, 2)
+
+    f1080 = {"fp16_batched": frames_1080p(pw.PRECISION_FP16, B, args.streams),
+             "fp32_single": frames_1080p(pw.PRECISION_FP32, 1, 1, steps=60, warmup=5)}
+    f1080["speedup"] = round(f1080["fp32_single"] / f1080["fp16_batched"], 3)
+    print(f"[bench] 1080p done", file=sys.stderr, flush=True)
+
     # One batch of 32 uneven frames (8 each of 1280x720 4:2:0, 1920x1080 4:2:0,
     # 2560x1440 4:4:4, 3840x2160 4:2:0; crops of the resident sequence, the
     # 4:4:4 chroma planes luma crops of neighbouring frames), 1.6 bits/pixel,
@@ -601,6 +643,7 @@ def main():
         "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1]), "inverse", fp16["tuned"][1]),
         "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0]), "forward",
                                      fp16["tuned"][0]),
+        "frames_1080p_us_per_frame": f1080,
         "single_1080p_fp16_us_per_frame": single,
         "uneven_batch_32_fp16_us_per_batch": uneven,
         "serving": serving,
