@@ -654,7 +654,8 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 
 // --- forward, streamed (shape 4) ------------------------------------------
 //
-// One wave per tile of kStreamTileW x kStreamTileH input samples and no LDS.
+// One wave per tile of kFwdVariantTileW x kFwdVariantTileH[kFwdStreamVariant]
+// (120 x 32) input samples and no LDS.
 // Lane l holds the even / odd input pair at columns x0 - 4 + 2l and
 // x0 - 3 + 2l; lanes 2 .. 61 produce the tile's outputs, lanes 0-1 and 62-63
 // are the apron.  The wave walks down the kStreamTileH + 8 rows of the tile's
@@ -717,9 +718,29 @@ struct VLine
 // Raw loads through global (not generic) pointers: a generic load also
 // counts on the LDS counter, and any wait for the oldest prefetched row would
 // then wait for every later one too.
-typedef const __attribute__((address_space(1))) uint8_t *GU8;
-typedef const __attribute__((address_space(1))) uint16_t *GU16;
 typedef const __attribute__((address_space(1))) uint32_t *GU32;
+
+// A streamed source: its address split into the aligned word address and the
+// byte offset within that word (u8 planes come from the caller and may start
+// at any byte, an NV12 Cr plane at data + 1; fp16 / fp32 sources are band
+// storage, 256-byte aligned).
+struct StreamSrc
+{
+	GU32 words;
+	uint32_t lo;
+};
+
+__device__ __forceinline__ StreamSrc stream_src(const void *p)
+{
+	const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+	return StreamSrc{reinterpret_cast<GU32>(a & ~uintptr_t(3)), uint32_t(a & 3u)};
+}
+
+template <int KIND>
+constexpr uint32_t stream_esize()
+{
+	return KIND == kStoreU8 ? 1u : KIND == kStoreF16 ? 2u : 4u;
+}
 
 // Every format is fetched as the aligned 32-bit word holding the sample (a
 // byte or half is extracted when it is converted): a narrower load's
@@ -727,25 +748,21 @@ typedef const __attribute__((address_space(1))) uint32_t *GU32;
 // the data.  An aligned word never crosses a page, so it stays inside the
 // allocation that holds the sample.
 template <int KIND>
-__device__ __forceinline__ uint32_t stream_raw(const void *base, uint32_t off)
+__device__ __forceinline__ uint32_t stream_raw(const StreamSrc &S, uint32_t off)
 {
-	if constexpr (KIND == kStoreU8)
-		return ((GU32)base)[off >> 2];
-	else if constexpr (KIND == kStoreF16)
-		return ((GU32)base)[off >> 1];
-	else
-		return ((GU32)base)[off];
+	return S.words[(S.lo + off * stream_esize<KIND>()) >> 2];
 }
 
 // The sample at element offset `off` out of its word, as a float (u8 with
 // the DC shift).
 template <int KIND>
-__device__ __forceinline__ float stream_value(uint32_t raw, uint32_t off)
+__device__ __forceinline__ float stream_value(uint32_t raw, const StreamSrc &S, uint32_t off)
 {
+	const uint32_t a = S.lo + off * stream_esize<KIND>();
 	if constexpr (KIND == kStoreU8)
-		return u8_dc((raw >> (8 * (off & 3))) & 0xffu);
+		return u8_dc((raw >> (8 * (a & 3))) & 0xffu);
 	else if constexpr (KIND == kStoreF16)
-		return __half2float(__ushort_as_half((unsigned short)(raw >> (16 * (off & 1)))));
+		return __half2float(__ushort_as_half((unsigned short)(raw >> (8 * (a & 2)))));
 	else
 		return __uint_as_float(raw);
 }
@@ -761,9 +778,10 @@ __device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx,
 // One tile of the streamed forward transform with source samples of format
 // KIND (fixed per wave, so the loads need no branches).
 template <int KIND>
-__device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void *src, uint32_t pitch, uint32_t step,
-                                                    int x0, int y0, int lane)
+__device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void *src_ptr, uint32_t pitch,
+                                                    uint32_t step, int x0, int y0, int lane)
 {
+	const StreamSrc src = stream_src(src_ptr);
 	constexpr int TH = kFwdVariantTileH[kFwdStreamVariant];
 	constexpr int kPairs = TH / 2 + kApron;
 	static_assert(kPairs % 2 == 0, "the row-pair loop is unrolled by two");
@@ -792,10 +810,10 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 		r[3] = stream_raw<KIND>(src, r1 + xo);
 	};
 	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
-		v[0] = stream_value<KIND>(r[0], r0 + xe);
-		v[1] = stream_value<KIND>(r[1], r0 + xo);
-		v[2] = stream_value<KIND>(r[2], r1 + xe);
-		v[3] = stream_value<KIND>(r[3], r1 + xo);
+		v[0] = stream_value<KIND>(r[0], src, r0 + xe);
+		v[1] = stream_value<KIND>(r[1], src, r0 + xo);
+		v[2] = stream_value<KIND>(r[2], src, r1 + xe);
+		v[3] = stream_value<KIND>(r[3], src, r1 + xo);
 	};
 	VLine lo, hi; // columns of horizontal low / high outputs
 	auto lift_pair = [&](int n, float e0, float o0, float e1, float o1) {
@@ -953,18 +971,20 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
 	};
 	// Raw words: LL and HL of the even row, LH and HH of the odd row.
+	const StreamSrc ll = stream_src(J.band[0]), hl = stream_src(J.band[1]), lh = stream_src(J.band[2]),
+	                hh = stream_src(J.band[3]);
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
-		r[0] = stream_raw<kStoreF32>(J.band[0], r0 + ce);
-		r[1] = stream_raw<HKIND>(J.band[1], r0 + co);
-		r[2] = stream_raw<HKIND>(J.band[2], r1 + ce);
-		r[3] = stream_raw<HKIND>(J.band[3], r1 + co);
+		r[0] = stream_raw<kStoreF32>(ll, r0 + ce);
+		r[1] = stream_raw<HKIND>(hl, r0 + co);
+		r[2] = stream_raw<HKIND>(lh, r1 + ce);
+		r[3] = stream_raw<HKIND>(hh, r1 + co);
 	};
 	// Scaled for the vertical pass: even rows by K, odd rows by 1/K.
 	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
-		v[0] = stream_value<kStoreF32>(r[0], r0 + ce) * kK;
-		v[1] = stream_value<HKIND>(r[1], r0 + co) * kK;
-		v[2] = stream_value<HKIND>(r[2], r1 + ce) * kInvK;
-		v[3] = stream_value<HKIND>(r[3], r1 + co) * kInvK;
+		v[0] = stream_value<kStoreF32>(r[0], ll, r0 + ce) * kK;
+		v[1] = stream_value<HKIND>(r[1], hl, r0 + co) * kK;
+		v[2] = stream_value<HKIND>(r[2], lh, r1 + ce) * kInvK;
+		v[3] = stream_value<HKIND>(r[3], hh, r1 + co) * kInvK;
 	};
 	auto store_row = [&](int py, float e, float o) {
 		if (!lane_out)

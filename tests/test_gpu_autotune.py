@@ -121,3 +121,61 @@ def test_streamed_shape_on_odd_sizes(precision):
         assert np.array_equal(a, b), ODD[i]
         for p in range(3):
             assert np.array_equal(pa[p], pb[p]), (ODD[i], p)
+
+
+@pytest.mark.parametrize("interleaved", [False, True])
+def test_streamed_shape_on_unaligned_planes(interleaved):
+    """The streamed forward shape reads u8 planes through the aligned words
+    that hold them: planes starting at odd byte addresses with odd pitches,
+    and NV12-style interleaved chroma (Cr at Cb + 1, step 2), placed at the
+    end of their buffer, encode to the same coefficients and bitstream as
+    contiguous planes under shape 0."""
+    import torch
+    w, h = 1000, 562
+    planes = synth.frame(w, h, True, seed=77)
+    budget = int(w * h * 2.0 / 8)
+    contiguous = [torch.from_numpy(p).cuda() for p in planes]
+    cw, ch = w // 2, h // 2
+    if interleaved:
+        y_pitch, uv_pitch = w + 3, 2 * cw + 5
+        y_off = 1
+        uv_off = y_off + y_pitch * h + 2
+        size = uv_off + uv_pitch * (ch - 1) + 2 * cw
+        buf = torch.zeros(size, dtype=torch.uint8, device="cuda")
+        buf[y_off:y_off + y_pitch * h].view(h, y_pitch)[:, :w].copy_(contiguous[0])
+        uv = buf[uv_off:].as_strided((ch, cw, 2), (uv_pitch, 2, 1))
+        uv[:, :, 0].copy_(contiguous[1])
+        uv[:, :, 1].copy_(contiguous[2])
+        base = buf.data_ptr()
+        src = pw.hip_frame([(base + y_off, y_pitch, 1), (base + uv_off, uv_pitch, 2),
+                            (base + uv_off + 1, uv_pitch, 2)])
+    else:
+        pitches = [w + 1, cw + 3, cw + 1]
+        offs = [3]
+        offs.append(offs[0] + pitches[0] * h + 1)
+        offs.append(offs[1] + pitches[1] * ch + 2)
+        size = offs[2] + pitches[2] * (ch - 1) + cw
+        buf = torch.zeros(size, dtype=torch.uint8, device="cuda")
+        for p, (o, pt) in enumerate(zip(offs, pitches)):
+            ph, pw_ = contiguous[p].shape
+            buf[o:o + pt * (ph - 1) + pw_].as_strided((ph, pw_), (pt, 1)).copy_(contiguous[p])
+        base = buf.data_ptr()
+        src = pw.hip_frame([(base + o, pt, 1) for o, pt in zip(offs, pitches)])
+    torch.cuda.synchronize()
+    results = {}
+    dev = pw.Device()
+    try:
+        for shape, frame in ((0, pw.hip_frame([(t.data_ptr(), t.stride(0), 1) for t in contiguous])), (4, src)):
+            dev.force_tile_variant(shape)
+            enc = pw.Encoder(dev, w, h, pw.CHROMA_420, precision=pw.PRECISION_FP16)
+            try:
+                pw.encode_batch([enc], [frame], [budget])
+                torch.cuda.synchronize()
+                data, pk = enc.packetize(1 << 30, budget + 64)
+                results[shape] = (data[:sum(z for _, z in pk)].copy(), enc.read_coefficients().view(np.uint32))
+            finally:
+                enc.close()
+    finally:
+        dev.close()
+    assert np.array_equal(results[0][1], results[4][1]), int(np.count_nonzero(results[0][1] != results[4][1]))
+    assert np.array_equal(results[0][0], results[4][0])
