@@ -675,8 +675,9 @@ int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int p
 				or_dwt_level_f32(in32, in_pitch, R_x, R_y, A_x, A_y,
 				                 coeffs + B[0].coeff_off, coeffs + B[1].coeff_off,
 				                 coeffs + B[2].coeff_off, coeffs + B[3].coeff_off, B[0].bw);
-				if (precision == 1 && l < 2)
-					for (int b = 1; b < 4; b++)
+				/* precision 3 rounds the LL too: it is the next level's input. */
+				if ((precision == 1 || precision == 3) && l < 2)
+					for (int b = precision == 3 ? 0 : 1; b < 4; b++)
 						or_round_to_f16_array(coeffs + B[b].coeff_off, (long)B[b].bw * B[b].bh);
 			}
 			if (l == first_level)
@@ -1746,9 +1747,10 @@ int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
 			const or_band *B = L->band[c][l];
 			int bw = B[0].bw, bh = B[0].bh;
 			/* precision 1: the decoder holds the high bands of levels 0-1 in
-			 * fp16 storage; LL intermediates stay FP32. */
-			if (precision == 1 && l < 2)
-				for (int b = 1; b < 4; b++)
+			 * fp16 storage; LL intermediates stay FP32.  precision 3: the LL
+			 * of levels 0-1 (written by the level above) in fp16 as well. */
+			if ((precision == 1 || precision == 3) && l < 2)
+				for (int b = precision == 3 ? 0 : 1; b < 4; b++)
 					or_round_to_f16_array(coeffs + B[b].coeff_off, (long)B[b].bw * B[b].bh);
 			if (l == first_level)
 			{

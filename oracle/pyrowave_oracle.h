@@ -113,6 +113,10 @@ void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const 
 /* Whole-frame forward transform into the flat coefficient array.
  * precision 2: FP32 everywhere.  precision 1: FP32 math; the high bands
  * (HL, LH, HH) of levels 0-1 rounded to fp16 storage, LL kept FP32.
+ * precision 3: FP32 math; all four bands of levels 0-1 rounded to fp16, so
+ * levels 1 and 2 transform the fp16-rounded LL (the reference's precision 1
+ * storage, pyrowave_common.cpp:133-189: one R16F image holds every band of
+ * levels 0-1, and component_ll_views feed the next level from it).
  * use_f64 != 0: FP64 math; result written to coeffs64 (may be NULL otherwise). */
 int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                  int precision, float *coeffs, double *coeffs64);
@@ -175,7 +179,8 @@ int or_dequant(const or_layout *L, const uint32_t *payload, long payload_words,
 
 /* idwt.comp chain: levels 4 -> 0, u8 output planes with DC shift and clamp.
  * precision 1 rounds the input high bands of levels 0-1 to fp16 first (the
- * decoder's fp16 band storage). */
+ * decoder's fp16 band storage); precision 3 also rounds the LL of levels 0-1,
+ * which the inverse of the level above wrote (R16F in the reference). */
 int or_idwt_frame(const or_layout *L, const float *coeffs, int precision,
                   uint8_t *const planes[3], const int pitches[3]);
 int or_idwt_frame_f64(const or_layout *L, const double *coeffs, double *const planes[3], const int pitches[3]);

@@ -36,6 +36,7 @@ FORMAT_YUV444P = 2
 
 PRECISION_FP16 = 1
 PRECISION_FP32 = 2
+PRECISION_FP16_LL = 3
 
 EXACT_PREFIX = 1
 

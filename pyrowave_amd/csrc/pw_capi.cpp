@@ -95,12 +95,14 @@ std::mutex g_device_creation_lock;
 int default_precision()
 {
 	// Reference pyrowave_common.cpp:15-28: PYROWAVE_PRECISION overrides the
-	// precision.  0 and 1 select the FP16 path here, 2 (default) FP32.
+	// precision.  1 selects the reference's precision 1 storage (all bands
+	// of levels 0-1 as FP16), 0 (FP16 everywhere in the reference) the
+	// nearest one here, the same; 2 (default) FP32.
 	if (const char *env = getenv("PYROWAVE_PRECISION"))
 	{
 		long v = strtol(env, nullptr, 0);
 		if (v == 0 || v == 1)
-			return PYROWAVE_HIP_PRECISION_FP16;
+			return PYROWAVE_HIP_PRECISION_FP16_LL;
 	}
 	return PYROWAVE_HIP_PRECISION_FP32;
 }
@@ -632,7 +634,7 @@ static pyrowave_result create_encoder(pyrowave_device device, int width, int hei
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	if (chroma == PYROWAVE_CHROMA_SUBSAMPLING_420 && (width % 2 || height % 2))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	if (precision != PYROWAVE_HIP_PRECISION_FP16 && precision != PYROWAVE_HIP_PRECISION_FP32)
+	if (!pw::valid_precision(precision))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	(void)hipSetDevice(device->dev.hip_device);
 	std::unique_ptr<pyrowave_encoder_opaque> e(new pyrowave_encoder_opaque);
@@ -853,7 +855,7 @@ static pyrowave_result create_decoder(pyrowave_device device, int width, int hei
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	if (chroma == PYROWAVE_CHROMA_SUBSAMPLING_420 && (width % 2 || height % 2))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	if (precision != PYROWAVE_HIP_PRECISION_FP16 && precision != PYROWAVE_HIP_PRECISION_FP32)
+	if (!pw::valid_precision(precision))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	(void)hipSetDevice(device->dev.hip_device);
 	std::unique_ptr<pyrowave_decoder_opaque> d(new pyrowave_decoder_opaque);

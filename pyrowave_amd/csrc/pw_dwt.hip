@@ -435,8 +435,9 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 // Loads the four bands' (TW/2 + 4) x (TH/2 + 4) shares of the interleaved
 // (TW + 8) x (TH + 8) tile
 // (band b holds the positions whose (row, column) parity is (b >> 1, b & 1))
-// into LDS; every global load is issued before the first LDS store.
-template <typename S, typename TH>
+// into LDS; every global load is issued before the first LDS store.  TL / TH
+// are the storage types of the LL and of the high bands.
+template <typename S, typename TL, typename TH>
 __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, const int *xmap, int tid, float *sA)
 {
 	constexpr int kQX = S::kInW / 2, kQY = S::kInH / 2;
@@ -444,7 +445,7 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 	// 32-bit element offsets from the uniform band bases (bands stay below
 	// 4 GiB), so loads use a scalar base and a 32-bit vector offset.
 	const uint32_t pitch = (uint32_t)J.band_pitch;
-	const float *ll = static_cast<const float *>(J.band[0]);
+	const TL *ll = static_cast<const TL *>(J.band[0]);
 	const TH *hb[3] = {static_cast<const TH *>(J.band[1]), static_cast<const TH *>(J.band[2]),
 	                   static_cast<const TH *>(J.band[3])};
 	float vals[4][kIters];
@@ -461,7 +462,10 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 			const uint32_t row0 = __umul24((uint32_t)(ymap[2 * rr] >> 1), pitch);
 			const uint32_t row1 = __umul24((uint32_t)(ymap[2 * rr + 1] >> 1), pitch);
 			const uint32_t col0 = (uint32_t)(xmap[2 * cc] >> 1), col1 = (uint32_t)(xmap[2 * cc + 1] >> 1);
-			vals[0][k] = ll[row0 + col0];
+			if constexpr (sizeof(TL) == 2)
+				vals[0][k] = __half2float(ll[row0 + col0]);
+			else
+				vals[0][k] = ll[row0 + col0];
 			if constexpr (sizeof(TH) == 2)
 			{
 				vals[1][k] = __half2float(hb[0][row0 + col1]);
@@ -551,10 +555,12 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 
 	// Interleaved position parity selects the band: LL (even, even), HL (x
 	// odd), LH (y odd), HH.  The extension preserves parity.
-	if (J.band_kind == kStoreF16)
-		load_bands<S, __half>(J, s_ymap, s_xmap, tid, sA);
+	if (J.ll_kind == kStoreF16)
+		load_bands<S, __half, __half>(J, s_ymap, s_xmap, tid, sA);
+	else if (J.band_kind == kStoreF16)
+		load_bands<S, float, __half>(J, s_ymap, s_xmap, tid, sA);
 	else
-		load_bands<S, float>(J, s_ymap, s_xmap, tid, sA);
+		load_bands<S, float, float>(J, s_ymap, s_xmap, tid, sA);
 	__syncthreads();
 
 	// Vertical pass over all TW + 8 columns; window seg covers staged rows
@@ -938,8 +944,8 @@ __device__ __forceinline__ void hlift_inv_row(float &e, float &o)
 	o = fmaf(-kAlpha, e + dpp_wave<kDppWaveShl1>(e), o);
 }
 
-// One tile with high bands of format HKIND (the LL band is FP32).
-template <int HKIND>
+// One tile with the LL band stored as LKIND and the high bands as HKIND.
+template <int LKIND, int HKIND>
 __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const FrameState *frames, int ox0, int oy0,
                                                     int lane)
 {
@@ -974,14 +980,14 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 	const StreamSrc ll = stream_src(J.band[0]), hl = stream_src(J.band[1]), lh = stream_src(J.band[2]),
 	                hh = stream_src(J.band[3]);
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
-		r[0] = stream_raw<kStoreF32>(ll, r0 + ce);
+		r[0] = stream_raw<LKIND>(ll, r0 + ce);
 		r[1] = stream_raw<HKIND>(hl, r0 + co);
 		r[2] = stream_raw<HKIND>(lh, r1 + ce);
 		r[3] = stream_raw<HKIND>(hh, r1 + co);
 	};
 	// Scaled for the vertical pass: even rows by K, odd rows by 1/K.
 	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
-		v[0] = stream_value<kStoreF32>(r[0], ll, r0 + ce) * kK;
+		v[0] = stream_value<LKIND>(r[0], ll, r0 + ce) * kK;
 		v[1] = stream_value<HKIND>(r[1], hl, r0 + co) * kK;
 		v[2] = stream_value<HKIND>(r[2], lh, r1 + ce) * kInvK;
 		v[3] = stream_value<HKIND>(r[3], hh, r1 + co) * kInvK;
@@ -1062,10 +1068,12 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const 
 	const int local = tile - J.tile_base;
 	const int ox0 = (local % J.tiles_x) * TW;
 	const int oy0 = (local / J.tiles_x) * TH;
-	if (J.band_kind == kStoreF16)
-		inverse_stream_tile<kStoreF16>(J, frames, ox0, oy0, lane);
+	if (J.ll_kind == kStoreF16)
+		inverse_stream_tile<kStoreF16, kStoreF16>(J, frames, ox0, oy0, lane);
+	else if (J.band_kind == kStoreF16)
+		inverse_stream_tile<kStoreF32, kStoreF16>(J, frames, ox0, oy0, lane);
 	else
-		inverse_stream_tile<kStoreF32>(J, frames, ox0, oy0, lane);
+		inverse_stream_tile<kStoreF32, kStoreF32>(J, frames, ox0, oy0, lane);
 }
 } // namespace
 

@@ -126,10 +126,13 @@ bool Geometry::init(int w, int h, int chroma_, int precision_)
 				B.by32 = (B.bh + 31) / 32;
 				B.flat_off = flat;
 				flat += long(B.bw) * B.bh;
-				// FP16 path: the transmitted high bands of levels 0-1 are stored
-				// as fp16; LL intermediates stay FP32 (fp16 LL storage costs
-				// ~0.07 dB PSNR per side at high rates, the high bands ~0).
-				B.kind = (precision == PYROWAVE_HIP_PRECISION_FP16 && l < 2 && b != 0) ? kStoreF16 : kStoreF32;
+				// FP16: the transmitted high bands of levels 0-1 are stored as
+				// fp16 and the LL intermediates stay FP32 (fp16 LL storage costs
+				// 0.11 dB PSNR at high rates, the high bands ~0); FP16_LL: all
+				// four bands of levels 0-1 as fp16, as the reference stores them.
+				const bool f16 = l < 2 && ((precision == PYROWAVE_HIP_PRECISION_FP16 && b != 0) ||
+				                           precision == PYROWAVE_HIP_PRECISION_FP16_LL);
+				B.kind = f16 ? kStoreF16 : kStoreF32;
 				B.pitch = B.bx32 * 32;
 				size_t bytes = size_t(B.pitch) * size_t(B.by32 * 32) * (B.kind == kStoreF16 ? 2 : 4);
 				B.store_off = store;

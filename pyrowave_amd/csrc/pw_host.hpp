@@ -14,6 +14,12 @@
 
 namespace pw
 {
+inline bool valid_precision(int precision)
+{
+	return precision == PYROWAVE_HIP_PRECISION_FP16 || precision == PYROWAVE_HIP_PRECISION_FP32 ||
+	       precision == PYROWAVE_HIP_PRECISION_FP16_LL;
+}
+
 enum Stage
 {
 	kStageDWT = 0,

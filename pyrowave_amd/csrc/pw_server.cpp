@@ -393,7 +393,7 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_server_create(const pyrowave_hi
 {
 	if (!info || !server || !info->device)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	if (info->precision != PYROWAVE_HIP_PRECISION_FP16 && info->precision != PYROWAVE_HIP_PRECISION_FP32)
+	if (!valid_precision(info->precision))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	std::unique_ptr<pyrowave_hip_server_opaque> S(new pyrowave_hip_server_opaque);
 	S->device = info->device;

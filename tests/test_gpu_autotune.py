@@ -6,7 +6,8 @@ Every tile shape of each transform direction (forward 128x56, 64x56, and
 directions the streamed 120x32 shape that lifts in registers) gives the same
 coefficients bit for bit (compared directly), so a batch of uneven frames (1280x720 4:2:0, 640x480
 4:4:4, 1920x1080 4:2:0, FP16 and FP32) encodes and decodes to the same
-bitstreams and planes with any shape forced on every level; pyrowave_hip_autotune_batch
+bitstreams and planes with any shape forced on every level (also with the LL
+of levels 0-1 stored as fp16); pyrowave_hip_autotune_batch
 returns those same results while it times the shapes, reports a valid shape
 per level, and later calls (which use the tuned shapes) still agree.
 """
@@ -47,7 +48,7 @@ def _collect(encs, outs, budgets):
 MODES = {"shape0": 0, "shape1": 1, "shape2": 2, "shape3": 3, "shape4": 4}
 
 
-@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP32])
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP32, pw.PRECISION_FP16_LL])
 def test_tile_shapes_agree_and_autotune(precision):
     import torch
     inputs, frames, budgets = _batch(torch)
@@ -86,7 +87,7 @@ def test_tile_shapes_agree_and_autotune(precision):
 ODD = [(200, 130, pw.CHROMA_444), (258, 146, pw.CHROMA_420), (130, 66, pw.CHROMA_420), (1000, 562, pw.CHROMA_420)]
 
 
-@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP32])
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP32, pw.PRECISION_FP16_LL])
 def test_streamed_shape_on_odd_sizes(precision):
     """The streamed shape (4) on sizes that are not multiples of the tiles, of
     32 or of the 128 minimum (edge extension, partial tiles, tiny levels):

@@ -49,19 +49,28 @@ def test_dwt_fp32_bit_exact(device, w, h, chroma):
     assert mism.size == 0, f"{mism.size} coefficients differ, first at {mism[:5]}"
 
 
+# Oracle precision of each storage mode.
+ORACLE_PRECISION = {pw.PRECISION_FP32: 2, pw.PRECISION_FP16: 1, pw.PRECISION_FP16_LL: 3}
+
+
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
 @pytest.mark.parametrize("w,h,chroma", [(34, 30, pw.CHROMA_420), (200, 130, pw.CHROMA_444),
                                         (1920, 1080, pw.CHROMA_420)])
-def test_dwt_fp16_storage_bit_exact(device, w, h, chroma):
+def test_dwt_fp16_storage_bit_exact(device, w, h, chroma, precision):
+    """Every band of every level, the LL intermediates included, equals the
+    oracle in the same storage mode (FP16: high bands of levels 0-1 in fp16;
+    FP16_LL: all four bands of levels 0-1, so levels 1-2 read an fp16 LL)."""
     planes = _content(w, h, chroma)
     L = O.layout(w, h, chroma)
-    enc = pw.Encoder(device, w, h, chroma, precision=pw.PRECISION_FP16)
+    enc = pw.Encoder(device, w, h, chroma, precision=precision)
     try:
         assert enc.encode_cpu(planes, 1 << 20) == pw.SUCCESS
         gpu = enc.read_coefficients()
     finally:
         enc.close()
-    ref = O.dwt_frame(L, planes, precision=1)
-    assert np.array_equal(gpu.view(np.uint32), ref.view(np.uint32))
+    ref = O.dwt_frame(L, planes, precision=ORACLE_PRECISION[precision])
+    mism = np.flatnonzero(gpu.view(np.uint32) != ref.view(np.uint32))
+    assert mism.size == 0, f"{mism.size} coefficients differ, first at {mism[:5]}"
 
 
 @pytest.mark.parametrize("precision,max_abs,rms", [(pw.PRECISION_FP32, 2e-5, 2e-6),
@@ -151,7 +160,7 @@ def test_fp16_path_integer_stages_bit_exact(device):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("precision", [pw.PRECISION_FP32, pw.PRECISION_FP16])
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP32, pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
 @pytest.mark.parametrize("w,h,chroma,budget", [
     (34, 30, pw.CHROMA_420, 64 * 1024),
     (34, 30, pw.CHROMA_444, 64 * 1024),
@@ -160,7 +169,8 @@ def test_fp16_path_integer_stages_bit_exact(device):
 def test_decode_bit_exact(device, w, h, chroma, budget, precision):
     """Dequantized coefficients and decoded planes equal the oracle's.  The
     FP16 decoder keeps the high bands of levels 0-1 in fp16, which the oracle's
-    precision-1 decode reproduces."""
+    precision-1 decode reproduces; the FP16_LL decoder also the LL that the
+    inverse of levels 1-2 writes (oracle precision 3)."""
     planes = _content(w, h, chroma, seed=13)
     L = O.layout(w, h, chroma)
     ref_bytes, pk, _ = oracle_encode(L, planes, budget)
@@ -174,7 +184,7 @@ def test_decode_bit_exact(device, w, h, chroma, budget, precision):
     for off, size in pk:
         assert d.push(ref_bytes[off:off + size].tobytes())
     rcoef = O.dequant(L, d.payload(), d.offsets())
-    oprec = 2 if precision == pw.PRECISION_FP32 else 1
+    oprec = ORACLE_PRECISION[precision]
     for c in range(3):
         for l in range(5):
             for b in range(4):
@@ -182,7 +192,7 @@ def test_decode_bit_exact(device, w, h, chroma, budget, precision):
                 if not B.present:
                     continue
                 r = O.band_view(L, rcoef, c, l, b)
-                if oprec == 1 and l < 2 and b != 0:
+                if oprec != 2 and l < 2 and b != 0:
                     r = r.astype(np.float16).astype(np.float32)
                 assert np.array_equal(O.band_view(L, gcoef, c, l, b).view(np.uint32), r.view(np.uint32)), (c, l, b)
     ref = O.idwt_frame(L, rcoef, precision=oprec)

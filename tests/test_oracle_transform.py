@@ -143,3 +143,43 @@ def test_fp32_transform_close_to_fp64():
     u8 = O.idwt_frame(L, c32)
     for p, r in zip(planes, u8):
         assert np.abs(p.astype(int) - r.astype(int)).max() <= 1
+
+
+def _band(L, coeffs, c, l, b):
+    B = L.b(c, l, b)
+    return coeffs[B.coeff_off:B.coeff_off + B.bw * B.bh].reshape(B.bh, B.bw)
+
+
+@pytest.mark.parametrize("w,h,chroma", [(160, 96, 0), (98, 70, 1)])
+def test_fp16_ll_storage_is_levelwise_fp16_rounding(w, h, chroma):
+    """precision 3 (every band of levels 0-1 stored as fp16, the LL included)
+    equals an FP32 level-by-level transform whose level 0-1 outputs are
+    rounded by numpy's float16 before the next level reads the LL; precision
+    1 rounds only the high bands, so its level-2 input is the FP32 LL."""
+    L = O.layout(w, h, chroma)
+    rng = np.random.default_rng(13)
+    planes = [rng.integers(0, 256, O.plane_shape(L, c), dtype=np.uint8) for c in range(3)]
+    c2 = O.dwt_frame(L, planes, precision=2)
+    c1 = O.dwt_frame(L, planes, precision=1)
+    c3 = O.dwt_frame(L, planes, precision=3)
+    f16 = lambda a: a.astype(np.float16).astype(np.float32)
+    for c in range(3):
+        first = 1 if (c and chroma == 0) else 0
+        # The first level reads the plane: identical in every mode before storage.
+        bands = [_band(L, c2, c, first, b) for b in range(4)]
+        for l in range(first, O.LEVELS):
+            if l > first:
+                ll = bands[0]
+                bands = O.dwt_level(ll, ll.shape[1], ll.shape[0], ll.shape[1], ll.shape[0], f64=False)
+            if l < 2:
+                bands = [f16(b) for b in bands]
+            for b in range(4):
+                got = _band(L, c3, c, l, b)
+                assert np.array_equal(got.view(np.uint32), bands[b].view(np.uint32)), (c, l, b)
+        # precision 1 keeps the LL of levels 0-1 in FP32.
+        ll1 = _band(L, c1, c, first, 0)
+        assert np.array_equal(ll1, _band(L, c2, c, first, 0))
+    assert not np.array_equal(c1, c3)
+    # The inverse rounds the LL it wrote for levels 0-1: the round trip stays within one code value.
+    for p, r in zip(planes, O.idwt_frame(L, c3, precision=3)):
+        assert np.abs(p.astype(int) - r.astype(int)).max() <= 1
