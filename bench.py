@@ -51,6 +51,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16-ll", action="store_true",
                     help="skip the FP16_LL leg (profiled runs: its transform launches have the headline's grids)")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the 1080p, uneven-batch and serving legs (profiled runs: some of their transform "
+                         "launches have the grids of the headline batch's levels)")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
                     help="length of each serving-loop run (64 sessions); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -214,6 +217,40 @@ def pmc_summary(fetch_csv, write_csv):
         fv = sum(fagg[key]) / len(fagg[key]) * ff * 1024
         wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
         print(f"{key[0]:40s} {key[1]:10d} {len(fagg[key]):6d} {fv / 1e6:10.3f} {wv / 1e6:10.3f} {(fv + wv) / 1e6:10.3f}")
+    # Per (direction, level, shape) of the headline batch: the row of the
+    # launch with that shape's workgroup count at that level.
+    print()
+    for direction, shapes in TRANSFORM_SHAPES.items():
+        for level in range(LEVELS):
+            for shape, (kernel, tw, th, per_wg) in enumerate(shapes):
+                wgs = -(-transform_tiles(level, tw, th, 32) // per_wg)
+                key = (kernel, wgs)
+                if key not in fagg:
+                    print(f"# missing {direction} level {level} shape {shape} ({kernel}, {wgs} workgroups)")
+                    continue
+                fv = sum(fagg[key]) / len(fagg[key]) * ff * 1024
+                wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
+                print(f"traffic_level {direction} {level} {shape} {int(round(fv + wv, -3))}")
+
+
+# Transform tile shapes by index: (kernel, tile width, tile height, tiles per workgroup).
+TRANSFORM_SHAPES = {
+    "forward": [("k_dwt_forward<128,56,true>", 128, 56, 1), ("k_dwt_forward<64,56,true>", 64, 56, 1),
+                ("k_dwt_forward<128,48,false>", 128, 48, 1), ("k_dwt_forward<128,40,false>", 128, 40, 1),
+                ("k_dwt_forward_stream", 120, 32, 4)],
+    "inverse": [("k_dwt_inverse<64,64,256,16,false>", 64, 64, 1), ("k_dwt_inverse<32,32,128,16,false>", 32, 32, 1),
+                ("k_dwt_inverse<64,56,256,16,false>", 64, 56, 1), ("k_dwt_inverse<64,48,256,24,true>", 64, 48, 1),
+                ("k_dwt_inverse_stream", 120, 32, 4)],
+}
+
+
+def transform_tiles(level, tw, th, batch):
+    """Tiles of one transform launch over a batch of 4K 4:2:0 frames: level 0
+    is the luma plane alone; launch l > 0 runs luma level l with chroma level
+    l - 1, three planes of the same (aligned) size."""
+    w, h = 3840 >> level, 2176 >> level
+    planes = 1 if level == 0 else 3
+    return planes * -(-w // tw) * -(-h // th) * batch
 
 
 SERVING_SHAPES = ((1280, 720, 0), (1920, 1080, 0), (2560, 1440, 1), (3840, 2160, 0))
@@ -503,8 +540,10 @@ def main():
         d2.close()
         return round(us, 2)
 
-    single = {"eager": single_1080p(False), "graph": single_1080p(True)}
-    print(f"[bench] single-frame graphs done", file=sys.stderr, flush=True)
+    single = f1080 = uneven = None
+    if not args.headline_only:
+        single = {"eager": single_1080p(False), "graph": single_1080p(True)}
+        print(f"[bench] single-frame graphs done", file=sys.stderr, flush=True)
 
     # 1920x1080 4:2:0 (crops of the resident sequence) at 400,000 B per frame:
     # FP16 batched like the headline (batches of B frames on `streams`
@@ -543,10 +582,11 @@ def main():
         d2.close()
         return round(us, 2)
 
-    f1080 = {"fp16_batched": frames_1080p(pw.PRECISION_FP16, B, args.streams),
-             "fp32_single": frames_1080p(pw.PRECISION_FP32, 1, 1, steps=60, warmup=5)}
-    f1080["speedup"] = round(f1080["fp32_single"] / f1080["fp16_batched"], 3)
-    print(f"[bench] 1080p done", file=sys.stderr, flush=True)
+    if not args.headline_only:
+        f1080 = {"fp16_batched": frames_1080p(pw.PRECISION_FP16, B, args.streams),
+                 "fp32_single": frames_1080p(pw.PRECISION_FP32, 1, 1, steps=60, warmup=5)}
+        f1080["speedup"] = round(f1080["fp32_single"] / f1080["fp16_batched"], 3)
+        print(f"[bench] 1080p done", file=sys.stderr, flush=True)
 
     # One batch of 32 uneven frames (8 each of 1280x720 4:2:0, 1920x1080 4:2:0,
     # 2560x1440 4:4:4, 3840x2160 4:2:0; crops of the resident sequence, the
@@ -592,12 +632,13 @@ def main():
         d2.close()
         return round(us, 1)
 
-    uneven = {"eager": uneven_batch(False), "graph": uneven_batch(True)}
-    uneven["frames_per_s_eager"] = round(32 / uneven["eager"] * 1e6, 1)
-    print(f"[bench] uneven batch done", file=sys.stderr, flush=True)
+    if not args.headline_only:
+        uneven = {"eager": uneven_batch(False), "graph": uneven_batch(True)}
+        uneven["frames_per_s_eager"] = round(32 / uneven["eager"] * 1e6, 1)
+        print(f"[bench] uneven batch done", file=sys.stderr, flush=True)
 
     serving = None
-    if args.serving_seconds > 0:
+    if args.serving_seconds > 0 and not args.headline_only:
         serving = serving_leg(pw, torch, device, dev_frames, args.serving_seconds)
         print(f"[bench] serving done", file=sys.stderr, flush=True)
 
