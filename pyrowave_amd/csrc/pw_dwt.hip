@@ -664,8 +664,8 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 // (120 x 32) input samples and no LDS.
 // Lane l holds the even / odd input pair at columns x0 - 4 + 2l and
 // x0 - 3 + 2l; lanes 2 .. 61 produce the tile's outputs, lanes 0-1 and 62-63
-// are the apron.  The wave walks down the kStreamTileH + 8 rows of the tile's
-// window two rows at a time: each row is lifted horizontally across the lanes
+// are the apron.  The wave walks down the 32 + 8 rows of the tile's window
+// two rows at a time: each row is lifted horizontally across the lanes
 // (the neighbouring lane's sample comes from a wave-wide DPP shift), and the
 // two columns each lane then holds (horizontal low and high) are lifted
 // vertically by a delay line in registers, which completes row pair p when
