@@ -536,8 +536,23 @@ __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ fr
 	// Words of every block at quant level 0 (the reference accumulates them
 	// with one atomic per block, reference shaders/analyze_rate_control.comp:127-131).
 	{
+		// Eight independent loads per thread in flight at a time: one
+		// workgroup reads a whole frame's column (one 2-byte word every 32
+		// bytes), so the sum is bound by load latency, not bandwidth.
+		constexpr int kUnroll = 8;
 		uint32_t acc = 0;
-		for (int b = t; b < F.n32; b += 512)
+		int b = t;
+		for (; b + (kUnroll - 1) * 512 < F.n32; b += kUnroll * 512)
+		{
+			uint32_t w[kUnroll];
+#pragma unroll
+			for (int i = 0; i < kUnroll; i++)
+				w[i] = F.words[(long)(b + i * 512) * 16];
+#pragma unroll
+			for (int i = 0; i < kUnroll; i++)
+				acc += w[i];
+		}
+		for (; b < F.n32; b += 512)
 			acc += F.words[(long)b * 16];
 		for (int d = 32; d >= 1; d >>= 1)
 			acc += __shfl_xor(acc, d, 64);
