@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 cpu_baseline leg of bench.py, always as the checker, never as the thing
 measured or shipped.  The product library never loads this module.
 """
+import contextlib
 import ctypes as C
 import os
 
@@ -42,76 +43,99 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
-        L = C.CDLL(LIB_PATH)
-        P = C.c_void_p
-        L.or_layout_init.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int]
-        L.or_layout_size.restype = C.c_long
-        L.or_block_descs.argtypes = [C.POINTER(Layout), P]
-        L.or_quant_resolution.restype = C.c_float
-        L.or_quant_resolution.argtypes = [C.c_int, C.c_int, C.c_int]
-        L.or_rdo_distortion_scale.restype = C.c_float
-        L.or_rdo_distortion_scale.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int]
-        L.or_encode_quant.restype = C.c_uint8
-        L.or_encode_quant.argtypes = [C.c_float]
-        L.or_decode_quant.restype = C.c_float
-        L.or_decode_quant.argtypes = [C.c_uint8]
-        L.or_f32_to_f16.restype = C.c_uint16
-        L.or_f32_to_f16.argtypes = [C.c_float]
-        L.or_f16_to_f32.restype = C.c_float
-        L.or_f16_to_f32.argtypes = [C.c_uint16]
-        L.or_round_to_f16_array.argtypes = [P, C.c_long]
-        L.or_round_f16_fast.restype = C.c_float
-        L.or_round_f16_fast.argtypes = [C.c_float]
-        L.or_check_round_f16_fast.restype = C.c_long
-        L.or_fwd_ext.argtypes = [C.c_int, C.c_int, C.c_int]
-        for name in ("or_dwt_level_f64", "or_dwt_level_f32"):
-            getattr(L, name).argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
-        for name in ("or_idwt_level_f64", "or_idwt_level_f32"):
-            getattr(L, name).argtypes = [P, P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
-        L.or_dwt_frame.argtypes = [C.POINTER(Layout), P, P, C.c_int, P, P]
-        L.or_quant_analyze.argtypes = [C.POINTER(Layout), P, P, P, P, P, P, P]
-        L.or_finalize.argtypes = [P, P, C.c_int]
-        L.or_resolve.argtypes = [C.POINTER(Layout), P, P, P, C.c_uint32, C.c_size_t, P]
-        L.or_pack.restype = C.c_long
-        L.or_pack.argtypes = [C.POINTER(Layout), P, P, C.c_int, P, C.c_long, P]
-        L.or_encode.restype = C.c_long
-        L.or_encode.argtypes = [C.POINTER(Layout), P, P, C.c_int, C.c_size_t, C.c_int, C.c_int,
-                                P, C.c_long, P, P]
-        L.or_compute_num_packets.restype = C.c_size_t
-        L.or_compute_num_packets.argtypes = [C.POINTER(Layout), P, C.c_size_t]
-        L.or_packetize.restype = C.c_size_t
-        L.or_packetize.argtypes = [C.POINTER(Layout), P, P, C.c_int, C.c_size_t, P, C.c_size_t, P]
-        L.or_validate_block.argtypes = [C.POINTER(Layout), P, C.c_uint32, C.c_uint32]
-        L.or_decoder_create.restype = P
-        L.or_decoder_create.argtypes = [C.POINTER(Layout)]
-        L.or_decoder_destroy.argtypes = [P]
-        L.or_decoder_clear.argtypes = [P]
-        L.or_decoder_push_packet.argtypes = [P, P, C.c_size_t]
-        L.or_decoder_is_ready.argtypes = [P, C.c_int]
-        L.or_decoder_mark_decoded.argtypes = [P]
-        L.or_decoder_payload_words.restype = C.c_long
-        L.or_decoder_payload_words.argtypes = [P]
-        L.or_decoder_payload.restype = P
-        L.or_decoder_payload.argtypes = [P]
-        L.or_decoder_offsets.restype = P
-        L.or_decoder_offsets.argtypes = [P]
-        L.or_decoder_decoded_blocks.argtypes = [P]
-        L.or_decoder_total_blocks.argtypes = [P]
-        L.or_dequant.argtypes = [C.POINTER(Layout), P, C.c_long, P, P]
-        L.or_idwt_frame.argtypes = [C.POINTER(Layout), P, C.c_int, P, P]
-        L.or_idwt_frame_f64.argtypes = [C.POINTER(Layout), P, P, P]
-        L.or_dct8x8.argtypes = [P, P]
-        L.or_maskeff.restype = C.c_float
-        L.or_maskeff.argtypes = [P, P, P]
-        L.or_psnr_hvs_m_tables.argtypes = [C.c_int, C.c_float, P, P, P]
-        L.or_psnr_height_factor.restype = C.c_float
-        L.or_psnr_height_factor.argtypes = [C.c_int]
-        L.or_psnr_hvs_m_totals.restype = C.c_uint64
-        L.or_psnr_hvs_m_totals.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_int, P]
-        L.or_psnr_hvs_m_db.restype = C.c_double
-        L.or_psnr_hvs_m_db.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
-        _lib = L
+        _lib = load_library(LIB_PATH)
     return _lib
+
+
+@contextlib.contextmanager
+def use_library(path):
+    """Runs the block with the oracle functions bound to another build of the
+    oracle sources (the mutation tests build planted mistakes this way)."""
+    global _lib
+    saved = _lib
+    _lib = load_library(path)
+    try:
+        yield _lib
+    finally:
+        _lib = saved
+
+
+def load_library(path):
+    L = C.CDLL(path)
+    P = C.c_void_p
+    L.or_layout_init.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int]
+    L.or_layout_size.restype = C.c_long
+    L.or_block_descs.argtypes = [C.POINTER(Layout), P]
+    L.or_quant_resolution.restype = C.c_float
+    L.or_quant_resolution.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.or_rdo_distortion_scale.restype = C.c_float
+    L.or_rdo_distortion_scale.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int]
+    L.or_encode_quant.restype = C.c_uint8
+    L.or_encode_quant.argtypes = [C.c_float]
+    L.or_decode_quant.restype = C.c_float
+    L.or_decode_quant.argtypes = [C.c_uint8]
+    L.or_f32_to_f16.restype = C.c_uint16
+    L.or_f32_to_f16.argtypes = [C.c_float]
+    L.or_f16_to_f32.restype = C.c_float
+    L.or_f16_to_f32.argtypes = [C.c_uint16]
+    L.or_round_to_f16_array.argtypes = [P, C.c_long]
+    L.or_round_f16_fast.restype = C.c_float
+    L.or_round_f16_fast.argtypes = [C.c_float]
+    L.or_check_round_f16_fast.restype = C.c_long
+    L.or_fwd_ext.argtypes = [C.c_int, C.c_int, C.c_int]
+    for name in ("or_dwt_level_f64", "or_dwt_level_f32"):
+        getattr(L, name).argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
+    for name in ("or_idwt_level_f64", "or_idwt_level_f32"):
+        getattr(L, name).argtypes = [P, P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
+    L.or_dwt_frame.argtypes = [C.POINTER(Layout), P, P, C.c_int, P, P]
+    L.or_quant_analyze.argtypes = [C.POINTER(Layout), P, P, P, P, P, P, P]
+    L.or_quant_analyze_ex.argtypes = [C.POINTER(Layout), P, P, P, P, P, P, P, P]
+    L.or_block8_stats.argtypes = [C.POINTER(Layout), P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                  P, P, P, P]
+    L.or_bucket_of_ratio.restype = C.c_uint32
+    L.or_bucket_of_ratio.argtypes = [C.c_float]
+    L.or_bucket_scan.argtypes = [P, P]
+    L.or_finalize.argtypes = [P, P, C.c_int]
+    L.or_resolve.argtypes = [C.POINTER(Layout), P, P, P, C.c_uint32, C.c_size_t, P]
+    L.or_pack.restype = C.c_long
+    L.or_pack.argtypes = [C.POINTER(Layout), P, P, C.c_int, P, C.c_long, P]
+    L.or_encode.restype = C.c_long
+    L.or_encode.argtypes = [C.POINTER(Layout), P, P, C.c_int, C.c_size_t, C.c_int, C.c_int,
+                            P, C.c_long, P, P]
+    L.or_compute_num_packets.restype = C.c_size_t
+    L.or_compute_num_packets.argtypes = [C.POINTER(Layout), P, C.c_size_t]
+    L.or_packetize.restype = C.c_size_t
+    L.or_packetize.argtypes = [C.POINTER(Layout), P, P, C.c_int, C.c_size_t, P, C.c_size_t, P]
+    L.or_validate_block.argtypes = [C.POINTER(Layout), P, C.c_uint32, C.c_uint32]
+    L.or_decoder_create.restype = P
+    L.or_decoder_create.argtypes = [C.POINTER(Layout)]
+    L.or_decoder_destroy.argtypes = [P]
+    L.or_decoder_clear.argtypes = [P]
+    L.or_decoder_push_packet.argtypes = [P, P, C.c_size_t]
+    L.or_decoder_is_ready.argtypes = [P, C.c_int]
+    L.or_decoder_mark_decoded.argtypes = [P]
+    L.or_decoder_payload_words.restype = C.c_long
+    L.or_decoder_payload_words.argtypes = [P]
+    L.or_decoder_payload.restype = P
+    L.or_decoder_payload.argtypes = [P]
+    L.or_decoder_offsets.restype = P
+    L.or_decoder_offsets.argtypes = [P]
+    L.or_decoder_decoded_blocks.argtypes = [P]
+    L.or_decoder_total_blocks.argtypes = [P]
+    L.or_dequant.argtypes = [C.POINTER(Layout), P, C.c_long, P, P]
+    L.or_idwt_frame.argtypes = [C.POINTER(Layout), P, C.c_int, P, P]
+    L.or_idwt_frame_f64.argtypes = [C.POINTER(Layout), P, P, P]
+    L.or_dct8x8.argtypes = [P, P]
+    L.or_maskeff.restype = C.c_float
+    L.or_maskeff.argtypes = [P, P, P]
+    L.or_psnr_hvs_m_tables.argtypes = [C.c_int, C.c_float, P, P, P]
+    L.or_psnr_height_factor.restype = C.c_float
+    L.or_psnr_height_factor.argtypes = [C.c_int]
+    L.or_psnr_hvs_m_totals.restype = C.c_uint64
+    L.or_psnr_hvs_m_totals.argtypes = [P, C.c_int, P, C.c_int, C.c_int, C.c_int, C.c_int, P]
+    L.or_psnr_hvs_m_db.restype = C.c_double
+    L.or_psnr_hvs_m_db.argtypes = [C.c_uint64, C.c_uint64, C.c_int]
+    return L
 
 
 def ptr(a):
@@ -165,7 +189,9 @@ def dwt_frame(L, planes, precision=2, f64=False):
     return c32
 
 
-def quant_analyze(L, coeffs):
+def quant_analyze(L, coeffs, with_dist=False):
+    """Quantizer + analyze; with_dist also returns dist32[n32][16], each 32x32
+    block's summed distortion per quant level."""
     coeffs = np.ascontiguousarray(coeffs, dtype=np.float32)
     words = np.zeros((L.count32, 16), dtype=np.uint16)
     bucket = np.zeros((L.count32, 16), dtype=np.uint8)
@@ -173,9 +199,37 @@ def quant_analyze(L, coeffs):
     np8 = np.zeros(L.count8, dtype=np.uint8)
     totals = np.zeros(FLAT, dtype=np.uint32)
     consumed = np.zeros(1, dtype=np.uint32)
-    lib().or_quant_analyze(C.byref(L), ptr(coeffs), ptr(words), ptr(bucket), ptr(code8), ptr(np8),
-                           ptr(totals), ptr(consumed))
-    return dict(words=words, bucket=bucket, code8=code8, np8=np8, totals=totals, consumed=int(consumed[0]))
+    dist32 = np.zeros((L.count32, 16), dtype=np.float32) if with_dist else None
+    lib().or_quant_analyze_ex(C.byref(L), ptr(coeffs), ptr(words), ptr(bucket), ptr(code8), ptr(np8),
+                              ptr(totals), ptr(consumed), ptr(dist32))
+    r = dict(words=words, bucket=bucket, code8=code8, np8=np8, totals=totals, consumed=int(consumed[0]))
+    if with_dist:
+        r["dist32"] = dist32
+    return r
+
+
+def block8_stats(L, coeffs, c, l, b, x8, y8):
+    """BlockStats of one 8x8 block: (cost[16] bits, dist[16], code word, msb)."""
+    coeffs = np.ascontiguousarray(coeffs, dtype=np.float32)
+    cost = np.zeros(16, np.uint16)
+    dist = np.zeros(16, np.float32)
+    cw = C.c_uint32()
+    msb = C.c_int()
+    if lib().or_block8_stats(C.byref(L), ptr(coeffs), c, l, b, x8, y8, ptr(cost), ptr(dist),
+                             C.byref(cw), C.byref(msb)) != 0:
+        raise ValueError("block outside the band")
+    return cost, dist, int(cw.value), int(msb.value)
+
+
+def bucket_of_ratio(r):
+    return int(lib().or_bucket_of_ratio(float(r)))
+
+
+def bucket_scan(raw):
+    raw = np.ascontiguousarray(raw, np.uint32)
+    out = np.zeros(16, np.uint32)
+    lib().or_bucket_scan(ptr(raw), ptr(out))
+    return [int(x) for x in out]
 
 
 def finalize(totals, q1=True):

@@ -900,9 +900,59 @@ static void band_constants(const or_layout *L, int c, int l, int b, float *qres,
 	*rdo = or_rdo_distortion_scale(L, l, c, b);
 }
 
+uint32_t or_bucket_of_ratio(float r) { return bucket_of_ratio(r); }
+
+void or_bucket_scan(const uint32_t raw[16], uint32_t out[16])
+{
+	/* inclusive_max_clustered16 (analyze_rate_control.comp:85-99): clamp to
+	 * 128 - 16 + q, then the inclusive max of v[k] + (q - k) over k <= q. */
+	uint32_t v[16];
+	for (int q = 0; q < 16; q++)
+		v[q] = raw[q] < 112u + (uint32_t)q ? raw[q] : 112u + (uint32_t)q;
+	for (int q = 0; q < 16; q++)
+	{
+		uint32_t best = v[q];
+		for (int k = 0; k < q; k++)
+		{
+			uint32_t cand = v[k] + (uint32_t)(q - k);
+			if (cand > best)
+				best = cand;
+		}
+		out[q] = best;
+	}
+}
+
+int or_block8_stats(const or_layout *L, const float *coeffs, int c, int l, int b, int x8, int y8,
+                    uint16_t cost[16], float dist[16], uint32_t *code_word, int *msb)
+{
+	const or_band *B = &L->band[c][l][b];
+	if (!B->present || x8 < 0 || y8 < 0 || x8 >= B->bx8 || y8 >= B->by8)
+		return -1;
+	float qres, rdo;
+	band_constants(L, c, l, b, &qres, &rdo);
+	q8x8 *Q = (q8x8 *)malloc(sizeof(q8x8));
+	quantize_8x8(coeffs + B->coeff_off, B->bw, B->bw, B->bh, 8 * x8, 8 * y8, qres, rdo, Q);
+	for (int q = 0; q < 16; q++)
+	{
+		cost[q] = Q->cost[q];
+		dist[q] = or_f16_to_f32(Q->dist[q]);
+	}
+	*code_word = Q->code_word;
+	*msb = Q->msb;
+	free(Q);
+	return 0;
+}
+
 int or_quant_analyze(const or_layout *L, const float *coeffs,
                      uint16_t *words, uint8_t *bucket, uint32_t *code8, uint8_t *np8,
                      uint32_t *totals, uint32_t *consumed)
+{
+	return or_quant_analyze_ex(L, coeffs, words, bucket, code8, np8, totals, consumed, NULL);
+}
+
+int or_quant_analyze_ex(const or_layout *L, const float *coeffs,
+                        uint16_t *words, uint8_t *bucket, uint32_t *code8, uint8_t *np8,
+                        uint32_t *totals, uint32_t *consumed, float *dist32)
 {
 	memset(totals, 0, sizeof(uint32_t) * FLAT);
 	*consumed = 0;
@@ -958,6 +1008,8 @@ int or_quant_analyze(const or_layout *L, const float *coeffs,
 							const float *d = dist[q];
 							D[q] = (((d[0] + d[1]) + (d[2] + d[3])) + ((d[4] + d[5]) + (d[6] + d[7]))) +
 							       (((d[8] + d[9]) + (d[10] + d[11])) + ((d[12] + d[13]) + (d[14] + d[15])));
+							if (dist32)
+								dist32[block * 16 + q] = D[q];
 						}
 
 						/* emit_rdo_operations (analyze_rate_control.comp:101-147). */
@@ -976,20 +1028,10 @@ int or_quant_analyze(const or_layout *L, const float *coeffs,
 									bi = bucket_of_ratio(dd / (base_f - cost_f));
 								}
 							}
-							v[q] = bi < 112u + q ? bi : 112u + q;
+							v[q] = bi;
 						}
 						uint32_t B_q[16];
-						for (int q = 0; q < 16; q++)
-						{
-							uint32_t best = v[q];
-							for (int k = 0; k < q; k++)
-							{
-								uint32_t cand = v[k] + (uint32_t)(q - k);
-								if (cand > best)
-									best = cand;
-							}
-							B_q[q] = best;
-						}
+						or_bucket_scan(v, B_q);
 
 						*consumed += w[0];
 						for (int q = 0; q < 16; q++)

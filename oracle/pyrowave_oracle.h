@@ -20,11 +20,27 @@
  *                                  with the textbook 9/7 analysis filters
  *   FP32 DWT / iDWT .............. error bound against the FP64 transform
  *   fp16 conversion .............. exhaustive comparison with numpy float16
- *   quant scale / magnitudes ..... closed-form dequant error bound
- *                                  |c - c'| <= step/2, M in [1,1.25) -> zero
- *   stats / analyze / buckets .... brute-force per-block recomputation in the
- *                                  tests, monotone word counts, bucket
- *                                  strictly increasing in q
+ *   quant scale codes ............ exact-rational minimal code keeping the
+ *                                  block peak under 2^e - 1/4
+ *   magnitudes / quant levels .... closed-form deadzone bound at step 2^q*D
+ *                                  through pack -> parse -> dequant
+ *   distortion tables ............ squared reconstruction error of the
+ *                                  dequantized block at every quant level,
+ *                                  times the distortion weight, fp16-stored
+ *   cost tables / words .......... structural size of every block packed at
+ *                                  every quant level (validate_bitstream)
+ *   buckets ...................... float64 log2 away from ties; literal run of
+ *                                  the shader's max-scan, clamp included
+ *   savings totals / consumed .... recomputed from the word/bucket tables
+ *   distortion weight ............ published Mannos-Sakrison CSF at the band
+ *                                  midpoints
+ *   quant resolution, band term .. measured analysis gains of the FP64 9/7
+ *   (tests/test_oracle_pinning.py; tests/test_oracle_mutants.py plants a
+ *   mistake in each of these and requires a check to fail)
+ *   parity unpinned, constants only: the level and chroma terms of the quant
+ *   resolution (+1 bit per level, -1 bit for chroma) and its caps
+ *   (pyrowave_encoder.cpp:144-172) are the reference encoder's choices, fixed
+ *   by no bitstream rule or mathematical property; they are transcribed.
  *   finalize (quirk Q1) .......... literal simulation of the reference's
  *                                  512-thread Hillis-Steele loop
  *   resolve ...................... literal simulation of the reference's
@@ -129,6 +145,23 @@ int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int p
 int or_quant_analyze(const or_layout *L, const float *coeffs,
                      uint16_t *words, uint8_t *bucket, uint32_t *code8, uint8_t *np8,
                      uint32_t *totals, uint32_t *consumed);
+/* The same, also writing dist32[b*16+q]: the 32x32 block's summed distortion
+ * at quant level q (analyze_rate_control.comp:169-215), when not NULL. */
+int or_quant_analyze_ex(const or_layout *L, const float *coeffs,
+                        uint16_t *words, uint8_t *bucket, uint32_t *code8, uint8_t *np8,
+                        uint32_t *totals, uint32_t *consumed, float *dist32);
+/* One 8x8 block of band (c, l, b) at 8x8 position (x8, y8): the BlockStats
+ * entries errors[min(q, num_planes)] for q = 0..15 (payload cost in bits and
+ * the fp16-stored distortion), the code word at q = 0 and msb (num_planes - 1,
+ * -1 when the block quantizes to zero).  Returns -1 outside the band. */
+int or_block8_stats(const or_layout *L, const float *coeffs, int c, int l, int b, int x8, int y8,
+                    uint16_t cost[16], float dist[16], uint32_t *code_word, int *msb);
+/* uint(max(60 + 2 * log2(r) + 0.5, 0)), evaluated exactly
+ * (analyze_rate_control.comp:73-83). */
+uint32_t or_bucket_of_ratio(float r);
+/* The monotone bucket assignment over the 16 quant levels of a 32x32 block
+ * (analyze_rate_control.comp:85-99): raw[0] must be 0. */
+void or_bucket_scan(const uint32_t raw[16], uint32_t out[16]);
 
 /* analyze_rate_control_finalize.comp.  q1 != 0 reproduces the reference's
  * 8-step scan (windowed sums for flat index >= 1024); q1 == 0 is a true
