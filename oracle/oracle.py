@@ -29,7 +29,8 @@ class Layout(C.Structure):
                 ("aligned_w", C.c_int), ("aligned_h", C.c_int),
                 ("count8", C.c_int), ("count32", C.c_int), ("coeff_count", C.c_long),
                 ("per_subdivision", C.c_int),
-                ("band", Band * BANDS * LEVELS * COMPONENTS)]
+                ("band", Band * BANDS * LEVELS * COMPONENTS),
+                ("precision0", C.c_int)]
 
     def b(self, c, l, b):
         return self.band[c][l][b]
@@ -68,6 +69,10 @@ def load_library(path):
     L.or_block_descs.argtypes = [C.POINTER(Layout), P]
     L.or_quant_resolution.restype = C.c_float
     L.or_quant_resolution.argtypes = [C.c_int, C.c_int, C.c_int]
+    L.or_quant_resolution_policy.restype = C.c_float
+    L.or_quant_resolution_policy.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+    L.or_round_f64_to_f16.restype = C.c_float
+    L.or_round_f64_to_f16.argtypes = [C.c_double]
     L.or_rdo_distortion_scale.restype = C.c_float
     L.or_rdo_distortion_scale.argtypes = [C.POINTER(Layout), C.c_int, C.c_int, C.c_int]
     L.or_encode_quant.restype = C.c_uint8
@@ -85,6 +90,8 @@ def load_library(path):
     L.or_fwd_ext.argtypes = [C.c_int, C.c_int, C.c_int]
     for name in ("or_dwt_level_f64", "or_dwt_level_f32"):
         getattr(L, name).argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
+    L.or_dwt_level_h.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P, P, P, P, C.c_int]
+    L.or_idwt_level_h.argtypes = [P, P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
     for name in ("or_idwt_level_f64", "or_idwt_level_f32"):
         getattr(L, name).argtypes = [P, P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
     L.or_dwt_frame.argtypes = [C.POINTER(Layout), P, P, C.c_int, P, P]
@@ -142,11 +149,14 @@ def ptr(a):
     return a.ctypes.data_as(C.c_void_p) if a is not None else None
 
 
-def layout(width, height, chroma):
+def layout(width, height, chroma, precision=None):
+    """precision 0 selects the precision-0 quantizer policy for the functions
+    that take the layout (quant_analyze, pack)."""
     assert lib().or_layout_size() == C.sizeof(Layout)
     L = Layout()
     if lib().or_layout_init(C.byref(L), width, height, chroma) != 0:
         raise ValueError("invalid dimensions")
+    L.precision0 = 1 if precision == 0 else 0
     return L
 
 

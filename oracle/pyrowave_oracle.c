@@ -151,10 +151,11 @@ int or_block_descs(const or_layout *L, or_block_desc *out)
 /* Quantizer constants: pyrowave_encoder.cpp:107-172, pyrowave_common.hpp:124-145 */
 /* ------------------------------------------------------------------------ */
 
-static float noise_power_normalized_quant_resolution(int level, int component, int band)
+static float noise_power_normalized_quant_resolution(int level, int component, int band, int precision0)
 {
-	/* pyrowave_encoder.cpp:152-172 with precision >= 1 (8 base bits). */
-	int bits = 8;
+	/* pyrowave_encoder.cpp:152-172: 8 base bits for precision >= 1, 6 for
+	 * precision 0. */
+	int bits = precision0 ? 6 : 8;
 	if (band == 0)
 		bits += 2;
 	else if (band < 3)
@@ -165,11 +166,18 @@ static float noise_power_normalized_quant_resolution(int level, int component, i
 	return (float)(1 << bits);
 }
 
+float or_quant_resolution_policy(int level, int component, int band, int precision0)
+{
+	/* pyrowave_encoder.cpp:144-150: cap 4096 for precision >= 1, 512 for
+	 * precision 0 ("FP16 range is limited"). */
+	float r = noise_power_normalized_quant_resolution(level, component, band, precision0);
+	float cap = precision0 ? 512.0f : 4096.0f;
+	return r < cap ? r : cap;
+}
+
 float or_quant_resolution(int level, int component, int band)
 {
-	/* pyrowave_encoder.cpp:144-150: cap 4096 for precision >= 1. */
-	float r = noise_power_normalized_quant_resolution(level, component, band);
-	return r < 4096.0f ? r : 4096.0f;
+	return or_quant_resolution_policy(level, component, band, 0);
 }
 
 float or_rdo_distortion_scale(const or_layout *L, int level, int component, int band)
@@ -190,7 +198,7 @@ float or_rdo_distortion_scale(const or_layout *L, int level, int component, int 
 		if (L->chroma == 0)
 			csf *= 0.6f;
 	}
-	float resolution = noise_power_normalized_quant_resolution(level, component, band);
+	float resolution = noise_power_normalized_quant_resolution(level, component, band, L->precision0);
 	float weighted = csf * resolution;
 	return (weighted * weighted) * (1.0f / 256.0f);
 }
@@ -338,6 +346,27 @@ void or_round_to_f16_array(float *v, long n)
 		v[i] = or_f16_to_f32(or_f32_to_f16(v[i]));
 }
 
+float or_round_f64_to_f16(double d)
+{
+	/* Round to nearest even, subnormals kept (the magnitudes here stay far
+	 * from the overflow threshold). */
+	if (d == 0.0 || d != d)
+		return (float)d;
+	double a = fabs(d);
+	int e;
+	frexp(a, &e); /* a = m * 2^e, m in [0.5, 1) */
+	/* fp16 spacing: 2^(e - 11) in the normal range, 2^-24 below 2^-14. */
+	int qe = e - 11;
+	if (qe < -24)
+		qe = -24;
+	double q = ldexp(a, -qe);
+	double r = nearbyint(q); /* default rounding mode: to nearest even */
+	double out = ldexp(r, qe);
+	if (out > 65504.0)
+		out = INFINITY;
+	return (float)(d < 0 ? -out : out);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Forward DWT: shaders/dwt.comp                                            */
 /* ------------------------------------------------------------------------ */
@@ -393,6 +422,69 @@ static void lift_fwd_f64(double *e, int A)
 		x[p] += D_GAMMA * (x[p - 1] + x[p + 1]);
 	for (int p = 0; p <= A - 2; p += 2)
 		x[p] += D_DELTA * (x[p - 1] + x[p + 1]);
+}
+
+/* FP16 arithmetic (precision 0, dwt_common.h:29-34 FLOAT = float16_t): every
+ * value is an fp16 number held in a float.  A lifting step is an fp16 add
+ * followed by a fused fp16 multiply-add with the fp16-rounded constant (both
+ * evaluated exactly in double, then rounded once: the products of two fp16
+ * numbers and the sums here fit in a double's significand).  The K / 1/K
+ * scaling is done in fp32 and rounded to fp16. */
+static float h_add(float a, float b) { return or_round_f64_to_f16((double)a + (double)b); }
+static float h_fma(float c, float s, float v) { return or_round_f64_to_f16((double)c * (double)s + (double)v); }
+static float h_scale(float v, float k) { return or_f16_to_f32(or_f32_to_f16(v * k)); }
+static float h_const(float c) { return or_f16_to_f32(or_f32_to_f16(c)); }
+
+static void lift_fwd_h(float *e, int A)
+{
+	const float a = h_const(OR_ALPHA), b = h_const(OR_BETA), g = h_const(OR_GAMMA), d = h_const(OR_DELTA);
+	float *x = e + APRON;
+	for (int p = -3; p <= A + 1; p += 2)
+		x[p] = h_fma(a, h_add(x[p - 1], x[p + 1]), x[p]);
+	for (int p = -2; p <= A; p += 2)
+		x[p] = h_fma(b, h_add(x[p - 1], x[p + 1]), x[p]);
+	for (int p = -1; p <= A - 1; p += 2)
+		x[p] = h_fma(g, h_add(x[p - 1], x[p + 1]), x[p]);
+	for (int p = 0; p <= A - 2; p += 2)
+		x[p] = h_fma(d, h_add(x[p - 1], x[p + 1]), x[p]);
+}
+
+void or_dwt_level_h(const float *in, int in_pitch, int R_x, int R_y, int A_x, int A_y,
+                    float *ll, float *hl, float *lh, float *hh, int out_pitch)
+{
+	/* or_dwt_level_f32 in fp16 arithmetic; `in` holds fp16 values. */
+	int W2 = A_x / 2, H2 = A_y / 2;
+	float *hrows = (float *)malloc(sizeof(float) * (size_t)A_x * R_y);
+	float *line = (float *)malloc(sizeof(float) * (size_t)(imax(A_x, A_y) + 2 * APRON));
+	for (int y = 0; y < R_y; y++)
+	{
+		const float *row = in + (size_t)y * in_pitch;
+		for (int p = -APRON; p < A_x + APRON; p++)
+			line[p + APRON] = row[or_fwd_ext(p, R_x, A_x)];
+		lift_fwd_h(line, A_x);
+		float *o = hrows + (size_t)y * A_x;
+		for (int k = 0; k < W2; k++)
+		{
+			o[k] = h_scale(line[APRON + 2 * k], OR_INV_K);
+			o[W2 + k] = h_scale(line[APRON + 2 * k + 1], OR_K);
+		}
+	}
+	for (int x = 0; x < A_x; x++)
+	{
+		for (int p = -APRON; p < A_y + APRON; p++)
+			line[p + APRON] = hrows[(size_t)or_fwd_ext(p, R_y, A_y) * A_x + x];
+		lift_fwd_h(line, A_y);
+		float *lo = x < W2 ? ll : hl;
+		float *hi = x < W2 ? lh : hh;
+		int ox = x < W2 ? x : x - W2;
+		for (int k = 0; k < H2; k++)
+		{
+			lo[(size_t)k * out_pitch + ox] = h_scale(line[APRON + 2 * k], OR_INV_K);
+			hi[(size_t)k * out_pitch + ox] = h_scale(line[APRON + 2 * k + 1], OR_K);
+		}
+	}
+	free(hrows);
+	free(line);
 }
 
 void or_dwt_level_f32(const float *in, int in_pitch, int R_x, int R_y, int A_x, int A_y,
@@ -525,6 +617,57 @@ static void lift_inv_f64(double *e, int N)
 		x[p] -= D_ALPHA * (x[p - 1] + x[p + 1]);
 }
 
+static void lift_inv_h(float *e, int N)
+{
+	const float a = -h_const(OR_ALPHA), b = -h_const(OR_BETA), g = -h_const(OR_GAMMA), d = -h_const(OR_DELTA);
+	float *x = e + APRON;
+	for (int p = -2; p <= N + 2; p += 2)
+		x[p] = h_fma(d, h_add(x[p - 1], x[p + 1]), x[p]);
+	for (int p = -1; p <= N + 1; p += 2)
+		x[p] = h_fma(g, h_add(x[p - 1], x[p + 1]), x[p]);
+	for (int p = 0; p <= N; p += 2)
+		x[p] = h_fma(b, h_add(x[p - 1], x[p + 1]), x[p]);
+	for (int p = 1; p <= N - 1; p += 2)
+		x[p] = h_fma(a, h_add(x[p - 1], x[p + 1]), x[p]);
+}
+
+void or_idwt_level_h(const float *ll, const float *hl, const float *lh, const float *hh,
+                     int in_pitch, int bw, int bh, float *out, int out_pitch)
+{
+	/* or_idwt_level_f32 in fp16 arithmetic; the bands hold fp16 values. */
+	int W = 2 * bw, H = 2 * bh;
+	float *cols = (float *)malloc(sizeof(float) * (size_t)W * H);
+	float *line = (float *)malloc(sizeof(float) * (size_t)(imax(W, H) + 2 * APRON + 2));
+	for (int x = 0; x < W; x++)
+	{
+		const float *lo = (x & 1) ? hl : ll;
+		const float *hi = (x & 1) ? hh : lh;
+		int bx = x >> 1;
+		for (int p = -APRON; p < H + APRON; p++)
+		{
+			int q = isym(p, H);
+			float v = (q & 1) ? hi[(size_t)(q >> 1) * in_pitch + bx] : lo[(size_t)(q >> 1) * in_pitch + bx];
+			line[p + APRON] = h_scale(v, (p & 1) ? OR_INV_K : OR_K);
+		}
+		lift_inv_h(line, H);
+		for (int y = 0; y < H; y++)
+			cols[(size_t)y * W + x] = line[APRON + y];
+	}
+	for (int y = 0; y < H; y++)
+	{
+		for (int p = -APRON; p < W + APRON; p++)
+		{
+			float v = cols[(size_t)y * W + isym(p, W)];
+			line[p + APRON] = h_scale(v, (p & 1) ? OR_INV_K : OR_K);
+		}
+		lift_inv_h(line, W);
+		for (int x = 0; x < W; x++)
+			out[(size_t)y * out_pitch + x] = line[APRON + x];
+	}
+	free(cols);
+	free(line);
+}
+
 void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const float *hh,
                        int in_pitch, int bw, int bh, float *out, int out_pitch)
 {
@@ -609,6 +752,13 @@ static float unorm8_dc(uint8_t u)
 	return (float)u / 255.0f - 0.5f;
 }
 
+static float unorm8_dc_h(uint8_t u)
+{
+	/* Precision 0: the texel converted to fp16 (VEC4 = f16vec4), then the DC
+	 * shift in fp16 (dwt.comp:44-48). */
+	return h_add(or_f16_to_f32(or_f32_to_f16((float)u / 255.0f)), -0.5f);
+}
+
 static void plane_dims(const or_layout *L, int c, int *w, int *h)
 {
 	*w = L->width;
@@ -651,6 +801,8 @@ int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int p
 						uint8_t u = planes[c][(size_t)y * pitches[c] + x];
 						if (use_f64)
 							in64[(size_t)y * pw + x] = (double)u / 255.0 - 0.5;
+						else if (precision == 0)
+							in32[(size_t)y * pw + x] = unorm8_dc_h(u);
 						else
 							in32[(size_t)y * pw + x] = unorm8_dc(u);
 					}
@@ -670,6 +822,10 @@ int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int p
 				or_dwt_level_f64(in64, in_pitch, R_x, R_y, A_x, A_y,
 				                 coeffs64 + B[0].coeff_off, coeffs64 + B[1].coeff_off,
 				                 coeffs64 + B[2].coeff_off, coeffs64 + B[3].coeff_off, B[0].bw);
+			else if (precision == 0)
+				or_dwt_level_h(in32, in_pitch, R_x, R_y, A_x, A_y,
+				               coeffs + B[0].coeff_off, coeffs + B[1].coeff_off,
+				               coeffs + B[2].coeff_off, coeffs + B[3].coeff_off, B[0].bw);
 			else
 			{
 				or_dwt_level_f32(in32, in_pitch, R_x, R_y, A_x, A_y,
@@ -895,7 +1051,7 @@ static uint32_t bucket_of_ratio(float r)
 static void band_constants(const or_layout *L, int c, int l, int b, float *qres, float *rdo)
 {
 	/* pyrowave_encoder.cpp:450-459. */
-	float quant_res = or_quant_resolution(l, c, b);
+	float quant_res = or_quant_resolution_policy(l, c, b, L->precision0);
 	*qres = 1.0f / or_decode_quant(or_encode_quant(1.0f / quant_res));
 	*rdo = or_rdo_distortion_scale(L, l, c, b);
 }
@@ -1206,7 +1362,7 @@ long or_pack(const or_layout *L, const float *coeffs, const int32_t *quant, int 
 					continue;
 				float qres, rdo;
 				band_constants(L, c, l, b, &qres, &rdo);
-				uint32_t qcode_band = or_encode_quant(1.0f / or_quant_resolution(l, c, b));
+				uint32_t qcode_band = or_encode_quant(1.0f / or_quant_resolution_policy(l, c, b, L->precision0));
 				const float *band = coeffs + B->coeff_off;
 
 				for (int by = 0; by < B->by32; by++)
@@ -1325,6 +1481,11 @@ long or_encode(const or_layout *L, const uint8_t *const planes[3], const int pit
 	uint8_t *np8 = (uint8_t *)calloc((size_t)L->count8, 1);
 	uint32_t totals[FLAT], prefix[FLAT], consumed;
 	int32_t *quant = quant_out ? quant_out : (int32_t *)malloc(sizeof(int32_t) * (size_t)L->count32);
+	/* Precision 0 selects its own quantizer policy (pyrowave_encoder.cpp:144-172). */
+	or_layout L0 = *L;
+	if (precision == 0)
+		L0.precision0 = 1;
+	L = &L0;
 
 	or_dwt_frame(L, planes, pitches, precision, coeffs, NULL);
 	or_quant_analyze(L, coeffs, words, bucket, code8, np8, totals, &consumed);
@@ -1794,13 +1955,19 @@ int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
 			if ((precision == 1 || precision == 3) && l < 2)
 				for (int b = precision == 3 ? 0 : 1; b < 4; b++)
 					or_round_to_f16_array(coeffs + B[b].coeff_off, (long)B[b].bw * B[b].bh);
+			/* precision 0: every band in fp16 storage, fp16 arithmetic. */
+			if (precision == 0)
+				for (int b = l == OR_LEVELS - 1 ? 0 : 1; b < 4; b++)
+					or_round_to_f16_array(coeffs + B[b].coeff_off, (long)B[b].bw * B[b].bh);
+			void (*level)(const float *, const float *, const float *, const float *, int, int, int, float *,
+			              int) = precision == 0 ? or_idwt_level_h : or_idwt_level_f32;
 			if (l == first_level)
 			{
 				int pw, ph;
 				plane_dims(L, c, &pw, &ph);
 				float *tmp = (float *)malloc(sizeof(float) * (size_t)4 * bw * bh);
-				or_idwt_level_f32(coeffs + B[0].coeff_off, coeffs + B[1].coeff_off, coeffs + B[2].coeff_off,
-				                  coeffs + B[3].coeff_off, bw, bw, bh, tmp, 2 * bw);
+				level(coeffs + B[0].coeff_off, coeffs + B[1].coeff_off, coeffs + B[2].coeff_off,
+				      coeffs + B[3].coeff_off, bw, bw, bh, tmp, 2 * bw);
 				for (int y = 0; y < ph && y < 2 * bh; y++)
 					for (int x = 0; x < pw && x < 2 * bw; x++)
 						planes[c][(size_t)y * pitches[c] + x] = unorm8_store(tmp[(size_t)y * 2 * bw + x]);
@@ -1809,8 +1976,8 @@ int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
 			else
 			{
 				const or_band *P = &L->band[c][l - 1][0];
-				or_idwt_level_f32(coeffs + B[0].coeff_off, coeffs + B[1].coeff_off, coeffs + B[2].coeff_off,
-				                  coeffs + B[3].coeff_off, bw, bw, bh, coeffs + P->coeff_off, P->bw);
+				level(coeffs + B[0].coeff_off, coeffs + B[1].coeff_off, coeffs + B[2].coeff_off,
+				      coeffs + B[3].coeff_off, bw, bw, bh, coeffs + P->coeff_off, P->bw);
 			}
 		}
 	}

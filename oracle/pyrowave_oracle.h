@@ -84,6 +84,7 @@ typedef struct or_layout
 	long coeff_count;          /* total floats of the flat coefficient array */
 	int per_subdivision;       /* next_pow2(ceil(count32 / 16)) */
 	or_band band[OR_COMPONENTS][OR_LEVELS][OR_BANDS];
+	int precision0;            /* 1: quantizer policy of precision 0 (set by the caller) */
 } or_layout;
 
 /* 32x32 block -> band lookup, in block-index order. */
@@ -99,11 +100,16 @@ int or_block_descs(const or_layout *L, or_block_desc *out);
 
 /* Encoder-side quantizer constants (pyrowave_encoder.cpp:107-172, 458-459). */
 float or_quant_resolution(int level, int component, int band);
+/* precision0 != 0: the policy of the reference's precision 0 (6 base bits,
+ * cap 512); else precision >= 1 (8 base bits, cap 4096).
+ * or_rdo_distortion_scale and the quantizer follow L->precision0. */
+float or_quant_resolution_policy(int level, int component, int band, int precision0);
 float or_rdo_distortion_scale(const or_layout *L, int level, int component, int band);
 uint8_t or_encode_quant(float decoder_q_scale);
 float or_decode_quant(uint8_t code);
 
 /* fp16 helpers (round to nearest even, subnormals kept). */
+float or_round_f64_to_f16(double d);
 uint16_t or_f32_to_f16(float f);
 float or_f16_to_f32(uint16_t h);
 void or_round_to_f16_array(float *v, long n);
@@ -125,6 +131,14 @@ void or_idwt_level_f64(const double *ll, const double *hl, const double *lh, con
                        int in_pitch, int bw, int bh, double *out, int out_pitch);
 void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const float *hh,
                        int in_pitch, int bw, int bh, float *out, int out_pitch);
+/* The same levels in fp16 arithmetic (precision 0): inputs hold fp16 values;
+ * lifting steps are an fp16 add then an fp16 fused multiply-add with the
+ * fp16-rounded constant; the K / 1/K scaling is an fp32 product rounded to
+ * fp16. */
+void or_dwt_level_h(const float *in, int in_pitch, int R_x, int R_y, int A_x, int A_y,
+                    float *ll, float *hl, float *lh, float *hh, int out_pitch);
+void or_idwt_level_h(const float *ll, const float *hl, const float *lh, const float *hh,
+                     int in_pitch, int bw, int bh, float *out, int out_pitch);
 
 /* Whole-frame forward transform into the flat coefficient array.
  * precision 2: FP32 everywhere.  precision 1: FP32 math; the high bands
@@ -133,6 +147,9 @@ void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const 
  * levels 1 and 2 transform the fp16-rounded LL (the reference's precision 1
  * storage, pyrowave_common.cpp:133-189: one R16F image holds every band of
  * levels 0-1, and component_ll_views feed the next level from it).
+ * precision 0: fp16 arithmetic and fp16 values at every level (the
+ * reference's precision 0: FLOAT = float16_t, R16F storage at all levels),
+ * level-0 input fp16(fp16(u / 255) - 0.5).
  * use_f64 != 0: FP64 math; result written to coeffs64 (may be NULL otherwise). */
 int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                  int precision, float *coeffs, double *coeffs64);
@@ -178,7 +195,8 @@ int or_resolve(const or_layout *L, const uint16_t *words, const uint8_t *bucket,
 long or_pack(const or_layout *L, const float *coeffs, const int32_t *quant, int sequence,
              uint32_t *bitstream, long bitstream_capacity_words, uint32_t *meta);
 
-/* Full encode = dwt + quant_analyze + finalize + resolve + pack. */
+/* Full encode = dwt + quant_analyze + finalize + resolve + pack.  precision 0
+ * also selects the precision-0 quantizer policy. */
 long or_encode(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                int precision, size_t budget, int sequence, int q1,
                uint32_t *bitstream, long capacity_words, uint32_t *meta, int32_t *quant_out);
@@ -213,7 +231,8 @@ int or_dequant(const or_layout *L, const uint32_t *payload, long payload_words,
 /* idwt.comp chain: levels 4 -> 0, u8 output planes with DC shift and clamp.
  * precision 1 rounds the input high bands of levels 0-1 to fp16 first (the
  * decoder's fp16 band storage); precision 3 also rounds the LL of levels 0-1,
- * which the inverse of the level above wrote (R16F in the reference). */
+ * which the inverse of the level above wrote (R16F in the reference);
+ * precision 0 rounds every input band to fp16 and runs or_idwt_level_h. */
 int or_idwt_frame(const or_layout *L, const float *coeffs, int precision,
                   uint8_t *const planes[3], const int pitches[3]);
 int or_idwt_frame_f64(const or_layout *L, const double *coeffs, double *const planes[3], const int pitches[3]);
