@@ -25,6 +25,15 @@ extern "C" {
 
 typedef enum pyrowave_hip_precision
 {
+	/* The reference's precision 0 (dwt_common.h:29-34, FLOAT = float16_t;
+	 * pyrowave_common.cpp:133-189, R16F at every level): the lifting steps in
+	 * packed FP16 (v_pk_add_f16 + v_pk_fma_f16 on two rows or two columns at
+	 * once), the K / 1/K scaling of each pass in FP32 rounded to FP16, every
+	 * band of every level stored as FP16, and the precision-0 quantizer
+	 * policy (6 base bits, resolution capped at 512;
+	 * pyrowave_encoder.cpp:144-172).  One batch must not mix it with the other
+	 * precisions. */
+	PYROWAVE_HIP_PRECISION_FP16_MATH = 0,
 	/* FP32 lifting; the high bands (HL, LH, HH) of levels 0-1 are stored as
 	 * FP16, everything else as FP32.  Unlike the reference's precision 1
 	 * (pyrowave_common.cpp:133-157) the LL intermediates of levels 0-1 stay

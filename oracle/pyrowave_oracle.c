@@ -429,10 +429,11 @@ static void lift_fwd_f64(double *e, int A)
  * followed by a fused fp16 multiply-add with the fp16-rounded constant (both
  * evaluated exactly in double, then rounded once: the products of two fp16
  * numbers and the sums here fit in a double's significand).  The K / 1/K
- * scaling is done in fp32 and rounded to fp16. */
+ * scaling multiplies by the FP32 constant and rounds the exact product once
+ * to fp16 (an fp16 x fp32 product has 35 significant bits). */
 static float h_add(float a, float b) { return or_round_f64_to_f16((double)a + (double)b); }
 static float h_fma(float c, float s, float v) { return or_round_f64_to_f16((double)c * (double)s + (double)v); }
-static float h_scale(float v, float k) { return or_f16_to_f32(or_f32_to_f16(v * k)); }
+static float h_scale(float v, float k) { return or_round_f64_to_f16((double)v * (double)k); }
 static float h_const(float c) { return or_f16_to_f32(or_f32_to_f16(c)); }
 
 static void lift_fwd_h(float *e, int A)

@@ -133,8 +133,8 @@ void or_idwt_level_f32(const float *ll, const float *hl, const float *lh, const 
                        int in_pitch, int bw, int bh, float *out, int out_pitch);
 /* The same levels in fp16 arithmetic (precision 0): inputs hold fp16 values;
  * lifting steps are an fp16 add then an fp16 fused multiply-add with the
- * fp16-rounded constant; the K / 1/K scaling is an fp32 product rounded to
- * fp16. */
+ * fp16-rounded constant; the K / 1/K scaling is the exact product of the
+ * fp16 value and the FP32 constant, rounded once to fp16. */
 void or_dwt_level_h(const float *in, int in_pitch, int R_x, int R_y, int A_x, int A_y,
                     float *ll, float *hl, float *lh, float *hh, int out_pitch);
 void or_idwt_level_h(const float *ll, const float *hl, const float *lh, const float *hh,

@@ -34,6 +34,7 @@ FORMAT_NV12 = 0
 FORMAT_YUV420P = 1
 FORMAT_YUV444P = 2
 
+PRECISION_FP16_MATH = 0
 PRECISION_FP16 = 1
 PRECISION_FP32 = 2
 PRECISION_FP16_LL = 3

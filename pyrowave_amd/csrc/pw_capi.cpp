@@ -92,19 +92,34 @@ namespace
 {
 std::mutex g_device_creation_lock;
 
+// The precision of the reference entry points (pyrowave_encoder_create,
+// pyrowave_decoder_create).  Reference pyrowave_common.cpp:15-28: the build's
+// PYROWAVE_PRECISION, 1 on desktop builds (CMakeLists.txt:44-49), unless the
+// PYROWAVE_PRECISION environment variable names another in [0, 2]; other
+// values are reported on stderr and ignored.  0 is FP16 arithmetic and
+// storage, 1 FP32 arithmetic with levels 0-1 stored as FP16, 2 FP32.
+constexpr int kReferenceDefaultPrecision = 1;
+
 int default_precision()
 {
-	// Reference pyrowave_common.cpp:15-28: PYROWAVE_PRECISION overrides the
-	// precision.  1 selects the reference's precision 1 storage (all bands
-	// of levels 0-1 as FP16), 0 (FP16 everywhere in the reference) the
-	// nearest one here, the same; 2 (default) FP32.
+	int precision = kReferenceDefaultPrecision;
 	if (const char *env = getenv("PYROWAVE_PRECISION"))
 	{
-		long v = strtol(env, nullptr, 0);
-		if (v == 0 || v == 1)
-			return PYROWAVE_HIP_PRECISION_FP16_LL;
+		const long v = strtol(env, nullptr, 0);
+		if (v < 0 || v > 2)
+			fprintf(stderr, "pyrowave: precision must be in range [0, 2].\n");
+		else
+			precision = int(v);
 	}
-	return PYROWAVE_HIP_PRECISION_FP32;
+	switch (precision)
+	{
+	case 0:
+		return PYROWAVE_HIP_PRECISION_FP16_MATH;
+	case 2:
+		return PYROWAVE_HIP_PRECISION_FP32;
+	default:
+		return PYROWAVE_HIP_PRECISION_FP16_LL;
+	}
 }
 
 struct PlanCache
@@ -137,6 +152,17 @@ struct PlanCache
 
 std::mutex g_cache_lock;
 std::map<Device *, PlanCache> g_caches;
+
+// A batch holds packed-FP16 frames or frames of the other precisions, not
+// both (their transforms are different kernels).
+bool mixes_half_math(const std::vector<Slot *> &slots)
+{
+	for (Slot *s : slots)
+		if ((s->geom.precision == PYROWAVE_HIP_PRECISION_FP16_MATH) !=
+		    (slots[0]->geom.precision == PYROWAVE_HIP_PRECISION_FP16_MATH))
+			return true;
+	return false;
+}
 
 Plan *get_plan(Device *dev, const std::vector<Slot *> &slots, bool encoder)
 {
@@ -1012,6 +1038,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encode_batch(pyrowave_encoder *
 	std::lock_guard<std::recursive_mutex> g(D.lock);
 	(void)hipSetDevice(D.hip_device);
 	hipStream_t stream = pick_stream(D, hip_stream);
+	if (mixes_half_math(slots))
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	Plan *P = get_plan(&D, slots, true);
 	if (!P)
 		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
@@ -1079,6 +1107,8 @@ static pyrowave_result decode_batch_common(pyrowave_decoder *decoders, pyrowave_
 	std::lock_guard<std::recursive_mutex> g(D.lock);
 	(void)hipSetDevice(D.hip_device);
 	hipStream_t stream = pick_stream(D, hip_stream);
+	if (mixes_half_math(slots))
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	Plan *P = get_plan(&D, slots, false);
 	if (!P)
 		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
@@ -1254,6 +1284,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_batch(pyrowave_d
 		J.h0_size = uint32_t(G.width - 1) | (uint32_t(G.height - 1) << 14);
 		J.chroma = G.chroma == PYROWAVE_CHROMA_SUBSAMPLING_444 ? 1u : 0u;
 	}
+	if (mixes_half_math(slots))
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	Plan *Pl = get_plan(&D, slots, false);
 	if (!Pl)
 		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;

@@ -35,13 +35,16 @@ constexpr int kFlat = kBuckets * kSubdiv;
 // windows; 3: 64 x 48 (256 threads) with 24-sample windows and each pass's
 // scaling applied once per sample (a third fewer lifting instructions).
 // Variant 4 of each direction streams instead: one wave per 120 x 32 tile
-// walks down its rows with the lifting in registers (no LDS).
+// walks down its rows with the lifting in registers (no LDS).  Variant 5 is
+// the same streamed shape in packed FP16 arithmetic; it is the only shape of
+// an FP16-arithmetic plan, and never one of another plan.
 constexpr int kApron = 4;
-constexpr int kFwdVariants = 5;
+constexpr int kFwdVariants = 5;  // FP32-arithmetic shapes
 constexpr int kInvVariants = 5;
-constexpr int kDwtVariants = 5; // table slots per level (the larger count)
+constexpr int kDwtVariants = 6;  // table slots per level
 constexpr int kFwdStreamVariant = 4;
 constexpr int kInvStreamVariant = 4;
+constexpr int kHalfStreamVariant = 5;
 // Shape of a level that has not been autotuned: the streamed one when the
 // level has at least kStreamMinTiles of its tiles (eight waves per CU; it is
 // the fastest on the large levels of a batch), else the tiled shape 0 (a
@@ -54,10 +57,10 @@ constexpr int default_dwt_variant(int stream_tiles)
 {
 	return stream_tiles >= kStreamMinTiles ? kFwdStreamVariant : 0;
 }
-constexpr int kFwdVariantTileW[kFwdVariants] = {128, 64, 128, 128, 120};
-constexpr int kFwdVariantTileH[kFwdVariants] = {56, 56, 48, 40, 32};
-constexpr int kInvVariantTileW[kInvVariants] = {64, 32, 64, 64, 120};
-constexpr int kInvVariantTileH[kInvVariants] = {64, 32, 56, 48, 32};
+constexpr int kFwdVariantTileW[kDwtVariants] = {128, 64, 128, 128, 120, 120};
+constexpr int kFwdVariantTileH[kDwtVariants] = {56, 56, 48, 40, 32, 32};
+constexpr int kInvVariantTileW[kDwtVariants] = {64, 32, 64, 64, 120, 120};
+constexpr int kInvVariantTileH[kDwtVariants] = {64, 32, 56, 48, 32, 32};
 
 // Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
 // sign bits, 8 B header and 3 B per 8x8 block, rounded up to words.

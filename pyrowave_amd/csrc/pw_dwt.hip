@@ -1075,6 +1075,362 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const 
 	else
 		inverse_stream_tile<kStoreF32, kStoreF32>(J, frames, ox0, oy0, lane);
 }
+
+// --- packed FP16 arithmetic, streamed (shape 5) ------------------------------
+//
+// The reference's precision 0 (shaders/dwt_common.h:29-34: FLOAT =
+// float16_t, f16vec2 lines): the streamed shape above with every lifting
+// step in packed FP16.  The horizontal pass packs the two rows of a row pair
+// (e0, e1) / (o0, o1), so one DPP shift moves both rows' neighbours; the
+// vertical delay line packs the two columns a lane holds (horizontal low and
+// high), so one v_pk_fma_f16 lifts both.  A step is an fp16 add and an fp16
+// fused multiply-add with the fp16-rounded constant.  The K / 1/K scaling
+// between the passes (and after the vertical one) multiplies by the FP32
+// constant and rounds the exact product once to fp16; it also transposes the
+// packing.  All sources and bands are
+// fp16; level 0 reads u8 through the table fp16(fp16(u / 255) - 0.5).
+typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+
+constexpr _Float16 kAlphaH = (_Float16)kAlpha;
+constexpr _Float16 kBetaH = (_Float16)kBeta;
+constexpr _Float16 kGammaH = (_Float16)kGamma;
+constexpr _Float16 kDeltaH = (_Float16)kDelta;
+
+__constant__ uint16_t c_u8_dc_h[256]; // fp16(fp16(u / 255) - 0.5)
+
+__device__ __forceinline__ h2 splat(_Float16 c) { return h2{c, c}; }
+__device__ __forceinline__ h2 fma2(h2 a, h2 b, h2 c) { return __builtin_elementwise_fma(a, b, c); }
+
+template <int Ctrl>
+__device__ __forceinline__ h2 dpp_wave_h2(h2 v)
+{
+	return __builtin_bit_cast(h2, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), Ctrl, 0xF, 0xF, false));
+}
+
+// The K / 1/K scaling: fp16 value times an FP32 constant, the exact product
+// rounded once to fp16 (v_fma_mix with S0 an fp16 half, S1 / S2 FP32, S2 =
+// -0 so that zeros keep their sign).  Written out because the compiler folds
+// only some of the equivalent cvt / mul / cvt sequences into this
+// instruction, and a folded and an unfolded product can round differently.
+// SEL: the half of `src` multiplied; mixlo writes the low half of the
+// result, mixhi the high half (the low half kept).
+template <int SEL>
+__device__ __forceinline__ h2 mixlo(h2 src, float k)
+{
+	h2 d;
+	const float mz = -0.0f;
+	if constexpr (SEL == 0)
+		asm("v_fma_mixlo_f16 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(d) : "v"(src), "v"(k), "v"(mz));
+	else
+		asm("v_fma_mixlo_f16 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(d) : "v"(src), "v"(k), "v"(mz));
+	return d;
+}
+
+template <int SEL>
+__device__ __forceinline__ h2 mixhi(h2 d, h2 src, float k)
+{
+	const float mz = -0.0f;
+	if constexpr (SEL == 0)
+		asm("v_fma_mixhi_f16 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "+v"(d) : "v"(src), "v"(k), "v"(mz));
+	else
+		asm("v_fma_mixhi_f16 %0, %1, %2, %3 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "+v"(d) : "v"(src), "v"(k), "v"(mz));
+	return d;
+}
+
+// (a.SA * ka, b.SB * kb), each product rounded once to fp16.
+template <int SA, int SB>
+__device__ __forceinline__ h2 scale_pair(h2 a, float ka, h2 b, float kb)
+{
+	return mixhi<SB>(mixlo<SA>(a, ka), b, kb);
+}
+
+// The sample at element offset `off` of a u8 plane (with the DC shift) or an
+// fp16 band, out of its aligned word.
+template <int KIND>
+__device__ __forceinline__ _Float16 stream_half(uint32_t raw, const StreamSrc &S, uint32_t off)
+{
+	const uint32_t a = S.lo + off * stream_esize<KIND>();
+	if constexpr (KIND == kStoreU8)
+		return __builtin_bit_cast(_Float16, c_u8_dc_h[(raw >> (8 * (a & 3))) & 0xffu]);
+	else
+		return __builtin_bit_cast(_Float16, (uint16_t)(raw >> (8 * (a & 2))));
+}
+
+__device__ __forceinline__ void store_half(void *base, uint32_t idx, _Float16 v)
+{
+	((__attribute__((address_space(1))) _Float16 *)base)[idx] = v;
+}
+
+// Horizontal lifting of a row pair across the lanes: e = (even sample of row
+// 0, of row 1), o likewise.
+__device__ __forceinline__ void hlift_rows_h(h2 &e, h2 &o)
+{
+	o = fma2(splat(kAlphaH), e + dpp_wave_h2<kDppWaveShl1>(e), o);
+	e = fma2(splat(kBetaH), dpp_wave_h2<kDppWaveShr1>(o) + o, e);
+	o = fma2(splat(kGammaH), e + dpp_wave_h2<kDppWaveShl1>(e), o);
+	e = fma2(splat(kDeltaH), dpp_wave_h2<kDppWaveShr1>(o) + o, e);
+}
+
+// VLine on the lane's two columns at once (x: horizontal low, y: high).
+struct VLineH
+{
+	h2 re = {}, ro = {}, po = {}, pe = {}, qo = {};
+
+	// Takes pair n (e: even row, o: odd row); gives pair n-2 scaled:
+	// low = (LL, HL), high = (LH, HH).
+	__device__ __forceinline__ void push(h2 e, h2 o, h2 &low, h2 &high)
+	{
+		const h2 o1 = fma2(splat(kAlphaH), re + e, ro);
+		const h2 e1 = fma2(splat(kBetaH), po + o1, re);
+		const h2 o2 = fma2(splat(kGammaH), pe + e1, po);
+		const h2 e2 = fma2(splat(kDeltaH), qo + o2, pe);
+		low = scale_pair<0, 1>(e2, kInvK, e2, kInvK);
+		high = scale_pair<0, 1>(o2, kK, o2, kK);
+		re = e;
+		ro = o;
+		po = o1;
+		pe = e1;
+		qo = o2;
+	}
+};
+
+template <int KIND>
+__device__ __forceinline__ void forward_stream_tile_h(const DwtJob &J, const void *src_ptr, uint32_t pitch,
+                                                      uint32_t step, int x0, int y0, int lane)
+{
+	const StreamSrc src = stream_src(src_ptr);
+	constexpr int TH = kFwdVariantTileH[kHalfStreamVariant];
+	constexpr int kPairs = TH / 2 + kApron;
+	static_assert(kPairs % 2 == 0, "the row-pair loop is unrolled by two");
+	const uint32_t xe = __umul24((uint32_t)fwd_ext(x0 - kApron + 2 * lane, J.R_x, J.A_x), step);
+	const uint32_t xo = __umul24((uint32_t)fwd_ext(x0 - kApron + 2 * lane + 1, J.R_x, J.A_x), step);
+	const int ox = (x0 >> 1) - kApron / 2 + lane;
+	const bool store_x = lane >= kApron / 2 && lane < 64 - kApron / 2 && ox < J.bw;
+	static_assert(2 * kPairs <= 64, "one window row per lane");
+	const uint32_t row_off =
+	    lane < 2 * kPairs ? __umul24((uint32_t)fwd_ext(y0 - kApron + lane, J.R_y, J.A_y), pitch) : 0u;
+	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
+		r0 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n);
+		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
+	};
+	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
+		r[0] = stream_raw<KIND>(src, r0 + xe);
+		r[1] = stream_raw<KIND>(src, r0 + xo);
+		r[2] = stream_raw<KIND>(src, r1 + xe);
+		r[3] = stream_raw<KIND>(src, r1 + xo);
+	};
+	VLineH line;
+	auto lift_pair = [&](int n, uint32_t r0, uint32_t r1, const uint32_t (&r)[4]) {
+		h2 e = h2{stream_half<KIND>(r[0], src, r0 + xe), stream_half<KIND>(r[2], src, r1 + xe)};
+		h2 o = h2{stream_half<KIND>(r[1], src, r0 + xo), stream_half<KIND>(r[3], src, r1 + xo)};
+		hlift_rows_h(e, o);
+		// Rows of the pair as (horizontal low, high), scaled.
+		const h2 v0 = scale_pair<0, 0>(e, kInvK, o, kK);
+		const h2 v1 = scale_pair<1, 1>(e, kInvK, o, kK);
+		h2 low, high;
+		line.push(v0, v1, low, high);
+		const int oy = (y0 >> 1) + n - kApron;
+		if (n >= kApron && store_x && oy < J.bh)
+		{
+			const uint32_t idx = __umul24((uint32_t)oy, (uint32_t)J.band_pitch) + (uint32_t)ox;
+			store_half(J.band[0], idx, low.x);
+			store_half(J.band[1], idx, low.y);
+			store_half(J.band[2], idx, high.x);
+			store_half(J.band[3], idx, high.y);
+		}
+	};
+	uint32_t a0, a1, b0, b1, ra[4], rb[4];
+	rows(0, a0, a1);
+	rows(1, b0, b1);
+	load_pair(a0, a1, ra);
+	load_pair(b0, b1, rb);
+#pragma unroll 1
+	for (int n = 0; n < kPairs; n += 2)
+	{
+		const uint32_t pa0 = a0, pa1 = a1, pb0 = b0, pb1 = b1;
+		uint32_t ca[4] = {ra[0], ra[1], ra[2], ra[3]}, cb[4] = {rb[0], rb[1], rb[2], rb[3]};
+		if (n + 2 < kPairs)
+		{
+			rows(n + 2, a0, a1);
+			rows(n + 3, b0, b1);
+			load_pair(a0, a1, ra);
+			load_pair(b0, b1, rb);
+		}
+		__builtin_amdgcn_sched_barrier(0);
+		lift_pair(n, pa0, pa1, ca);
+		lift_pair(n + 1, pb0, pb1, cb);
+	}
+}
+
+__global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream_h(const DwtJob *__restrict__ jobs,
+                                                                            const uint16_t *__restrict__ tile_job,
+                                                                            const FrameState *__restrict__ frames,
+                                                                            int tiles, int nxcd)
+{
+	constexpr int TW = kFwdVariantTileW[kHalfStreamVariant], TH = kFwdVariantTileH[kHalfStreamVariant];
+	const int lane = threadIdx.x & 63;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	if (tile >= tiles)
+		return; // whole waves: no wave-level or workgroup barrier follows
+	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
+	const int local = tile - J.tile_base;
+	const int x0 = (local % J.tiles_x) * TW;
+	const int y0 = (local / J.tiles_x) * TH;
+	if (J.src_kind == kStoreU8)
+	{
+		const FrameState &F = frames[J.frame];
+		forward_stream_tile_h<kStoreU8>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
+		                                (uint32_t)F.src_step[J.component], x0, y0, lane);
+	}
+	else
+		forward_stream_tile_h<kStoreF16>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+}
+
+// VLineInv on the lane's two columns at once (x: even column, y: odd).
+struct VLineInvH
+{
+	h2 ro = {}, e1 = {}, o2 = {}, e3 = {};
+
+	// Takes pair n (e: even row, o: odd row, scaled); gives pair n-2's rows.
+	__device__ __forceinline__ void push(h2 e, h2 o, h2 &even, h2 &odd)
+	{
+		const h2 ne1 = fma2(splat(-kDeltaH), ro + o, e);
+		const h2 no2 = fma2(splat(-kGammaH), e1 + ne1, ro);
+		const h2 ne3 = fma2(splat(-kBetaH), o2 + no2, e1);
+		odd = fma2(splat(-kAlphaH), e3 + ne3, o2);
+		even = e3;
+		ro = o;
+		e1 = ne1;
+		o2 = no2;
+		e3 = ne3;
+	}
+};
+
+// Horizontal inverse lifting of a row pair (e: even column of row 0, of row
+// 1; o likewise), after the scaling.
+__device__ __forceinline__ void hlift_inv_rows_h(h2 &e, h2 &o)
+{
+	e = fma2(splat(-kDeltaH), dpp_wave_h2<kDppWaveShr1>(o) + o, e);
+	o = fma2(splat(-kGammaH), e + dpp_wave_h2<kDppWaveShl1>(e), o);
+	e = fma2(splat(-kBetaH), dpp_wave_h2<kDppWaveShr1>(o) + o, e);
+	o = fma2(splat(-kAlphaH), e + dpp_wave_h2<kDppWaveShl1>(e), o);
+}
+
+__device__ __forceinline__ void inverse_stream_tile_h(const DwtJob &J, const FrameState *frames, int ox0, int oy0,
+                                                      int lane)
+{
+	constexpr int TH = kInvVariantTileH[kHalfStreamVariant];
+	constexpr int kPairs = TH / 2 + kApron;
+	static_assert(kPairs % 2 == 0, "the row-pair loop is unrolled by two");
+	const uint32_t pitch = (uint32_t)J.band_pitch;
+	const int xe = inv_ext(ox0 - kApron + 2 * lane, J.A_x), xo = inv_ext(ox0 - kApron + 2 * lane + 1, J.A_x);
+	const uint32_t ce = (uint32_t)(xe >> 1), co = (uint32_t)(xo >> 1);
+	const int px = ox0 - kApron + 2 * lane;
+	const bool lane_out = lane >= kApron / 2 && lane < 64 - kApron / 2;
+
+	uint8_t *plane = nullptr;
+	long dpitch = J.dst_pitch;
+	if (J.dst_kind == kStoreU8)
+	{
+		plane = frames[J.frame].dst[J.component];
+		dpitch = frames[J.frame].dst_pitch[J.component];
+	}
+	static_assert(2 * kPairs <= 64, "one window row per lane");
+	const uint32_t row_off =
+	    lane < 2 * kPairs ? __umul24((uint32_t)(inv_ext(oy0 - kApron + lane, J.A_y) >> 1), pitch) : 0u;
+	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
+		r0 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n);
+		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
+	};
+	const StreamSrc ll = stream_src(J.band[0]), hl = stream_src(J.band[1]), lh = stream_src(J.band[2]),
+	                hh = stream_src(J.band[3]);
+	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
+		r[0] = stream_raw<kStoreF16>(ll, r0 + ce);
+		r[1] = stream_raw<kStoreF16>(hl, r0 + co);
+		r[2] = stream_raw<kStoreF16>(lh, r1 + ce);
+		r[3] = stream_raw<kStoreF16>(hh, r1 + co);
+	};
+	auto store_row = [&](int py, _Float16 e, _Float16 o) {
+		if (!lane_out)
+			return;
+		if (J.dst_kind == kStoreU8)
+		{
+			if (py < J.R_y)
+			{
+				uint8_t *d = plane + (long)py * dpitch + px;
+				if (px < J.R_x)
+					d[0] = unorm8_store((float)e);
+				if (px + 1 < J.R_x)
+					d[1] = unorm8_store((float)o);
+			}
+		}
+		else if (py < J.A_y)
+		{
+			const uint32_t idx = __umul24((uint32_t)py, (uint32_t)dpitch) + (uint32_t)px;
+			if (px < J.A_x)
+				store_half(J.dst, idx, e);
+			if (px + 1 < J.A_x)
+				store_half(J.dst, idx + 1, o);
+		}
+	};
+	VLineInvH line;
+	auto lift_pair = [&](int n, uint32_t r0, uint32_t r1, const uint32_t (&r)[4]) {
+		// Scaled for the vertical pass: even rows by K, odd rows by 1/K.
+		const h2 re = h2{stream_half<kStoreF16>(r[0], ll, r0 + ce), stream_half<kStoreF16>(r[1], hl, r0 + co)};
+		const h2 ro = h2{stream_half<kStoreF16>(r[2], lh, r1 + ce), stream_half<kStoreF16>(r[3], hh, r1 + co)};
+		const h2 e = scale_pair<0, 1>(re, kK, re, kK);
+		const h2 o = scale_pair<0, 1>(ro, kInvK, ro, kInvK);
+		h2 even, odd; // rows 2p, 2p+1 of the finished pair p = n - 2: (even column, odd column)
+		line.push(e, o, even, odd);
+		if (n < kApron)
+			return;
+		h2 he = scale_pair<0, 0>(even, kK, odd, kK);
+		h2 ho = scale_pair<1, 1>(even, kInvK, odd, kInvK);
+		hlift_inv_rows_h(he, ho);
+		const int py = oy0 - kApron + 2 * (n - 2);
+		store_row(py, he.x, ho.x);
+		store_row(py + 1, he.y, ho.y);
+	};
+	uint32_t a0, a1, b0, b1, ra[4], rb[4];
+	rows(0, a0, a1);
+	rows(1, b0, b1);
+	load_pair(a0, a1, ra);
+	load_pair(b0, b1, rb);
+#pragma unroll 1
+	for (int n = 0; n < kPairs; n += 2)
+	{
+		const uint32_t pa0 = a0, pa1 = a1, pb0 = b0, pb1 = b1;
+		uint32_t ca[4] = {ra[0], ra[1], ra[2], ra[3]}, cb[4] = {rb[0], rb[1], rb[2], rb[3]};
+		if (n + 2 < kPairs)
+		{
+			rows(n + 2, a0, a1);
+			rows(n + 3, b0, b1);
+			load_pair(a0, a1, ra);
+			load_pair(b0, b1, rb);
+		}
+		__builtin_amdgcn_sched_barrier(0);
+		lift_pair(n, pa0, pa1, ca);
+		lift_pair(n + 1, pb0, pb1, cb);
+	}
+}
+
+__global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream_h(const DwtJob *__restrict__ jobs,
+                                                                            const uint16_t *__restrict__ tile_job,
+                                                                            const FrameState *__restrict__ frames,
+                                                                            int tiles, int nxcd)
+{
+	constexpr int TW = kInvVariantTileW[kHalfStreamVariant], TH = kInvVariantTileH[kHalfStreamVariant];
+	static_assert(TW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
+	const int lane = threadIdx.x & 63;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	if (tile >= tiles)
+		return; // whole waves: no wave-level or workgroup barrier follows
+	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
+	const int local = tile - J.tile_base;
+	const int ox0 = (local % J.tiles_x) * TW;
+	const int oy0 = (local / J.tiles_x) * TH;
+	inverse_stream_tile_h(J, frames, ox0, oy0, lane);
+}
 } // namespace
 
 void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -1082,7 +1438,10 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	if (tiles <= 0)
 		return;
-	if (variant == kFwdStreamVariant)
+	if (variant == kHalfStreamVariant)
+		hipLaunchKernelGGL(k_dwt_forward_stream_h, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
+		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+	else if (variant == kFwdStreamVariant)
 		hipLaunchKernelGGL(k_dwt_forward_stream, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
 		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
 	else if (variant == 1)
@@ -1104,7 +1463,10 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	if (tiles <= 0)
 		return;
-	if (variant == kInvStreamVariant)
+	if (variant == kHalfStreamVariant)
+		hipLaunchKernelGGL(k_dwt_inverse_stream_h, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
+		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+	else if (variant == kInvStreamVariant)
 		hipLaunchKernelGGL(k_dwt_inverse_stream, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
 		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
 	else if (variant == 1)
@@ -1143,5 +1505,16 @@ void upload_constants()
 	for (int u = 0; u < 256; u++)
 		lut[u] = (float)u / 255.0f - 0.5f;
 	(void)hipMemcpyToSymbol(HIP_SYMBOL(c_u8_dc), lut, sizeof(lut));
+	// fp16(fp16(u / 255) - 0.5): the fp16 subtraction rounds the exact
+	// difference once (a float holds it exactly, so the float difference
+	// rounded to fp16 is the fp16 result).
+	uint16_t lut_h[256];
+	for (int u = 0; u < 256; u++)
+	{
+		const _Float16 t = (_Float16)((float)u / 255.0f);
+		const _Float16 d = (_Float16)((float)t - 0.5f);
+		lut_h[u] = __builtin_bit_cast(uint16_t, d);
+	}
+	(void)hipMemcpyToSymbol(HIP_SYMBOL(c_u8_dc_h), lut_h, sizeof(lut_h));
 }
 } // namespace pw

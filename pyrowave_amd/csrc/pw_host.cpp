@@ -37,10 +37,10 @@ int floor_log2(int v)
 // pyrowave_common.hpp:124-145).
 // ---------------------------------------------------------------------------
 
-static float noise_power_normalized_quant_resolution(int level, int component, int band)
+static float noise_power_normalized_quant_resolution(int level, int component, int band, bool precision0)
 {
-	// Precision >= 1 policy: 8 base bits.
-	int bits = 8;
+	// 8 base bits for precision >= 1, 6 for precision 0.
+	int bits = precision0 ? 6 : 8;
 	if (band == 0)
 		bits += 2;
 	else if (band < 3)
@@ -51,12 +51,14 @@ static float noise_power_normalized_quant_resolution(int level, int component, i
 	return float(1 << bits);
 }
 
-float quant_resolution(int level, int component, int band)
+float quant_resolution(int level, int component, int band, bool precision0)
 {
-	return std::min<float>(4096.0f, noise_power_normalized_quant_resolution(level, component, band));
+	// "FP16 range is limited": cap 512 for precision 0, 4096 otherwise.
+	return std::min<float>(precision0 ? 512.0f : 4096.0f,
+	                       noise_power_normalized_quant_resolution(level, component, band, precision0));
 }
 
-float rdo_distortion_scale(int chroma, int level, int component, int band)
+float rdo_distortion_scale(int chroma, int level, int component, int band, bool precision0)
 {
 	float horiz_midpoint = (band & 1) ? 0.75f : 0.25f;
 	float vert_midpoint = (band & 2) ? 0.75f : 0.25f;
@@ -69,7 +71,7 @@ float rdo_distortion_scale(int chroma, int level, int component, int band)
 	float csf = 2.6f * (0.0192f + 0.114f * cpd) * std::exp(-std::pow(0.114f * cpd, 1.1f));
 	if (component != 0 && level != kLevels - 1 && chroma == PYROWAVE_CHROMA_SUBSAMPLING_420)
 		csf *= 0.6f;
-	float resolution = noise_power_normalized_quant_resolution(level, component, band);
+	float resolution = noise_power_normalized_quant_resolution(level, component, band, precision0);
 	float weighted = csf * resolution;
 	return (weighted * weighted) * (1.0f / 256.0f);
 }
@@ -130,8 +132,10 @@ bool Geometry::init(int w, int h, int chroma_, int precision_)
 				// fp16 and the LL intermediates stay FP32 (fp16 LL storage costs
 				// 0.11 dB PSNR at high rates, the high bands ~0); FP16_LL: all
 				// four bands of levels 0-1 as fp16, as the reference stores them.
-				const bool f16 = l < 2 && ((precision == PYROWAVE_HIP_PRECISION_FP16 && b != 0) ||
-				                           precision == PYROWAVE_HIP_PRECISION_FP16_LL);
+				// FP16_MATH: every band of every level as fp16.
+				const bool f16 = (l < 2 && ((precision == PYROWAVE_HIP_PRECISION_FP16 && b != 0) ||
+				                            precision == PYROWAVE_HIP_PRECISION_FP16_LL)) ||
+				                 precision == PYROWAVE_HIP_PRECISION_FP16_MATH;
 				B.kind = f16 ? kStoreF16 : kStoreF32;
 				B.pitch = B.bx32 * 32;
 				size_t bytes = size_t(B.pitch) * size_t(B.by32 * 32) * (B.kind == kStoreF16 ? 2 : 4);
@@ -509,6 +513,12 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	std::vector<int32_t> frame_block_base(n), frame_group_base(n);
 	std::vector<uint32_t> grp_frame;
 	max_n32 = 0;
+	// FP16-arithmetic frames have their own transform kernels (one shape);
+	// a batch holds either those or the others.
+	half_math = slots[0]->geom.precision == PYROWAVE_HIP_PRECISION_FP16_MATH;
+	for (Slot *S : slots)
+		if ((S->geom.precision == PYROWAVE_HIP_PRECISION_FP16_MATH) != half_math)
+			return false;
 
 	for (int f = 0; f < n; f++)
 	{
@@ -578,8 +588,10 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 						J.R_y = J.A_y;
 					}
 				}
-				for (int v = 0; v < (encoder ? kFwdVariants : kInvVariants); v++)
+				for (int v = 0; v < kDwtVariants; v++)
 				{
+					if ((v == kHalfStreamVariant) != half_math)
+						continue;
 					const int tw = encoder ? kFwdVariantTileW[v] : kInvVariantTileW[v];
 					const int th = encoder ? kFwdVariantTileH[v] : kInvVariantTileH[v];
 					DwtJob Jv = J;
@@ -613,9 +625,10 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 					J.by32 = B.by32;
 					J.off32 = B.off32;
 					J.frame = f;
-					float res = quant_resolution(l, c, b);
+					const bool p0 = G.precision == PYROWAVE_HIP_PRECISION_FP16_MATH;
+					float res = quant_resolution(l, c, b, p0);
 					J.qres = 1.0f / decode_quant(encode_quant(1.0f / res));
-					J.rdo_scale = rdo_distortion_scale(G.chroma, l, c, b);
+					J.rdo_scale = rdo_distortion_scale(G.chroma, l, c, b, p0);
 					J.qcode = encode_quant(1.0f / res);
 					uint32_t idx = uint32_t(band_jobs.size());
 					band_jobs.push_back(J);
@@ -640,6 +653,11 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 			tiles[l][v] = int(tile_map[l][v].size());
 			if (jobs[l][v].size() > 65535)
 				return false;
+		}
+		if (half_math)
+		{
+			variant[l] = kHalfStreamVariant;
+			continue;
 		}
 		variant[l] = default_dwt_variant(tiles[l][kFwdStreamVariant]);
 		auto it = D.tuned_variant.find(tune_key(encoder, l, tiles[l][0]));
@@ -838,7 +856,7 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 		auto launch = [&](int v) {
 			launch_dwt_forward(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
 		};
-		if (reps > 0)
+		if (reps > 0 && !P.half_math)
 			tune_level(P, true, l, reps, stream, launch);
 		else
 			launch(level_variant(P, l));
@@ -882,7 +900,7 @@ void run_decode(Plan &P, hipStream_t stream, bool timed)
 		auto launch = [&](int v) {
 			launch_dwt_inverse(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
 		};
-		if (reps > 0)
+		if (reps > 0 && !P.half_math)
 			tune_level(P, false, l, reps, stream, launch);
 		else
 			launch(level_variant(P, l));

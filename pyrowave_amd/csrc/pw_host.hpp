@@ -17,7 +17,7 @@ namespace pw
 inline bool valid_precision(int precision)
 {
 	return precision == PYROWAVE_HIP_PRECISION_FP16 || precision == PYROWAVE_HIP_PRECISION_FP32 ||
-	       precision == PYROWAVE_HIP_PRECISION_FP16_LL;
+	       precision == PYROWAVE_HIP_PRECISION_FP16_LL || precision == PYROWAVE_HIP_PRECISION_FP16_MATH;
 }
 
 enum Stage
@@ -198,6 +198,7 @@ struct Plan
 {
 	std::vector<Slot *> slots;
 	bool encoder = true;
+	bool half_math = false; // FP16-arithmetic frames (transform variant kHalfStreamVariant only)
 	DeviceBuffer mem;
 	FrameState *d_frames = nullptr;
 
@@ -235,9 +236,10 @@ struct Plan
 	std::vector<CallParams> params;
 };
 
-// Shared helpers.
-float quant_resolution(int level, int component, int band);
-float rdo_distortion_scale(int chroma, int level, int component, int band);
+// Shared helpers.  precision0: the quantizer policy of the reference's
+// precision 0 (PYROWAVE_HIP_PRECISION_FP16_MATH).
+float quant_resolution(int level, int component, int band, bool precision0);
+float rdo_distortion_scale(int chroma, int level, int component, int band, bool precision0);
 uint8_t encode_quant(float decoder_q_scale);
 float decode_quant(uint8_t code);
 

@@ -28,8 +28,8 @@ def gpu_encode(enc, planes, budget, boundary=1 << 30):
     return out[:total].copy(), pk
 
 
-def oracle_encode(L, planes, budget, sequence=1, q1=True, boundary=1 << 30):
-    bs, meta, quant = O.encode(L, planes, budget, sequence=sequence, q1=q1)
+def oracle_encode(L, planes, budget, sequence=1, q1=True, boundary=1 << 30, precision=2):
+    bs, meta, quant = O.encode(L, planes, budget, sequence=sequence, q1=q1, precision=precision)
     out, pk = O.packetize(L, meta, bs, sequence, boundary)
     return out, pk, quant
 

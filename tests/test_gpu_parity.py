@@ -50,16 +50,17 @@ def test_dwt_fp32_bit_exact(device, w, h, chroma):
 
 
 # Oracle precision of each storage mode.
-ORACLE_PRECISION = {pw.PRECISION_FP32: 2, pw.PRECISION_FP16: 1, pw.PRECISION_FP16_LL: 3}
+ORACLE_PRECISION = {pw.PRECISION_FP32: 2, pw.PRECISION_FP16: 1, pw.PRECISION_FP16_LL: 3, pw.PRECISION_FP16_MATH: 0}
 
 
-@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP16_LL, pw.PRECISION_FP16_MATH])
 @pytest.mark.parametrize("w,h,chroma", [(34, 30, pw.CHROMA_420), (200, 130, pw.CHROMA_444),
                                         (1920, 1080, pw.CHROMA_420)])
 def test_dwt_fp16_storage_bit_exact(device, w, h, chroma, precision):
     """Every band of every level, the LL intermediates included, equals the
     oracle in the same storage mode (FP16: high bands of levels 0-1 in fp16;
-    FP16_LL: all four bands of levels 0-1, so levels 1-2 read an fp16 LL)."""
+    FP16_LL: all four bands of levels 0-1, so levels 1-2 read an fp16 LL;
+    FP16_MATH: packed-FP16 lifting and fp16 bands at every level)."""
     planes = _content(w, h, chroma)
     L = O.layout(w, h, chroma)
     enc = pw.Encoder(device, w, h, chroma, precision=precision)
@@ -74,11 +75,15 @@ def test_dwt_fp16_storage_bit_exact(device, w, h, chroma, precision):
 
 
 @pytest.mark.parametrize("precision,max_abs,rms", [(pw.PRECISION_FP32, 2e-5, 2e-6),
-                                                   (pw.PRECISION_FP16, 4e-3, 6e-4)])
+                                                   (pw.PRECISION_FP16, 4e-3, 6e-4),
+                                                   (pw.PRECISION_FP16_LL, 1e-3, 1.5e-4),
+                                                   (pw.PRECISION_FP16_MATH, 5e-3, 8e-4)])
 def test_dwt_vs_fp64(device, precision, max_abs, rms):
     """Per-subband error against the FP64 lifting oracle.  Tolerances:
     FP32 max-abs 2e-5, RMS 2e-6; FP16 storage max-abs 4e-3, RMS 6e-4
-    (coefficients are O(1); fp16 has an 11-bit significand)."""
+    (coefficients are O(1); fp16 has an 11-bit significand); FP16_LL storage
+    1e-3 / 1.5e-4; packed-FP16 lifting 5e-3 / 8e-4 (every step rounds to
+    fp16: measured 3.1e-3 / 5.2e-4 at 1080p)."""
     w, h = 1920, 1080
     planes = synth.frame(w, h, True, seed=3)
     L = O.layout(w, h, 0)
@@ -116,6 +121,43 @@ def test_encode_bit_exact_fp32(device, w, h, chroma, budget):
     assert got.size <= budget
     assert got.size == ref.size and np.array_equal(got, ref)
     assert np.array_equal(rc["quant"], quant)
+
+
+@pytest.mark.parametrize("w,h,chroma,budget", [
+    (34, 30, pw.CHROMA_420, 64 * 1024),
+    (200, 130, pw.CHROMA_444, 9000),
+    (1920, 1080, pw.CHROMA_420, 400000),
+])
+def test_encode_bit_exact_fp16_math(device, w, h, chroma, budget):
+    """The packed-FP16 encoder (reference precision 0, with its quantizer
+    policy) gives the oracle's precision-0 bitstream byte for byte."""
+    planes = _content(w, h, chroma, seed=37)
+    L = O.layout(w, h, chroma, precision=0)
+    enc = pw.Encoder(device, w, h, chroma, precision=pw.PRECISION_FP16_MATH)
+    try:
+        got, _ = gpu_encode(enc, planes, budget)
+    finally:
+        enc.close()
+    ref, _, _ = oracle_encode(L, planes, budget, sequence=1, precision=0)
+    assert got.size <= budget
+    assert np.array_equal(got, ref)
+
+
+def test_fp16_math_batch_cannot_mix_precisions(device):
+    """One batch holds packed-FP16 frames or the others, not both."""
+    import torch
+    w, h = 256, 144
+    planes = _content(w, h, pw.CHROMA_420)
+    encs = [pw.Encoder(device, w, h, pw.CHROMA_420, precision=p)
+            for p in (pw.PRECISION_FP16_MATH, pw.PRECISION_FP32)]
+    try:
+        ts = [torch.from_numpy(p).cuda() for p in planes]
+        frame = pw.hip_frame([(t.data_ptr(), t.stride(0), 1) for t in ts])
+        with pytest.raises(pw.PyroWaveError):
+            pw.encode_batch(encs, [frame, frame], [20000, 20000])
+    finally:
+        for e in encs:
+            e.close()
 
 
 def test_rate_control_state_matches_oracle(device):
@@ -160,7 +202,8 @@ def test_fp16_path_integer_stages_bit_exact(device):
     assert np.array_equal(got, ref)
 
 
-@pytest.mark.parametrize("precision", [pw.PRECISION_FP32, pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP32, pw.PRECISION_FP16, pw.PRECISION_FP16_LL,
+                                       pw.PRECISION_FP16_MATH])
 @pytest.mark.parametrize("w,h,chroma,budget", [
     (34, 30, pw.CHROMA_420, 64 * 1024),
     (34, 30, pw.CHROMA_444, 64 * 1024),
@@ -170,7 +213,8 @@ def test_decode_bit_exact(device, w, h, chroma, budget, precision):
     """Dequantized coefficients and decoded planes equal the oracle's.  The
     FP16 decoder keeps the high bands of levels 0-1 in fp16, which the oracle's
     precision-1 decode reproduces; the FP16_LL decoder also the LL that the
-    inverse of levels 1-2 writes (oracle precision 3)."""
+    inverse of levels 1-2 writes (oracle precision 3); the FP16_MATH decoder
+    every band in fp16 and the inverse in packed-FP16 lifting (precision 0)."""
     planes = _content(w, h, chroma, seed=13)
     L = O.layout(w, h, chroma)
     ref_bytes, pk, _ = oracle_encode(L, planes, budget)
@@ -192,7 +236,7 @@ def test_decode_bit_exact(device, w, h, chroma, budget, precision):
                 if not B.present:
                     continue
                 r = O.band_view(L, rcoef, c, l, b)
-                if oprec != 2 and l < 2 and b != 0:
+                if (oprec != 2 and l < 2 and b != 0) or oprec == 0:
                     r = r.astype(np.float16).astype(np.float32)
                 assert np.array_equal(O.band_view(L, gcoef, c, l, b).view(np.uint32), r.view(np.uint32)), (c, l, b)
     ref = O.idwt_frame(L, rcoef, precision=oprec)

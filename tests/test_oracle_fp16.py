@@ -51,7 +51,8 @@ def _lift_inv_np(x, N):
 
 
 def _scale(v, k):
-    return f16((v.astype(np.float32) * k).astype(np.float32))
+    """fp16 value times the FP32 constant, the exact product rounded once."""
+    return f16(np.asarray(v, np.float64) * np.float64(k))
 
 
 def _dwt_level_np(img, R_x, R_y, A_x, A_y):
