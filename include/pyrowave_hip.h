@@ -233,10 +233,15 @@ pyrowave_hip_encoder_read_packets(pyrowave_encoder encoder, void *data, size_t s
 /* Parses each decoder's packets on the device and decodes the frame into the
  * target.  Equivalent to pyrowave_decoder_clear then
  * pyrowave_decoder_push_packet of every packet in order, for packets that all
- * carry one sequence number; a packet that push_packet would reject, a second
- * sequence number, or a block shorter than 8 bytes (even a duplicate) is a
- * parse error, reported by pyrowave_hip_decoder_parse_result.  Missing blocks
- * decode as zero.  Asynchronous. */
+ * carry one sequence number (one frame's packets: the reference's restart on
+ * a newer sequence and its dropping of stale packets are not done here; a
+ * second sequence number is a parse error).  A packet that push_packet would
+ * reject is a parse error too, and its units before the rejected one count,
+ * as in the reference: a block shorter than its 8-byte header is present
+ * (first copy in push order wins) and ends its packet.  Errors are reported
+ * by pyrowave_hip_decoder_parse_result; the frame still decodes from the
+ * blocks accepted, as a reference decoder would after a rejected push.
+ * Missing blocks decode as zero.  Asynchronous. */
 PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_decode_packets_batch(pyrowave_decoder *decoders, const pyrowave_hip_packet_stream *packets,
                                   const pyrowave_hip_decode_target *targets, uint32_t count, void *hip_stream);

@@ -144,12 +144,20 @@ __global__ __launch_bounds__(kPkThreads) void k_parse(ParseBlock blk)
 			}
 			const uint32_t psize = ((w0 >> 16) & 0xfffu) * 4u;
 			const uint32_t block = w1 >> 8;
-			if (psize < 8u || psize > size - u || block >= J.n32)
+			if (psize > size - u || block >= J.n32)
 			{
 				bad = kParseBadBlock;
 				break;
 			}
+			// Counted and placed before the length check, as in
+			// pyrowave_decoder.cpp:126-150: a block shorter than its header is
+			// present (its first copy wins) and ends the packet with an error.
 			atomicMin(&J.offsets[block], (off + u) >> 2);
+			if (psize < 8u)
+			{
+				bad = kParseBadBlock;
+				break;
+			}
 			u += psize;
 		}
 		if (!bad && u != size)
