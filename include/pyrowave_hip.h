@@ -246,6 +246,24 @@ PYROWAVE_PUBLIC_API pyrowave_result
 pyrowave_hip_decode_packets_batch(pyrowave_decoder *decoders, const pyrowave_hip_packet_stream *packets,
                                   const pyrowave_hip_decode_target *targets, uint32_t count, void *hip_stream);
 
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (a serving loop that must not call hipMalloc, a harness that forbids
+ * it).  pyrowave_hip_decode_packets_batch for one decoder that allocates
+ * nothing: the parse's block offset table lives in a static device array of
+ * the library (131072 entries, 512 KiB per GPU, held until the process ends),
+ * so frames of more than 131072 32x32 blocks (beyond 8192x8192 4:4:4) are
+ * refused with PYROWAVE_ERROR_INVALID_ARGUMENT.  The parse's info words
+ * {blocks present, blocks announced, error bits, sequence} go to
+ * device_info[4] (device memory) when it is not null; the call does not
+ * update pyrowave_hip_decoder_parse_result.  Every call on a GPU uses that
+ * GPU's one array: calls that overlap on one GPU (other streams, other
+ * threads) race and must be ordered by the caller.  Never captured in a
+ * graph and not timed.  Asynchronous. */
+PYROWAVE_PUBLIC_API pyrowave_result
+pyrowave_hip_decode_packets_noalloc(pyrowave_decoder decoder, const pyrowave_hip_packet_stream *packets,
+                                    const pyrowave_hip_decode_target *target, uint32_t *device_info,
+                                    void *hip_stream);
+
 /* Waits for the decoder's last device parse.  Blocks present and blocks the
  * sequence header announced; PYROWAVE_ERROR_INVALID_ARGUMENT after a parse
  * error, PYROWAVE_ERROR_GENERIC if there was no device parse. */

@@ -157,6 +157,7 @@ EXPORTS = (
     "pyrowave_hip_decoder_read_coefficients", "pyrowave_hip_device_query_stage_times",
     "pyrowave_hip_device_set_timing", "pyrowave_hip_device_set_timing_stages",
     "pyrowave_hip_encoder_packetize_batch", "pyrowave_hip_encoder_get_packets", "pyrowave_hip_decode_packets_batch",
+    "pyrowave_hip_decode_packets_noalloc",
     "pyrowave_hip_decoder_parse_result", "pyrowave_hip_encoder_read_packets", "pyrowave_hip_device_set_graphs",
     "pyrowave_hip_psnr_hvs_m_accumulate", "pyrowave_hip_psnr_hvs_m_db", "pyrowave_hip_autotune_batch",
     "pyrowave_hip_device_force_tile_variant",
@@ -235,6 +236,7 @@ def load():
         "pyrowave_hip_encoder_packetize_batch": (I, [P, U32, SZ, P]),
         "pyrowave_hip_encoder_get_packets": (I, [P, C.POINTER(PacketStream), C.POINTER(P)]),
         "pyrowave_hip_decode_packets_batch": (I, [P, P, P, U32, P]),
+        "pyrowave_hip_decode_packets_noalloc": (I, [P, P, P, P, P]),
         "pyrowave_hip_decoder_parse_result": (I, [P, C.POINTER(U32), C.POINTER(U32)]),
         "pyrowave_hip_encoder_read_packets": (I, [P, P, SZ, C.POINTER(Packet), U32, C.POINTER(SZ),
                                                   C.POINTER(U32)]),
@@ -568,6 +570,12 @@ def decode_packets_batch(decoders, streams, targets, stream=None):
     pa = (PacketStream * n)(*streams)
     ta = (HipDecodeTarget * n)(*targets)
     _check(load().pyrowave_hip_decode_packets_batch(_handles(decoders), pa, ta, n, stream), "decode_packets_batch")
+
+
+def decode_packets_noalloc(decoder, packet_stream_desc, target, device_info=None, stream=None):
+    """pyrowave_hip_decode_packets_noalloc: returns the result code."""
+    return load().pyrowave_hip_decode_packets_noalloc(decoder.handle, C.byref(packet_stream_desc), C.byref(target),
+                                                       device_info, stream)
 
 
 def parse_result(decoder):

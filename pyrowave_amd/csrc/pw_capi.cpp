@@ -483,6 +483,9 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_create_device_by_compat(uint32_t vi
 	if (hipStreamCreateWithFlags(&d->dev.stream, hipStreamNonBlocking) != hipSuccess)
 		return PYROWAVE_ERROR_GENERIC;
 	upload_constants();
+	// The allocation-free decode's scratch address on this GPU (current).
+	if (!noalloc_scratch_init(d->dev.hip_device))
+		return PYROWAVE_ERROR_GENERIC;
 	if (const char *env = getenv("PYROWAVE_TIMING"))
 		d->dev.timing = strtol(env, nullptr, 0) != 0 ? (1u << pw::kStageCount) - 1 : 0u;
 	*device = d.release();
@@ -888,6 +891,10 @@ static pyrowave_result create_decoder(pyrowave_device device, int width, int hei
 	d->device = device;
 	if (!d->slot.init(&device->dev, width, height, int(chroma), precision, false))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	// The single-frame launch tables up front, so that decoding one frame
+	// (pyrowave_hip_decode_packets_noalloc in particular) allocates nothing.
+	if (!get_plan(&device->dev, {&d->slot}, false))
+		return PYROWAVE_ERROR_OUT_OF_DEVICE_MEMORY;
 	device->dev.live_objects++;
 	*decoder = d.release();
 	return PYROWAVE_SUCCESS;
@@ -1309,6 +1316,62 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_batch(pyrowave_d
 	}
 	else
 		run_decode(*Pl, stream);
+	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
+}
+
+PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_noalloc(pyrowave_decoder decoder,
+                                                                       const pyrowave_hip_packet_stream *packets,
+                                                                       const pyrowave_hip_decode_target *target,
+                                                                       uint32_t *device_info, void *hip_stream)
+{
+	// pyrowave_hip_decode_packets_batch for one decoder, with the parse's
+	// offset table in the device's static scratch instead of decoder memory
+	// allocated on first use; no allocation, no graph capture, no timing
+	// events.
+	if (!decoder || !packets || !target)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	Slot &S = decoder->slot;
+	const Geometry &G = S.geom;
+	for (int c = 0; c < 3; c++)
+		if (!target->planes[c].data || target->planes[c].row_stride_in_bytes < size_t(G.plane_w(c)))
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	const pyrowave_hip_packet_stream &P = *packets;
+	if (!P.data || (reinterpret_cast<uintptr_t>(P.data) & 3) || P.data_bytes > UINT32_MAX || !P.packets || !P.count)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	if (uint32_t(G.count32) > kNoAllocMaxBlocks)
+	{
+		if (getenv("PYROWAVE_DEBUG"))
+			fprintf(stderr, "pyrowave: %d blocks exceed the allocation-free decode's %u\n", G.count32,
+			        kNoAllocMaxBlocks);
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	}
+	Device &D = *S.dev;
+	std::lock_guard<std::recursive_mutex> g(D.lock);
+	(void)hipSetDevice(D.hip_device);
+	uint32_t *offsets = noalloc_scratch(D.hip_device);
+	Plan *Pl = S.single_plan.get();
+	if (!offsets || !Pl)
+		return PYROWAVE_ERROR_GENERIC;
+	hipStream_t stream = pick_stream(D, hip_stream);
+	ParseJob J = {};
+	J.data = static_cast<const uint32_t *>(P.data);
+	J.packets = P.packets;
+	J.count = P.count;
+	J.offsets = offsets;
+	J.info = device_info ? device_info : offsets + kNoAllocMaxBlocks;
+	J.data_bytes = uint32_t(P.data_bytes);
+	J.max_packets = P.max_packets;
+	J.n32 = uint32_t(G.count32);
+	J.h0_size = uint32_t(G.width - 1) | (uint32_t(G.height - 1) << 14);
+	J.chroma = G.chroma == PYROWAVE_CHROMA_SUBSAMPLING_444 ? 1u : 0u;
+	launch_parse(&J, 1, stream);
+	CallParams &F = Pl->call(0);
+	fill_decode_call(F, S, nullptr, *target);
+	F.dec_offsets = offsets;
+	F.dec_offset_stride = 1;
+	F.dec_payload = J.data;
+	F.dec_payload_bytes = J.data_bytes;
+	run_decode(*Pl, stream, false);
 	return hipGetLastError() == hipSuccess ? PYROWAVE_SUCCESS : PYROWAVE_ERROR_GENERIC;
 }
 

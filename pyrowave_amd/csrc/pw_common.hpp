@@ -299,6 +299,13 @@ void launch_set_params(FrameState *frames, const CallParams *params, int nframes
 const void *set_params_kernel();
 void launch_packetize(const PacketizeJob *jobs, int count, hipStream_t stream);
 void launch_parse(const ParseJob *jobs, int count, hipStream_t stream);
+// The allocation-free decode's parse scratch: kNoAllocMaxBlocks offsets and
+// four info words per GPU.  noalloc_scratch_init (with hip_device current)
+// looks up its device address once; noalloc_scratch returns it (null before).
+constexpr uint32_t kNoAllocMaxBlocks = 1u << 17;
+constexpr int kMaxDevices = 64;
+uint32_t *noalloc_scratch_init(int hip_device);
+uint32_t *noalloc_scratch(int hip_device);
 // Adds each height factor's fixed-point block errors to totals[kPsnrFactors].
 void launch_psnr_hvs_m(const uint8_t *a, uint32_t pitch_a, const uint8_t *b, uint32_t pitch_b, int w, int h,
                        const float *tables, uint64_t *totals, hipStream_t stream);

@@ -17,8 +17,49 @@
 // as in the reference.  Packets must all carry one sequence number.
 #include "pw_common.hpp"
 
+#include <mutex>
+
 namespace pw
 {
+// Parse scratch of the allocation-free decode
+// (pyrowave_hip_decode_packets_noalloc): one frame's block offset table and
+// four info words.  Each GPU holds its own copy until the process ends; its
+// device address comes from hipGetSymbolAddress (the array's name is a
+// host-side stand-in, not a device address).
+static __device__ __align__(256) uint32_t scratch[kNoAllocMaxBlocks + 4];
+
+namespace
+{
+std::mutex g_scratch_lock;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+} // namespace
+
+uint32_t *noalloc_scratch_init(int hip_device)
+{
+	if (hip_device < 0 || hip_device >= kMaxDevices)
+		return nullptr;
+	std::lock_guard<std::mutex> g(g_scratch_lock);
+	if (!g_scratch_ptr[hip_device])
+	{
+		int current = -1;
+		if (hipGetDevice(&current) != hipSuccess || current != hip_device)
+			return nullptr;
+		void *ptr = nullptr;
+		if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr)
+			return nullptr;
+		g_scratch_ptr[hip_device] = static_cast<uint32_t *>(ptr);
+	}
+	return g_scratch_ptr[hip_device];
+}
+
+uint32_t *noalloc_scratch(int hip_device)
+{
+	if (hip_device < 0 || hip_device >= kMaxDevices)
+		return nullptr;
+	std::lock_guard<std::mutex> g(g_scratch_lock);
+	return g_scratch_ptr[hip_device];
+}
+
 namespace
 {
 constexpr int kPkThreads = 256;
