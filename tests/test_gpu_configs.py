@@ -3,11 +3,13 @@
 - 1920x1080 4:2:0 at 400,000 B (FP32): bitstream and decode equal the oracle's,
   packets at 8 KiB decode in order, budget filled to 95-100 %.
 - 3840x2160 4:2:0 seeded sequence at 1,600,000 B: FP32 and FP16 bitstreams
-  equal the oracle run in the same mode; per-subband error of both transforms
-  against the FP64 oracle; FP16 within 0.02 dB PSNR of FP32 on every frame.
+  of eight frames (first, panned, redrawn, last) equal the oracle run in the
+  same mode; per-subband error of both transforms against the FP64 oracle;
+  FP16 within 0.02 dB PSNR of FP32 on every frame.
 - 2560x1440 4:4:4 with 10 % checker patterns, eight budgets from 150 KB to
-  2 MB (log-spaced), FP32 and FP16: bytes never exceed the budget, PSNR grows
-  with the budget, FP16 within 0.02 dB of FP32 at every budget.
+  2 MB (log-spaced), FP32 and FP16: every bitstream equals the oracle's in the
+  same mode, bytes never exceed the budget, PSNR grows with the budget, FP16
+  within 0.02 dB of FP32 at every budget.
 - A batch of 32 uneven frames (8 x 1280x720 4:2:0, 8 x 1920x1080 4:2:0,
   8 x 2560x1440 4:4:4, 8 x 3840x2160 4:2:0) at 1.6 bits/pixel, FP16, one
   launch per stage: every frame's bytes and decoded planes equal its
@@ -51,6 +53,45 @@ def test_config_1080p_fp32(device):
     finally:
         enc.close()
         dec.close()
+
+
+def _oracle_streams(jobs):
+    """Oracle encodes run concurrently (the C oracle releases the GIL):
+    jobs = [(L, planes, budget, sequence, precision, boundary)] -> bytes."""
+    import concurrent.futures as cf
+
+    def one(job):
+        L, planes, budget, seq, prec, boundary = job
+        bs, meta, _ = O.encode(L, planes, budget, sequence=seq, precision=prec)
+        out, opk = O.packetize(L, meta, bs, seq, boundary)
+        return _bytes(out, opk)
+
+    with cf.ThreadPoolExecutor(max_workers=min(16, len(jobs))) as ex:
+        return list(ex.map(one, jobs))
+
+
+SEQ_FRAMES = (0, 1, 2, 3, 10, 20, 30, 59)
+
+
+def test_config_4k_sequence_oracle_frames(device):
+    """Eight frames of the 60-frame sequence (the pan moves 3 px and 5 % of the
+    rectangles are redrawn per frame), FP32 and FP16: every bitstream equals
+    the oracle's in the same mode, with the encoder's sequence counter."""
+    w, h, budget = 3840, 2160, 1600000
+    gen = synth.SceneGenerator(w, h, True, seed=2024, frames=60, pan=3, redraw=0.05)
+    L = O.layout(w, h, 0)
+    frames = [gen.frame(t) for t in SEQ_FRAMES]
+    for prec, oracle_prec in ((pw.PRECISION_FP32, 2), (pw.PRECISION_FP16, 1)):
+        enc = pw.Encoder(device, w, h, pw.CHROMA_420, precision=prec)
+        try:
+            got = [gpu_encode(enc, planes, budget)[0] for planes in frames]
+        finally:
+            enc.close()
+        refs = _oracle_streams([(L, planes, budget, enc_sequence(i), oracle_prec, 1 << 30)
+                                for i, planes in enumerate(frames)])
+        for t, a, b in zip(SEQ_FRAMES, got, refs):
+            assert 0.95 * budget <= a.size <= budget, (prec, t, a.size)
+            assert np.array_equal(a, b), (prec, t)
 
 
 def test_config_4k_sequence(device):
@@ -118,16 +159,20 @@ def test_config_1440p_444_budget_sweep(device):
         enc = pw.Encoder(device, w, h, pw.CHROMA_444, precision=prec)
         dec = pw.Decoder(device, w, h, pw.CHROMA_444, precision=prec)
         try:
-            vals = []
+            vals, streams = [], []
             for i, budget in enumerate(BUDGETS_1440):
                 data, pk = gpu_encode(enc, planes, budget, boundary=8192)
                 assert 0 < data.size <= budget
-                if prec == pw.PRECISION_FP32 and i in (0, 7):
-                    ref, _, _ = oracle_encode(L, planes, budget, sequence=enc_sequence(i), boundary=8192)
-                    assert np.array_equal(data, ref)
+                streams.append(data)
                 rec = decode_bytes(dec, data, pk)
                 vals.append(np.mean([synth.psnr(planes[c], rec[c]) for c in range(3)]))
             curves[prec] = vals
+            # Every budget's bitstream equals the oracle's in the same mode.
+            oprec = 2 if prec == pw.PRECISION_FP32 else 1
+            refs = _oracle_streams([(L, planes, budget, enc_sequence(i), oprec, 8192)
+                                    for i, budget in enumerate(BUDGETS_1440)])
+            for budget, a, b in zip(BUDGETS_1440, streams, refs):
+                assert np.array_equal(a, b), (prec, budget)
         finally:
             enc.close()
             dec.close()
