@@ -47,7 +47,9 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--streams", type=int, default=2, help="batches in flight, one stream each")
-    ap.add_argument("--cpu-frames", type=int, default=24, help="frames of the oracle CPU baseline sample")
+    ap.add_argument("--cpu-frames", type=int, default=64, help="frames of the oracle CPU baseline sample")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: the host cores this process may use, at most 16)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16-ll", action="store_true",
                     help="skip the FP16_LL leg (profiled runs: its transform launches have the headline's grids)")
@@ -254,6 +256,55 @@ def transform_tiles(level, tw, th, batch):
 
 
 SERVING_SHAPES = ((1280, 720, 0), (1920, 1080, 0), (2560, 1440, 1), (3840, 2160, 0))
+
+
+def reference_cpu_probe():
+    """Whether the reference itself could run on these host cores: it needs a
+    Vulkan loader with Mesa's lavapipe ICD and a Granite checkout to build.
+    Returns None when all are there, else the reason it cannot."""
+    import ctypes.util
+    import glob
+    import shutil
+    missing = []
+    if not ctypes.util.find_library("vulkan"):
+        missing.append("no Vulkan loader (libvulkan)")
+    icds = []
+    for d in ("/usr/share/vulkan/icd.d", "/etc/vulkan/icd.d", "/usr/local/share/vulkan/icd.d"):
+        icds += glob.glob(os.path.join(d, "lvp_icd*.json"))
+    if not icds:
+        missing.append("no lavapipe ICD (lvp_icd*.json)")
+    if not shutil.which("vulkaninfo"):
+        missing.append("no vulkaninfo")
+    if not glob.glob("/**/Granite/vulkan", recursive=False) and not os.environ.get("GRANITE_DIR"):
+        missing.append("no Granite checkout (the reference build fetches it from the network)")
+    return "; ".join(missing) if missing else None
+
+
+def cpu_baseline(host_frames, W, H, budget, n, threads):
+    """The oracle's C restatement of the codec (not the reference) on the host
+    cores: encode + packetize + decode of n frames of the sequence, one frame
+    per thread at a time (frames are independent), wall time over all."""
+    import concurrent.futures as cf
+    from oracle import oracle as O
+    if threads <= 0:
+        threads = min(16, len(os.sched_getaffinity(0)))
+    L = O.layout(W, H, 0)
+
+    def one(i):
+        bs, meta, _ = O.encode(L, host_frames[i % len(host_frames)], budget, sequence=1)
+        out, pk = O.packetize(L, meta, bs, 1, 1 << 30)
+        O.decode(L, [out[o:o + s].tobytes() for o, s in pk])
+
+    t = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        list(ex.map(one, range(n)))
+    cpu_s = time.perf_counter() - t
+    probe = reference_cpu_probe()
+    return {"value": round(cpu_s / n * 1e6, 1), "unit": "us/frame", "cores": threads, "kind": "port",
+            "sample": f"{n} frames of the 4K 4:2:0 sequence at {budget} B, encode+packetize+decode by the "
+                      f"oracle's C restatement (not the reference), {threads} threads, one frame each at a time "
+                      f"({cpu_s:.1f} s wall)",
+            "reference_cpu": "reference CPU timing unavailable: " + probe if probe else "lavapipe present"}
 
 
 def serving_leg(pw, torch, device, dev_frames, seconds, sessions=64, seed=4):
@@ -644,18 +695,8 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        from oracle import oracle as O
-        L = O.layout(W, H, 0)
-        n = args.cpu_frames
-        t = time.perf_counter()
-        for i in range(n):
-            bs, meta, _ = O.encode(L, host_frames[i], BUDGET, sequence=1)
-            out, pk = O.packetize(L, meta, bs, 1, 1 << 30)
-            O.decode(L, [out[o:o + s].tobytes() for o, s in pk])
-        cpu_s = time.perf_counter() - t
-        cpu = {"value": round(cpu_s / n * 1e6, 1), "unit": "us/frame", "cores": 1, "kind": "port",
-               "sample": f"{n} frames of the 4K 4:2:0 sequence, encode+packetize+decode, oracle C restatement, "
-                         f"1 thread ({cpu_s:.1f} s)"}
+        cpu = cpu_baseline(host_frames, W, H, BUDGET, args.cpu_frames, args.cpu_threads)
+        print(f"[bench] cpu baseline done", file=sys.stderr, flush=True)
 
     stage_us = {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in fp16["stages"].items()}
     stage_us32 = {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in fp32["stages"].items()}
