@@ -4,9 +4,10 @@
 //                     per coded 32x32 block (reference shaders/wavelet_quant.comp
 //                     and shaders/analyze_rate_control.comp).  The per-8x8
 //                     statistics stay in LDS; only 48 B per block leave.
-//   k_rc_reset      : clears one frame's rate-control accumulators.
-//   k_rc_finalize   : savings scan + per-(bucket,subdivision) classification
-//                     (shaders/analyze_rate_control_finalize.comp).
+//                     The frame's last block to finish then runs the
+//                     finalize: savings scan + per-(bucket,subdivision)
+//                     classification (shaders/analyze_rate_control_finalize.comp).
+//   k_set_params    : per-call fields; clears the rate-control accumulators.
 //   k_rc_partial    : ordered greedy walk of the partially applied entries
 //                     (shaders/resolve_rate_control.comp).
 //   k_rc_prepare    : final truncation per block and the block-order word
@@ -329,6 +330,87 @@ __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const u
 	return C;
 }
 
+// The rate-control finalize of one frame, run by the frame's last quantized
+// block (128 threads, 16 entries each): the scan of the 2048 savings totals
+// and, per (bucket, subdivision) entry, the skip / full / partial class and
+// the savings still required there (reference
+// shaders/analyze_rate_control_finalize.comp:14-38 and the skip / required
+// logic of shaders/resolve_rate_control.comp:38-56).  The reference scans
+// with 512 invocations of 4 entries and stops its Hillis-Steele loop at step
+// 256, so entry x of group g = x / 4 holds the sum over groups g - 255 .. g
+// only: the exact prefix P minus P[4g - 1021] once g >= 256 (quirk Q1; the
+// exact-prefix flag keeps P).  Totals and the consumed words are read with
+// device-coherent loads: other blocks' atomics wrote them.
+__device__ __noinline__ void rc_finalize(const FrameState &F, int t)
+{
+	__shared__ uint32_t s_wave[2];
+	__shared__ uint32_t s_q1[kBlockThreads][4];
+	__shared__ uint32_t s_tail[kBlockThreads];
+	constexpr int kPer = kFlat / kBlockThreads; // 16
+	uint32_t v[kPer];
+#pragma unroll
+	for (int i = 0; i < kPer; i++)
+		v[i] = __hip_atomic_load(&F.totals[kPer * t + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	uint32_t tot[kPer];
+#pragma unroll
+	for (int i = 0; i < kPer; i++)
+		tot[i] = v[i];
+#pragma unroll
+	for (int i = 1; i < kPer; i++)
+		v[i] += v[i - 1];
+	// Exclusive scan of the threads' sums over the two waves.
+	const int lane = t & 63, wave = t >> 6;
+	const uint32_t incl = (uint32_t)wave_incl_scan((int)v[kPer - 1]);
+	if (lane == 63)
+		s_wave[wave] = incl;
+	__syncthreads();
+	const uint32_t base = incl - v[kPer - 1] + (wave ? s_wave[0] : 0u);
+#pragma unroll
+	for (int i = 0; i < kPer; i++)
+		v[i] += base; // exact inclusive prefix P[16 t + i]
+	// P at the last entry of each of this thread's four groups, for the
+	// windows of the groups 64 threads further on.
+#pragma unroll
+	for (int k = 0; k < 4; k++)
+		s_q1[t][k] = v[4 * k + 3];
+	__syncthreads();
+	if (!(F.flags & 1u) && t >= 64)
+	{
+#pragma unroll
+		for (int i = 0; i < kPer; i++)
+			v[i] -= s_q1[t - 64][i >> 2];
+	}
+	s_tail[t] = v[kPer - 1];
+	__syncthreads();
+	const uint32_t consumed = __hip_atomic_load(&F.rc[kRcConsumed], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+	const int required_base = (int)consumed - F.target_words;
+	uint32_t prev = t ? s_tail[t - 1] : 0u;
+#pragma unroll
+	for (int i = 0; i < kPer; i++)
+	{
+		const int x = kPer * t + i;
+		F.prefix[x] = v[i];
+		bool empty;
+		int required;
+		if (x == 0)
+		{
+			empty = v[0] == 0;
+			required = required_base;
+		}
+		else
+		{
+			empty = v[i] == prev;
+			required = required_base - (int)prev;
+		}
+		uint8_t cls = kRcSkip;
+		if (!empty && required > 0)
+			cls = (int)tot[i] <= required ? kRcFull : kRcPartial;
+		F.cls[x] = cls;
+		F.required[x] = required;
+		prev = v[i];
+	}
+}
+
 __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
                                                                  const uint32_t *__restrict__ blk_band,
                                                                  const FrameState *__restrict__ frames,
@@ -483,15 +565,49 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 			if (saving)
 				atomicAdd(&F.totals[Bq * kSubdiv + ((uint32_t)C.b >> F.shamt)], saving);
 		}
+		else
+			// Unquantized words into the frame's consumed total, one atomic per
+			// block (reference shaders/analyze_rate_control.comp:127-131).
+			atomicAdd(&F.rc[kRcConsumed], s_words[0]);
+		__threadfence(); // this block's sums are done before it counts itself
+	}
+	__syncthreads();
+	__shared__ uint32_t s_last;
+	if (tid == 0)
+		s_last = atomicAdd(&F.rc[kRcDone], 1u) == (uint32_t)F.n32 - 1u ? 1u : 0u;
+	__syncthreads();
+	if (s_last)
+	{
+		__threadfence(); // every other block of the frame has counted itself
+		rc_finalize(F, tid);
 	}
 }
 
+// Grid (clear blocks, frames of the parameter block).  Block (0, i) writes
+// frame i's per-call fields; every block clears its share of the frame's
+// rate-control accumulators (encoder frames only: a decoder's FrameState has
+// no rate-control arrays).  A kernel, not memsets: it is captured into graphs.
 __global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, int first, int count)
 {
-	const int i = threadIdx.x;
+	const int i = blockIdx.y;
 	if (i >= count)
 		return;
 	FrameState &F = frames[first + i];
+	if (F.totals)
+	{
+		const int n = F.n32 > kFlat ? F.n32 : kFlat;
+		for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+		{
+			if (k < kFlat)
+				F.totals[k] = 0;
+			if (k < F.n32)
+				F.partial_quant[k] = 0;
+			if (k < kRcCount)
+				F.rc[k] = 0;
+		}
+	}
+	if (blockIdx.x != 0 || threadIdx.x != 0)
+		return;
 	const CallParams &P = block.p[i];
 	for (int c = 0; c < 3; c++)
 	{
@@ -508,111 +624,6 @@ __global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, 
 	F.dec_payload = P.dec_payload;
 	F.dec_offset_stride = P.dec_offset_stride;
 	F.dec_payload_bytes = P.dec_payload_bytes;
-}
-
-__global__ void k_rc_reset(FrameState *__restrict__ frames)
-{
-	const FrameState &F = frames[blockIdx.y];
-	const int n = F.n32 > kFlat ? F.n32 : kFlat;
-	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-	{
-		if (i < kFlat)
-			F.totals[i] = 0;
-		if (i < F.n32)
-			F.partial_quant[i] = 0;
-		if (i < kRcCount)
-			F.rc[i] = 0;
-	}
-}
-
-// reference shaders/analyze_rate_control_finalize.comp:14-38 (512 threads x 4
-// entries) and the skip / required logic of shaders/resolve_rate_control.comp:38-56.
-__global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ frames)
-{
-	__shared__ uint32_t s_scan[512];
-	__shared__ uint32_t s_consumed;
-	const FrameState &F = frames[blockIdx.x];
-	const int t = threadIdx.x;
-	// Words of every block at quant level 0 (the reference accumulates them
-	// with one atomic per block, reference shaders/analyze_rate_control.comp:127-131).
-	{
-		// Eight independent loads per thread in flight at a time: one
-		// workgroup reads a whole frame's column (one 2-byte word every 32
-		// bytes), so the sum is bound by load latency, not bandwidth.
-		constexpr int kUnroll = 8;
-		uint32_t acc = 0;
-		int b = t;
-		for (; b + (kUnroll - 1) * 512 < F.n32; b += kUnroll * 512)
-		{
-			uint32_t w[kUnroll];
-#pragma unroll
-			for (int i = 0; i < kUnroll; i++)
-				w[i] = F.words[(long)(b + i * 512) * 16];
-#pragma unroll
-			for (int i = 0; i < kUnroll; i++)
-				acc += w[i];
-		}
-		for (; b < F.n32; b += 512)
-			acc += F.words[(long)b * 16];
-		for (int d = 32; d >= 1; d >>= 1)
-			acc += __shfl_xor(acc, d, 64);
-		if (t == 0)
-			s_consumed = 0;
-		__syncthreads();
-		if ((t & 63) == 0)
-			atomicAdd(&s_consumed, acc);
-		__syncthreads();
-		if (t == 0)
-			F.rc[kRcConsumed] = s_consumed;
-	}
-	uint32_t v[4];
-#pragma unroll
-	for (int i = 0; i < 4; i++)
-		v[i] = F.totals[4 * t + i];
-	v[1] += v[0];
-	v[2] += v[1];
-	v[3] += v[2];
-	s_scan[t] = v[3];
-	__syncthreads();
-	// The reference stops at step < 256; the exact variant runs one more step.
-	const uint32_t limit = (F.flags & 1u) ? 512u : 256u;
-	for (uint32_t step = 1; step < limit; step *= 2)
-	{
-		__syncthreads();
-		uint32_t up = t >= (int)step ? s_scan[t - step] : 0u;
-		__syncthreads();
-#pragma unroll
-		for (int i = 0; i < 4; i++)
-			v[i] += up;
-		s_scan[t] = v[3];
-	}
-	__syncthreads();
-	const int required_base = (int)s_consumed - F.target_words;
-	uint32_t prev = t ? s_scan[t - 1] : 0u;
-#pragma unroll
-	for (int i = 0; i < 4; i++)
-	{
-		const int x = 4 * t + i;
-		F.prefix[x] = v[i];
-		bool empty;
-		int required;
-		if (x == 0)
-		{
-			empty = v[0] == 0;
-			required = required_base;
-		}
-		else
-		{
-			empty = v[i] == prev;
-			required = required_base - (int)prev;
-		}
-		uint8_t cls = kRcSkip;
-		if (!empty && required > 0)
-			cls = (int)F.totals[x] <= required ? kRcFull : kRcPartial;
-		F.cls[x] = cls;
-		F.required[x] = required;
-		prev = v[i];
-	}
 }
 
 __device__ __forceinline__ int block_incl_scan256(int v, int *s_tmp, int &total)
@@ -1142,24 +1153,23 @@ const void *set_params_kernel()
 	return reinterpret_cast<const void *>(&k_set_params);
 }
 
-void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream)
+void launch_set_params(FrameState *frames, const CallParams *params, int nframes, int max_n32, hipStream_t stream)
 {
+	int blocks = 1;
+	if (max_n32 > 0)
+	{
+		const int n = max_n32 > kFlat ? max_n32 : kFlat;
+		blocks = (n + 255) / 256;
+		blocks = blocks > 64 ? 64 : blocks;
+	}
 	for (int first = 0; first < nframes; first += kParamsPerLaunch)
 	{
 		ParamBlock block;
 		const int count = nframes - first < kParamsPerLaunch ? nframes - first : kParamsPerLaunch;
 		for (int i = 0; i < count; i++)
 			block.p[i] = params[first + i];
-		hipLaunchKernelGGL(k_set_params, dim3(1), dim3(64), 0, stream, frames, block, first, count);
+		hipLaunchKernelGGL(k_set_params, dim3(blocks, count), dim3(256), 0, stream, frames, block, first, count);
 	}
-}
-
-void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream)
-{
-	int n = max_n32 > kFlat ? max_n32 : kFlat;
-	int blocks = (n + 255) / 256;
-	blocks = blocks > 64 ? 64 : blocks;
-	hipLaunchKernelGGL(k_rc_reset, dim3(blocks, nframes), dim3(256), 0, stream, frames);
 }
 
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
@@ -1170,10 +1180,6 @@ void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nb
 		                   xcd_count());
 }
 
-void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream)
-{
-	hipLaunchKernelGGL(k_rc_finalize, dim3(nframes), dim3(512), 0, stream, frames);
-}
 
 void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream)
 {

@@ -93,6 +93,7 @@ enum RcWord
 {
 	kRcConsumed = 0,
 	kRcError = 1,
+	kRcDone = 2, // quantized blocks of the frame (the last one runs the finalize)
 	kRcCount = 4
 };
 
@@ -282,11 +283,10 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
                         int variant, hipStream_t stream);
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         int variant, hipStream_t stream);
-void launch_rc_reset(FrameState *frames, int nframes, int max_n32, hipStream_t stream);
-// blk_band: two words per block (band job index, bx | by << 16).
+// blk_band: two words per block (band job index, bx | by << 16).  Each
+// frame's last block to finish also runs the rate-control finalize.
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
                           hipStream_t stream);
-void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream);
 void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream);
 void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroups_total,
                        const int32_t *frame_group_base, hipStream_t stream);
@@ -294,7 +294,9 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
                  hipStream_t stream);
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
                     hipStream_t stream);
-void launch_set_params(FrameState *frames, const CallParams *params, int nframes, hipStream_t stream);
+// Writes the per-call fields and, for encoder frames, clears the rate-control
+// accumulators (max_n32: the largest frame's 32x32 block count, 0 to skip).
+void launch_set_params(FrameState *frames, const CallParams *params, int nframes, int max_n32, hipStream_t stream);
 // The per-call parameter kernel (identifies its nodes in a captured graph).
 const void *set_params_kernel();
 void launch_packetize(const PacketizeJob *jobs, int count, hipStream_t stream);

@@ -743,7 +743,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 
 FrameState *Plan::stage_frames(hipStream_t stream)
 {
-	launch_set_params(d_frames, params.data(), int(params.size()), stream);
+	launch_set_params(d_frames, params.data(), int(params.size()), encoder ? max_n32 : 0, stream);
 	return d_frames;
 }
 
@@ -847,8 +847,6 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 	if (!timed)
 		D.timing = 0;
 
-	launch_rc_reset(frames, n, P.max_n32, stream);
-
 	const int reps = timed ? D.tune_reps : 0;
 	D.begin_stage(stream, kStageDWT, n, &ev);
 	for (int l = 0; l < kLevels; l++)
@@ -868,7 +866,6 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 	D.end_stage(stream, kStageQuant, n, ev);
 
 	D.begin_stage(stream, kStageResolve, n, &ev);
-	launch_rc_finalize(frames, n, stream);
 	launch_rc_partial(frames, n, stream);
 	launch_rc_prepare(frames, P.d_grp_frame, P.ngroups, P.d_frame_group_base, stream);
 	D.end_stage(stream, kStageResolve, n, ev);
