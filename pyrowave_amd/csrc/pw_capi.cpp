@@ -1623,6 +1623,7 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encoder_encode_coefficients(pyr
 	set_encode_params(F, S, rate_control->maximum_bitstream_size);
 	FrameState *frames = P->stage_frames(D.stream);
 	launch_quant_analyze(P->d_bands, P->d_blk_band, P->nblocks, frames, D.stream);
+	launch_rc_finalize(frames, 1, D.stream);
 	launch_rc_partial(frames, 1, D.stream);
 	launch_rc_prepare(frames, P->d_grp_frame, P->ngroups, P->d_frame_group_base, D.stream);
 	launch_pack(P->d_bands, P->d_blk_band, P->nblocks, frames, D.stream);

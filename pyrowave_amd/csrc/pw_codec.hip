@@ -4,10 +4,9 @@
 //                     per coded 32x32 block (reference shaders/wavelet_quant.comp
 //                     and shaders/analyze_rate_control.comp).  The per-8x8
 //                     statistics stay in LDS; only 48 B per block leave.
-//                     The frame's last block to finish then runs the
-//                     finalize: savings scan + per-(bucket,subdivision)
-//                     classification (shaders/analyze_rate_control_finalize.comp).
 //   k_set_params    : per-call fields; clears the rate-control accumulators.
+//   k_rc_finalize   : savings scan + per-(bucket,subdivision) classification
+//                     (shaders/analyze_rate_control_finalize.comp).
 //   k_rc_partial    : ordered greedy walk of the partially applied entries
 //                     (shaders/resolve_rate_control.comp).
 //   k_rc_prepare    : final truncation per block and the block-order word
@@ -330,87 +329,6 @@ __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const u
 	return C;
 }
 
-// The rate-control finalize of one frame, run by the frame's last quantized
-// block (128 threads, 16 entries each): the scan of the 2048 savings totals
-// and, per (bucket, subdivision) entry, the skip / full / partial class and
-// the savings still required there (reference
-// shaders/analyze_rate_control_finalize.comp:14-38 and the skip / required
-// logic of shaders/resolve_rate_control.comp:38-56).  The reference scans
-// with 512 invocations of 4 entries and stops its Hillis-Steele loop at step
-// 256, so entry x of group g = x / 4 holds the sum over groups g - 255 .. g
-// only: the exact prefix P minus P[4g - 1021] once g >= 256 (quirk Q1; the
-// exact-prefix flag keeps P).  Totals and the consumed words are read with
-// device-coherent loads: other blocks' atomics wrote them.
-__device__ __noinline__ void rc_finalize(const FrameState &F, int t)
-{
-	__shared__ uint32_t s_wave[2];
-	__shared__ uint32_t s_q1[kBlockThreads][4];
-	__shared__ uint32_t s_tail[kBlockThreads];
-	constexpr int kPer = kFlat / kBlockThreads; // 16
-	uint32_t v[kPer];
-#pragma unroll
-	for (int i = 0; i < kPer; i++)
-		v[i] = __hip_atomic_load(&F.totals[kPer * t + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	uint32_t tot[kPer];
-#pragma unroll
-	for (int i = 0; i < kPer; i++)
-		tot[i] = v[i];
-#pragma unroll
-	for (int i = 1; i < kPer; i++)
-		v[i] += v[i - 1];
-	// Exclusive scan of the threads' sums over the two waves.
-	const int lane = t & 63, wave = t >> 6;
-	const uint32_t incl = (uint32_t)wave_incl_scan((int)v[kPer - 1]);
-	if (lane == 63)
-		s_wave[wave] = incl;
-	__syncthreads();
-	const uint32_t base = incl - v[kPer - 1] + (wave ? s_wave[0] : 0u);
-#pragma unroll
-	for (int i = 0; i < kPer; i++)
-		v[i] += base; // exact inclusive prefix P[16 t + i]
-	// P at the last entry of each of this thread's four groups, for the
-	// windows of the groups 64 threads further on.
-#pragma unroll
-	for (int k = 0; k < 4; k++)
-		s_q1[t][k] = v[4 * k + 3];
-	__syncthreads();
-	if (!(F.flags & 1u) && t >= 64)
-	{
-#pragma unroll
-		for (int i = 0; i < kPer; i++)
-			v[i] -= s_q1[t - 64][i >> 2];
-	}
-	s_tail[t] = v[kPer - 1];
-	__syncthreads();
-	const uint32_t consumed = __hip_atomic_load(&F.rc[kRcConsumed], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-	const int required_base = (int)consumed - F.target_words;
-	uint32_t prev = t ? s_tail[t - 1] : 0u;
-#pragma unroll
-	for (int i = 0; i < kPer; i++)
-	{
-		const int x = kPer * t + i;
-		F.prefix[x] = v[i];
-		bool empty;
-		int required;
-		if (x == 0)
-		{
-			empty = v[0] == 0;
-			required = required_base;
-		}
-		else
-		{
-			empty = v[i] == prev;
-			required = required_base - (int)prev;
-		}
-		uint8_t cls = kRcSkip;
-		if (!empty && required > 0)
-			cls = (int)tot[i] <= required ? kRcFull : kRcPartial;
-		F.cls[x] = cls;
-		F.required[x] = required;
-		prev = v[i];
-	}
-}
-
 __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
                                                                  const uint32_t *__restrict__ blk_band,
                                                                  const FrameState *__restrict__ frames,
@@ -566,20 +484,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 				atomicAdd(&F.totals[Bq * kSubdiv + ((uint32_t)C.b >> F.shamt)], saving);
 		}
 		else
-			// Unquantized words into the frame's consumed total, one atomic per
-			// block (reference shaders/analyze_rate_control.comp:127-131).
-			atomicAdd(&F.rc[kRcConsumed], s_words[0]);
-		__threadfence(); // this block's sums are done before it counts itself
-	}
-	__syncthreads();
-	__shared__ uint32_t s_last;
-	if (tid == 0)
-		s_last = atomicAdd(&F.rc[kRcDone], 1u) == (uint32_t)F.n32 - 1u ? 1u : 0u;
-	__syncthreads();
-	if (s_last)
-	{
-		__threadfence(); // every other block of the frame has counted itself
-		rc_finalize(F, tid);
+			// Unquantized words, also into a contiguous per-block array that
+			// the finalize sums (the reference adds them with one atomic per
+			// block into one word, shaders/analyze_rate_control.comp:127-131; a
+			// frame's thousands of same-address atomics serialize here).
+			F.words0[C.b] = (uint16_t)s_words[0];
 	}
 }
 
@@ -624,6 +533,88 @@ __global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, 
 	F.dec_payload = P.dec_payload;
 	F.dec_offset_stride = P.dec_offset_stride;
 	F.dec_payload_bytes = P.dec_payload_bytes;
+}
+
+// reference shaders/analyze_rate_control_finalize.comp:14-38 (512 threads x 4
+// entries) and the skip / required logic of shaders/resolve_rate_control.comp:38-56.
+__global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ frames)
+{
+	__shared__ uint32_t s_scan[512];
+	__shared__ uint32_t s_consumed;
+	const FrameState &F = frames[blockIdx.x];
+	const int t = threadIdx.x;
+	{
+		// Words of every block at quant level 0 (consumed payload), from the
+		// contiguous per-block array: eight blocks per 16-byte load (the array
+		// is padded to whole groups of eight, with zeros past the frame).
+		const uint4 *w = reinterpret_cast<const uint4 *>(F.words0);
+		const int groups = (F.n32 + 7) >> 3;
+		uint32_t acc = 0;
+		for (int g = t; g < groups; g += 512)
+		{
+			const uint4 q = w[g];
+			acc += (q.x & 0xffffu) + (q.x >> 16) + (q.y & 0xffffu) + (q.y >> 16) + (q.z & 0xffffu) + (q.z >> 16) +
+			       (q.w & 0xffffu) + (q.w >> 16);
+		}
+		for (int d = 32; d >= 1; d >>= 1)
+			acc += __shfl_xor(acc, d, 64);
+		if (t == 0)
+			s_consumed = 0;
+		__syncthreads();
+		if ((t & 63) == 0)
+			atomicAdd(&s_consumed, acc);
+		__syncthreads();
+		if (t == 0)
+			F.rc[kRcConsumed] = s_consumed;
+	}
+	uint32_t v[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+		v[i] = F.totals[4 * t + i];
+	v[1] += v[0];
+	v[2] += v[1];
+	v[3] += v[2];
+	s_scan[t] = v[3];
+	__syncthreads();
+	// The reference stops at step < 256; the exact variant runs one more step.
+	const uint32_t limit = (F.flags & 1u) ? 512u : 256u;
+	for (uint32_t step = 1; step < limit; step *= 2)
+	{
+		__syncthreads();
+		uint32_t up = t >= (int)step ? s_scan[t - step] : 0u;
+		__syncthreads();
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			v[i] += up;
+		s_scan[t] = v[3];
+	}
+	__syncthreads();
+	const int required_base = (int)s_consumed - F.target_words;
+	uint32_t prev = t ? s_scan[t - 1] : 0u;
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+	{
+		const int x = 4 * t + i;
+		F.prefix[x] = v[i];
+		bool empty;
+		int required;
+		if (x == 0)
+		{
+			empty = v[0] == 0;
+			required = required_base;
+		}
+		else
+		{
+			empty = v[i] == prev;
+			required = required_base - (int)prev;
+		}
+		uint8_t cls = kRcSkip;
+		if (!empty && required > 0)
+			cls = (int)F.totals[x] <= required ? kRcFull : kRcPartial;
+		F.cls[x] = cls;
+		F.required[x] = required;
+		prev = v[i];
+	}
 }
 
 __device__ __forceinline__ int block_incl_scan256(int v, int *s_tmp, int &total)
@@ -1180,6 +1171,11 @@ void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nb
 		                   xcd_count());
 }
 
+
+void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_rc_finalize, dim3(nframes), dim3(512), 0, stream, frames);
+}
 
 void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream)
 {

@@ -93,7 +93,6 @@ enum RcWord
 {
 	kRcConsumed = 0,
 	kRcError = 1,
-	kRcDone = 2, // quantized blocks of the frame (the last one runs the finalize)
 	kRcCount = 4
 };
 
@@ -103,6 +102,7 @@ struct FrameState
 {
 	// Encoder rate-control arrays.
 	uint16_t *words;        // [n32][16] words at each quant level
+	uint16_t *words0;       // [n32 rounded up to 8] words at quant level 0
 	uint8_t *bucket;        // [n32][16] inclusive R-D bucket
 	int32_t *partial_quant; // [n32] truncation chosen by partially applied buckets
 	uint8_t *quant;         // [n32] final truncation
@@ -283,10 +283,10 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
                         int variant, hipStream_t stream);
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         int variant, hipStream_t stream);
-// blk_band: two words per block (band job index, bx | by << 16).  Each
-// frame's last block to finish also runs the rate-control finalize.
+// blk_band: two words per block (band job index, bx | by << 16).
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
                           hipStream_t stream);
+void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream);
 void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream);
 void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroups_total,
                        const int32_t *frame_group_base, hipStream_t stream);

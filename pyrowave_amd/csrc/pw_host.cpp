@@ -312,6 +312,9 @@ SlotLayout slot_layout(const Geometry &G, bool encoder)
 	if (encoder)
 	{
 		L.words = take(n32 * 16 * sizeof(uint16_t));
+		// Padded to whole groups of eight (zeros past the frame: the slot is
+		// cleared once and the quantizer writes only real blocks).
+		L.words0 = take(((n32 + 7) & ~size_t(7)) * sizeof(uint16_t));
 		L.bucket = take(n32 * 16);
 		L.partial_quant = take(n32 * sizeof(int32_t));
 		L.quant = take(n32);
@@ -350,6 +353,7 @@ bool Slot::init(Device *dev_, int w, int h, int chroma, int precision, bool enco
 	if (encoder)
 	{
 		fs.words = reinterpret_cast<uint16_t *>(base + L.words);
+		fs.words0 = reinterpret_cast<uint16_t *>(base + L.words0);
 		fs.bucket = base + L.bucket;
 		fs.partial_quant = reinterpret_cast<int32_t *>(base + L.partial_quant);
 		fs.quant = base + L.quant;
@@ -866,6 +870,7 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 	D.end_stage(stream, kStageQuant, n, ev);
 
 	D.begin_stage(stream, kStageResolve, n, &ev);
+	launch_rc_finalize(frames, n, stream);
 	launch_rc_partial(frames, n, stream);
 	launch_rc_prepare(frames, P.d_grp_frame, P.ngroups, P.d_frame_group_base, stream);
 	D.end_stage(stream, kStageResolve, n, ev);

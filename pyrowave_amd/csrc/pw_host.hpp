@@ -121,7 +121,7 @@ struct Plan;
 // Byte offsets of a slot's parts inside its one device allocation.
 struct SlotLayout
 {
-	size_t bands, words, bucket, partial_quant, quant, excl, group_sum, meta, totals, prefix, cls, required, rc,
+	size_t bands, words, words0, bucket, partial_quant, quant, excl, group_sum, meta, totals, prefix, cls, required, rc,
 	    bitstream, packets, total;
 };
 SlotLayout slot_layout(const Geometry &G, bool encoder);
