@@ -186,6 +186,25 @@ struct DwtJob
 	int32_t component;
 };
 
+// The first two levels (l, l + 1) of one component of one frame in one
+// forward launch: a wave streams a column strip of level l and feeds the LL
+// rows it produces straight into level l + 1, so LL_l never leaves the
+// registers (it is not written).  A tile is kFusedTileW1 x kFusedTileH1
+// level-(l+1) coefficients per band.
+struct FusedJob
+{
+	DwtJob a; // level l (its source is the component's input; its LL is not written)
+	DwtJob b; // level l + 1 (b.src unused)
+	int32_t tiles_x;
+	int32_t tile_base;
+};
+constexpr int kFusedTileW1 = 26;
+constexpr int kFusedTileH1 = 32;
+// A fused wave walks 76 row pairs (twice a streamed tile's) at 80 VGPRs: it
+// pays only when there are waves enough to hide that, i.e. batches of about
+// seventeen 4K frames or more; smaller launches run level by level.
+constexpr int kFusedMinTiles = 16384;
+
 // One transmitted band of one frame, for the per-32x32-block kernels.
 struct BandJob
 {
@@ -283,6 +302,8 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
                         int variant, hipStream_t stream);
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         int variant, hipStream_t stream);
+void launch_dwt_forward_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                              hipStream_t stream);
 // blk_band: two words per block (band job index, bx | by << 16).
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
                           hipStream_t stream);

@@ -895,6 +895,189 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const 
 		forward_stream_tile<kStoreF32>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 }
 
+// --- forward, levels l and l + 1 fused -------------------------------------
+//
+// One wave per tile of kFusedTileW1 x kFusedTileH1 level-(l+1) coefficients
+// (per band).  The wave streams level l exactly as k_dwt_forward_stream does,
+// over the level-l window that the tile's level-(l+1) outputs need: LL_l
+// columns 2*ox1 - 4 .. 2*ox1 + 55 (lanes 2 .. 61) and rows 2*oy1 - 4 ..
+// 2*oy1 + 2*H1 + 3, one LL_l row per row-pair step.  It stores the level-l
+// high bands of the part it owns (lanes 6 .. 57, LL_l rows 2*oy1 .. +2*H1);
+// LL_l goes on in registers: every two LL_l rows, the pairs of LL_l columns
+// are gathered to lanes 0 .. 29 (ds_bpermute), lifted horizontally across
+// those lanes and vertically by a second delay line, and lanes 2 .. 27 store
+// level l + 1.  So LL_l is never written nor read back.
+//
+// Extension of level l + 1's input (whole-sample mirror about 0 and
+// A - 1 of LL_l, as the level-(l+1) kernels read it): above and left of the
+// frame the LL_l values the wave computes are already the mirror images
+// (level l's input is mirrored about 0, and the lifting is symmetric, down to
+// the order of every float addition); past the right edge the gather takes
+// the mirrored column, past the bottom a short history of LL_l rows supplies
+// the mirrored rows.  LL_l values are rounded to the storage the separate
+// launches would keep them in (fp16 for FP16_LL), so the coefficients equal
+// those of the level-by-level launches bit for bit.
+template <int KIND, bool LL16>
+__device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob &K, const void *src_ptr,
+                                                   uint32_t pitch, uint32_t step, int ox1, int oy1, int lane)
+{
+	constexpr int H1 = kFusedTileH1;
+	constexpr int kSteps = 2 * H1 + 12;
+	static_assert(kSteps % 2 == 0, "the row-pair loop is unrolled by two");
+	const StreamSrc src = stream_src(src_ptr);
+	const int x0 = 4 * ox1 - 8, y0 = 4 * oy1 - 8;
+	const uint32_t xe = __umul24((uint32_t)fwd_ext(x0 - kApron + 2 * lane, J.R_x, J.A_x), step);
+	const uint32_t xo = __umul24((uint32_t)fwd_ext(x0 - kApron + 2 * lane + 1, J.R_x, J.A_x), step);
+	const int ox = 2 * ox1 - 6 + lane; // level-l band column (LL_l column) of this lane
+	const bool store_x = lane >= 6 && lane < 58 && ox < J.bw;
+	// Level l + 1: lane k < 30 lifts the LL_l column pair 2*ox1 - 4 + 2k, +1,
+	// gathered from the lanes that hold them (mirrored past the right edge).
+	const int A1 = K.A_x; // LL_l columns (= J.bw)
+	auto ext1 = [&](int c) __attribute__((always_inline)) { return c < A1 ? c : 2 * A1 - 2 - c; };
+	const int ce = 2 * ox1 - 4 + 2 * lane;
+	int se = ext1(ce) - (2 * ox1 - 6), so = ext1(ce + 1) - (2 * ox1 - 6);
+	se = lane < 30 && se >= 0 && se < 64 ? se : lane;
+	so = lane < 30 && so >= 0 && so < 64 ? so : lane;
+	const int addr_e = se << 2, addr_o = so << 2;
+	const int oc = ox1 + lane - 2; // level-(l+1) band column of lane k
+	const bool store_x1 = lane >= 2 && lane < 2 + kFusedTileW1 && oc < K.bw;
+	const int B1 = K.A_y; // LL_l rows (= J.bh)
+
+	auto row_base = [&](int r) __attribute__((always_inline)) {
+		return __umul24((uint32_t)fwd_ext(r, J.R_y, J.A_y), pitch);
+	};
+	auto load_pair = [&](int n, uint32_t (&r)[4], uint32_t &r0, uint32_t &r1) __attribute__((always_inline)) {
+		const int row = y0 - kApron + 2 * n;
+		r0 = row_base(row);
+		r1 = row_base(row + 1);
+		r[0] = stream_raw<KIND>(src, r0 + xe);
+		r[1] = stream_raw<KIND>(src, r0 + xo);
+		r[2] = stream_raw<KIND>(src, r1 + xe);
+		r[3] = stream_raw<KIND>(src, r1 + xo);
+	};
+	VLine lo, hi;   // level l
+	VLine lo1, hi1; // level l + 1
+	float pend = 0.0f;
+	float h0 = 0.0f, h1 = 0.0f, h2 = 0.0f, h3 = 0.0f, h4 = 0.0f; // LL_l rows j-1 .. j-5 (bottom mirror)
+	auto feed = [&](int n, float ll) __attribute__((always_inline)) {
+		// LL_l row j = 2*oy1 - 8 + n, for n >= 4.
+		const int j = 2 * oy1 - 8 + n;
+		if (LL16)
+		{
+			// The FP32 product rounded to fp16, as the store of the separate
+			// launch does (the empty asm keeps the compiler from folding the
+			// product and the conversion into one v_fma_mix, which rounds the
+			// exact product once).
+			asm volatile("" : "+v"(ll));
+			ll = __half2float(__float2half_rn(ll));
+		}
+		if (j >= B1)
+		{
+			// Row B1 + t mirrors to B1 - 2 - t: the frozen history, consumed
+			// one row per step.
+			ll = h1;
+			h1 = h2;
+			h2 = h3;
+			h3 = h4;
+		}
+		else if (j >= B1 - 5)
+		{
+			h4 = h3;
+			h3 = h2;
+			h2 = h1;
+			h1 = h0;
+			h0 = ll;
+		}
+		if (((n - 4) & 1) == 0)
+		{
+			pend = ll;
+			return;
+		}
+		// Pair m of level l + 1: LL_l rows 2*oy1 - 4 + 2m (pend) and + 1 (ll).
+		const int m = (n - 5) >> 1;
+		float e0 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr_e, __float_as_int(pend)));
+		float o0 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr_o, __float_as_int(pend)));
+		float e1 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr_e, __float_as_int(ll)));
+		float o1 = __int_as_float(__builtin_amdgcn_ds_bpermute(addr_o, __float_as_int(ll)));
+		hlift_row(e0, o0);
+		hlift_row(e1, o1);
+		float a_ll, a_lh, a_hl, a_hh;
+		lo1.push(e0, e1, a_ll, a_lh);
+		hi1.push(o0, o1, a_hl, a_hh);
+		const int oy2 = oy1 + m - kApron;
+		if (m >= kApron && store_x1 && oy2 < K.bh)
+		{
+			const uint32_t idx = __umul24((uint32_t)oy2, (uint32_t)K.band_pitch) + (uint32_t)oc;
+			stream_store(K.band[0], K.ll_kind, idx, a_ll);
+			stream_store(K.band[1], K.band_kind, idx, a_hl);
+			stream_store(K.band[2], K.band_kind, idx, a_lh);
+			stream_store(K.band[3], K.band_kind, idx, a_hh);
+		}
+	};
+	auto lift_pair = [&](int n, uint32_t r0, uint32_t r1, const uint32_t (&r)[4]) __attribute__((always_inline)) {
+		float e0 = stream_value<KIND>(r[0], src, r0 + xe), o0 = stream_value<KIND>(r[1], src, r0 + xo);
+		float e1 = stream_value<KIND>(r[2], src, r1 + xe), o1 = stream_value<KIND>(r[3], src, r1 + xo);
+		hlift_row(e0, o0);
+		hlift_row(e1, o1);
+		float ll, lh, hl, hh;
+		lo.push(e0, e1, ll, lh);
+		hi.push(o0, o1, hl, hh);
+		if (n < kApron)
+			return;
+		const int j = 2 * oy1 - 8 + n;
+		if (n >= 8 && n < 8 + 2 * H1 && store_x && j < J.bh)
+		{
+			const uint32_t idx = __umul24((uint32_t)j, (uint32_t)J.band_pitch) + (uint32_t)ox;
+			stream_store(J.band[1], J.band_kind, idx, hl);
+			stream_store(J.band[2], J.band_kind, idx, lh);
+			stream_store(J.band[3], J.band_kind, idx, hh);
+		}
+		feed(n, ll);
+	};
+	uint32_t ra[4], rb[4], a0, a1, b0, b1;
+	load_pair(0, ra, a0, a1);
+	load_pair(1, rb, b0, b1);
+#pragma unroll 1
+	for (int n = 0; n < kSteps; n += 2)
+	{
+		const uint32_t pa0 = a0, pa1 = a1, pb0 = b0, pb1 = b1;
+		const uint32_t ca[4] = {ra[0], ra[1], ra[2], ra[3]}, cb[4] = {rb[0], rb[1], rb[2], rb[3]};
+		if (n + 2 < kSteps)
+		{
+			load_pair(n + 2, ra, a0, a1);
+			load_pair(n + 3, rb, b0, b1);
+		}
+		// The scheduler would otherwise sink the loads towards their use.
+		__builtin_amdgcn_sched_barrier(0);
+		lift_pair(n, pa0, pa1, ca);
+		lift_pair(n + 1, pb0, pb1, cb);
+	}
+}
+
+__global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_fused(const FusedJob *__restrict__ jobs,
+                                                                         const uint16_t *__restrict__ tile_job,
+                                                                         const FrameState *__restrict__ frames,
+                                                                         int tiles, int nxcd)
+{
+	const int lane = threadIdx.x & 63;
+	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	if (tile >= tiles)
+		return; // whole waves: no wave-level or workgroup barrier follows
+	const FusedJob &FJ = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
+	const DwtJob J = FJ.a, K = FJ.b;
+	const int local = tile - FJ.tile_base;
+	const int ox1 = (local % FJ.tiles_x) * kFusedTileW1;
+	const int oy1 = (local / FJ.tiles_x) * kFusedTileH1;
+	// The fused levels are a component's first two: level l reads the u8 plane.
+	const FrameState &F = frames[J.frame];
+	const void *p = F.src[J.component];
+	const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
+	if (J.ll_kind == kStoreF16)
+		forward_fused_tile<kStoreU8, true>(J, K, p, pitch, step, ox1, oy1, lane);
+	else
+		forward_fused_tile<kStoreU8, false>(J, K, p, pitch, step, ox1, oy1, lane);
+}
+
 // --- inverse, streamed (shape 4) ------------------------------------------
 //
 // The inverse counterpart of k_dwt_forward_stream: one wave per tile of
@@ -1456,6 +1639,14 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 	else
 		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[0], kFwdVariantTileH[0], true>), dim3(tiles),
 		                   dim3(2 * kFwdVariantTileW[0]), 0, stream, jobs, tile_job, frames, xcd_count());
+}
+
+void launch_dwt_forward_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                              hipStream_t stream)
+{
+	if (tiles > 0)
+		hipLaunchKernelGGL(k_dwt_forward_fused, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
+		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
 }
 
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,

@@ -503,6 +503,11 @@ bool Slot::decode_is_ready(bool allow_partial) const
 // Plan
 // ---------------------------------------------------------------------------
 
+static bool fusable(const Geometry &G, int c)
+{
+	return G.plane_w(c) >= 13 && G.plane_h(c) >= 13;
+}
+
 bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *storage, hipStream_t stream)
 {
 	slots = slots_;
@@ -512,6 +517,10 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	params.assign(n, CallParams{});
 	std::vector<DwtJob> jobs[kLevels][kDwtVariants];
 	std::vector<uint16_t> tile_map[kLevels][kDwtVariants];
+	std::vector<DwtJob> rjobs[kLevels][kDwtVariants];
+	std::vector<uint16_t> rtile_map[kLevels][kDwtVariants];
+	std::vector<FusedJob> fjobs;
+	std::vector<uint16_t> ftile_map;
 	std::vector<BandJob> band_jobs;
 	std::vector<uint32_t> blk_band;
 	std::vector<int32_t> frame_block_base(n), frame_group_base(n);
@@ -523,6 +532,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	for (Slot *S : slots)
 		if ((S->geom.precision == PYROWAVE_HIP_PRECISION_FP16_MATH) != half_math)
 			return false;
+	has_fused = encoder && !half_math;
 
 	for (int f = 0; f < n; f++)
 	{
@@ -532,6 +542,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 		max_n32 = std::max(max_n32, G.count32);
 		auto band_ptr = [&](int c, int l, int b) { return static_cast<void *>(S->bands + G.band[c][l][b].store_off); };
 
+		DwtJob first_two[kComponents][2] = {};
 		for (int l = 0; l < kLevels; l++)
 		{
 			for (int c = 0; c < kComponents; c++)
@@ -539,6 +550,11 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 				const int first = G.first_level(c);
 				if (l < first)
 					continue;
+				// The fused launch takes the LL values it computes above and left
+				// of the frame as the mirror images of those inside, which holds
+				// when the plane's extension mirrors the 12 rows and columns its
+				// window reaches past the edge: planes of at least 13 samples.
+				const bool in_fused = has_fused && l <= first + 1 && fusable(G, c);
 				const BandGeom &B0 = G.band[c][l][0];
 				DwtJob J = {};
 				J.frame = f;
@@ -592,6 +608,8 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 						J.R_y = J.A_y;
 					}
 				}
+				if (in_fused)
+					first_two[c][l - first] = J;
 				for (int v = 0; v < kDwtVariants; v++)
 				{
 					if ((v == kHalfStreamVariant) != half_math)
@@ -605,9 +623,31 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 					const uint16_t idx = uint16_t(jobs[l][v].size());
 					jobs[l][v].push_back(Jv);
 					tile_map[l][v].insert(tile_map[l][v].end(), size_t(Jv.tiles_x) * tiles_y, idx);
+					if (!in_fused)
+					{
+						Jv.tile_base = int(rtile_map[l][v].size());
+						const uint16_t ridx = uint16_t(rjobs[l][v].size());
+						rjobs[l][v].push_back(Jv);
+						rtile_map[l][v].insert(rtile_map[l][v].end(), size_t(Jv.tiles_x) * tiles_y, ridx);
+					}
 				}
 			}
 		}
+		if (has_fused)
+			for (int c = 0; c < kComponents; c++)
+			{
+				if (!fusable(G, c))
+					continue;
+				FusedJob FJ = {};
+				FJ.a = first_two[c][0];
+				FJ.b = first_two[c][1];
+				FJ.tiles_x = (FJ.b.bw + kFusedTileW1 - 1) / kFusedTileW1;
+				const int tiles_y = (FJ.b.bh + kFusedTileH1 - 1) / kFusedTileH1;
+				FJ.tile_base = int(ftile_map.size());
+				const uint16_t idx = uint16_t(fjobs.size());
+				fjobs.push_back(FJ);
+				ftile_map.insert(ftile_map.end(), size_t(FJ.tiles_x) * tiles_y, idx);
+			}
 
 		frame_block_base[f] = int32_t(blk_band.size() / 2);
 		for (int l = kLevels - 1; l >= 0; l--)
@@ -650,11 +690,15 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	nblocks = int(blk_band.size() / 2);
 	ngroups = int(grp_frame.size());
 	Device &D = *slots[0]->dev;
+	fused_tiles = int(ftile_map.size());
+	if (fjobs.size() > 65535)
+		return false;
 	for (int l = 0; l < kLevels; l++)
 	{
 		for (int v = 0; v < kDwtVariants; v++)
 		{
 			tiles[l][v] = int(tile_map[l][v].size());
+			rest_tiles[l][v] = int(rtile_map[l][v].size());
 			if (jobs[l][v].size() > 65535)
 				return false;
 		}
@@ -678,12 +722,17 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	};
 	size_t o_frames = take(sizeof(FrameState) * n);
 	size_t o_jobs[kLevels][kDwtVariants], o_tiles[kLevels][kDwtVariants];
+	size_t o_rjobs[kLevels][kDwtVariants], o_rtiles[kLevels][kDwtVariants];
 	for (int l = 0; l < kLevels; l++)
 		for (int v = 0; v < kDwtVariants; v++)
 		{
 			o_jobs[l][v] = take(sizeof(DwtJob) * jobs[l][v].size());
 			o_tiles[l][v] = take(sizeof(uint16_t) * tile_map[l][v].size());
+			o_rjobs[l][v] = take(sizeof(DwtJob) * rjobs[l][v].size());
+			o_rtiles[l][v] = take(sizeof(uint16_t) * rtile_map[l][v].size());
 		}
+	size_t o_fjobs = take(sizeof(FusedJob) * fjobs.size());
+	size_t o_ftiles = take(sizeof(uint16_t) * ftile_map.size());
 	size_t o_bands = take(sizeof(BandJob) * band_jobs.size());
 	size_t o_blk = take(sizeof(uint32_t) * blk_band.size());
 	size_t o_fbb = take(sizeof(int32_t) * n);
@@ -721,7 +770,15 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 			put(o_tiles[l][v], tile_map[l][v].data(), sizeof(uint16_t) * tile_map[l][v].size());
 			d_dwt_jobs[l][v] = reinterpret_cast<DwtJob *>(base + o_jobs[l][v]);
 			d_tile_job[l][v] = reinterpret_cast<uint16_t *>(base + o_tiles[l][v]);
+			put(o_rjobs[l][v], rjobs[l][v].data(), sizeof(DwtJob) * rjobs[l][v].size());
+			put(o_rtiles[l][v], rtile_map[l][v].data(), sizeof(uint16_t) * rtile_map[l][v].size());
+			d_rest_jobs[l][v] = reinterpret_cast<DwtJob *>(base + o_rjobs[l][v]);
+			d_rest_tile_job[l][v] = reinterpret_cast<uint16_t *>(base + o_rtiles[l][v]);
 		}
+	put(o_fjobs, fjobs.data(), sizeof(FusedJob) * fjobs.size());
+	put(o_ftiles, ftile_map.data(), sizeof(uint16_t) * ftile_map.size());
+	d_fused_jobs = reinterpret_cast<FusedJob *>(base + o_fjobs);
+	d_fused_tile_job = reinterpret_cast<uint16_t *>(base + o_ftiles);
 	put(o_bands, band_jobs.data(), sizeof(BandJob) * band_jobs.size());
 	put(o_blk, blk_band.data(), sizeof(uint32_t) * blk_band.size());
 	put(o_fbb, frame_block_base.data(), sizeof(int32_t) * n);
@@ -853,16 +910,29 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 
 	const int reps = timed ? D.tune_reps : 0;
 	D.begin_stage(stream, kStageDWT, n, &ev);
-	for (int l = 0; l < kLevels; l++)
+	if (P.has_fused && D.forced_variant < 0 && reps == 0 &&
+	    (D.fused >= 2 || (D.fused == 1 && P.fused_tiles >= kFusedMinTiles)))
 	{
-		auto launch = [&](int v) {
-			launch_dwt_forward(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
-		};
-		if (reps > 0 && !P.half_math)
-			tune_level(P, true, l, reps, stream, launch);
-		else
-			launch(level_variant(P, l));
+		// The first two levels of every component in one launch, then the
+		// remaining levels with their tuned shapes.
+		launch_dwt_forward_fused(P.d_fused_jobs, P.d_fused_tile_job, P.fused_tiles, frames, stream);
+		for (int l = 0; l < kLevels; l++)
+		{
+			const int v = P.variant[l];
+			launch_dwt_forward(P.d_rest_jobs[l][v], P.d_rest_tile_job[l][v], P.rest_tiles[l][v], frames, v, stream);
+		}
 	}
+	else
+		for (int l = 0; l < kLevels; l++)
+		{
+			auto launch = [&](int v) {
+				launch_dwt_forward(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
+			};
+			if (reps > 0 && !P.half_math)
+				tune_level(P, true, l, reps, stream, launch);
+			else
+				launch(level_variant(P, l));
+		}
 	D.end_stage(stream, kStageDWT, n, ev);
 
 	D.begin_stage(stream, kStageQuant, n, &ev);

@@ -105,6 +105,7 @@ struct Device
 	// variant 0); new plans of the same shape start from these.
 	std::map<uint64_t, int> tuned_variant;
 	int forced_variant = -1; // >= 0: every transform level uses this variant
+	int fused = 1;           // forward levels 0+1 in one launch (k_dwt_forward_fused): 0 never, 1 batches of kFusedMinTiles tiles or more, 2 always
 	int tune_reps = 0;       // > 0: the next eager encode / decode autotunes
 	// PSNR-HVS-M-H tables per image height (device copies).
 	std::map<int, std::unique_ptr<DeviceBuffer>> psnr_tables;
@@ -208,6 +209,15 @@ struct Plan
 	uint16_t *d_tile_job[kLevels][kDwtVariants] = {};
 	int tiles[kLevels][kDwtVariants] = {};
 	int variant[kLevels] = {};
+	// Encoder plans of FP32-arithmetic frames: the fused first two levels of
+	// every component, and the per-level tables of the levels that remain.
+	bool has_fused = false;
+	FusedJob *d_fused_jobs = nullptr;
+	uint16_t *d_fused_tile_job = nullptr;
+	int fused_tiles = 0;
+	DwtJob *d_rest_jobs[kLevels][kDwtVariants] = {};
+	uint16_t *d_rest_tile_job[kLevels][kDwtVariants] = {};
+	int rest_tiles[kLevels][kDwtVariants] = {};
 	BandJob *d_bands = nullptr;
 	uint32_t *d_blk_band = nullptr;
 	int32_t *d_frame_block_base = nullptr;

@@ -54,3 +54,16 @@ def band_errors(L, a, b):
                 d = x - y
                 res[(c, l, bd)] = (float(np.abs(d).max()), float(np.sqrt(np.mean(d * d))))
     return res
+
+
+def mask_unstored_ll(L, flat):
+    """The flat coefficient array with the LL band of each component's first
+    level zeroed: the fused forward launch keeps that intermediate in
+    registers and never stores it (every transmitted band, and the LL of the
+    later levels, is stored)."""
+    out = np.array(flat, copy=True)
+    for c in range(3):
+        first = 1 if (c and L.chroma == 0) else 0
+        B = L.band[c][first][0]
+        out[B.coeff_off:B.coeff_off + B.bw * B.bh] = 0
+    return out

@@ -1,0 +1,53 @@
+"""The fused first two forward levels (k_dwt_forward_fused, the forward path
+of large batches, forced here for every launch with PYROWAVE_FUSED=2) against
+the level-by-level launches (GPU): every stored coefficient and the bitstream
+are identical, on sizes that leave partial tiles at the right and bottom
+edges, 4:2:0 and 4:4:4, in every precision with FP32 arithmetic."""
+import numpy as np
+import pytest
+
+import pyrowave_amd as pw
+from oracle import oracle as O
+from pyrowave_amd import synth
+
+from helpers import mask_unstored_ll
+
+pytestmark = pytest.mark.gpu
+
+SIZES = [(4, 4, pw.CHROMA_420), (34, 30, pw.CHROMA_420), (26, 26, pw.CHROMA_420), (33, 17, pw.CHROMA_444),
+         (13, 13, pw.CHROMA_444), (200, 130, pw.CHROMA_444), (416, 240, pw.CHROMA_420),
+         (1000, 750, pw.CHROMA_444), (1920, 1080, pw.CHROMA_420), (3840, 2160, pw.CHROMA_420)]
+
+
+def _encode(dev, w, h, c, precision, planes, budget, force):
+    dev.force_tile_variant(force)
+    enc = pw.Encoder(dev, w, h, c, precision=precision)
+    try:
+        assert enc.encode_cpu(planes, budget) == pw.SUCCESS
+        out, pk = enc.packetize(1 << 30, budget + 64)
+        data = out[:sum(z for _, z in pk)].copy()
+        coeffs = mask_unstored_ll(O.layout(w, h, c), enc.read_coefficients())
+    finally:
+        enc.close()
+    return data, coeffs
+
+
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP32, pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
+@pytest.mark.parametrize("w,h,c", SIZES)
+def test_fused_forward_equals_level_by_level(w, h, c, precision, monkeypatch):
+    planes = synth.frame(w, h, c == pw.CHROMA_420, seed=w + h) if w * h >= 4096 else None
+    if planes is None:
+        from helpers import ramp_planes
+        planes = ramp_planes(w, h, c)
+    budget = max(2000, w * h // 4)
+    monkeypatch.setenv("PYROWAVE_FUSED", "2")
+    dev = pw.Device()
+    try:
+        fused = _encode(dev, w, h, c, precision, planes, budget, -1)
+        for shape in (0, 4):
+            ref = _encode(dev, w, h, c, precision, planes, budget, shape)
+            mism = np.flatnonzero(fused[1].view(np.uint32) != ref[1].view(np.uint32))
+            assert mism.size == 0, (shape, mism.size, mism[:8])
+            assert np.array_equal(fused[0], ref[0]), shape
+    finally:
+        dev.close()
