@@ -13,13 +13,16 @@ kernel durations are not stretched by the overlap.
 Reported: µs per frame of the FP16-storage batched path (value), the plain
 FP32 single-frame path beside it, ΔPSNR between the two, the same batched
 path with the LL of levels 0-1 also stored as fp16 (the reference's precision
-1 storage: less traffic, lower PSNR), the byte budget check, per-batch p50/p99
+1 storage: less traffic, lower PSNR), FP32 batched like the headline (batching
+apart from storage precision), packed-FP16 lifting batched (the reference's
+precision 0), the byte budget check, per-batch p50/p99
 latency, the transform rooflines from HIP events recorded on the launch
 stream, 1080p frames batched in FP16 and one at a time
 in FP32, one batch of 32 uneven frames (720p to 4K, 4:2:0 and 4:4:4) eager
 and as a replayed hipGraph, the serving loop (64 mixed sessions: latency
-p50/p99 at cadence and saturated frames/s), and the oracle (single-threaded
-CPU restatement) timed on a bounded sample as the CPU baseline.
+p50/p99 at cadence and saturated frames/s), and the oracle (the C
+restatement, on up to 16 host threads) timed on a bounded sample as the CPU
+baseline.
 """
 import argparse
 import json
@@ -53,6 +56,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp16-ll", action="store_true",
                     help="skip the FP16_LL leg (profiled runs: its transform launches have the headline's grids)")
+    ap.add_argument("--no-extra-precisions", action="store_true",
+                    help="skip the FP32-batched and packed-FP16 (FP16_MATH) legs")
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the 1080p, uneven-batch and serving legs (profiled runs: some of their transform "
                          "launches have the grids of the headline batch's levels)")
@@ -515,6 +520,15 @@ def main():
         fp16_ll = timed(pw.PRECISION_FP16_LL, B, args.steps, args.warmup, args.streams)
         print(f"[bench] fp16_ll batched done", file=sys.stderr, flush=True)
 
+    # FP32 batched like the headline (separates batching from storage
+    # precision) and packed-FP16 lifting (the reference's precision 0).
+    fp32b = fp16m = None
+    if not args.no_extra_precisions:
+        fp32b = timed(pw.PRECISION_FP32, B, args.steps, args.warmup, args.streams)
+        print(f"[bench] fp32 batched done", file=sys.stderr, flush=True)
+        fp16m = timed(pw.PRECISION_FP16_MATH, B, args.steps, args.warmup, args.streams)
+        print(f"[bench] fp16_math batched done", file=sys.stderr, flush=True)
+
     us_fp16 = fp16["wall"] / (args.steps * B * args.streams) * 1e6
     us_fp16_one = fp16["wall_one"] / (args.steps * B) * 1e6
     us_fp32 = fp32["wall"] / (max(args.steps, 8) * 1) * 1e6
@@ -522,7 +536,8 @@ def main():
     # Quality and budget on one batch of frames: both paths decode frames 0..B-1.
     psnr = {}
     max_bytes = 0
-    legs = [(n, r) for n, r in (("fp16", fp16), ("fp32", fp32), ("fp16_ll", fp16_ll)) if r is not None]
+    legs = [(n, r) for n, r in (("fp16", fp16), ("fp32", fp32), ("fp16_ll", fp16_ll), ("fp32_batched", fp32b),
+                                ("fp16_math", fp16m)) if r is not None]
     for name, r in legs:
         encs, decs, outs = r["encs"], r["decs"], r["outs"]
         n = len(encs)
@@ -740,6 +755,14 @@ def main():
             "us_per_frame": round(fp16_ll["wall"] / (args.steps * B * args.streams) * 1e6, 2),
             "psnr_y_db": round(psnr["fp16_ll"], 4), "delta_psnr_db": round(psnr["fp32"] - psnr["fp16_ll"], 4),
             "stage_us_per_frame": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in fp16_ll["stages"].items()}},
+        "fp32_batched": None if fp32b is None else {
+            "us_per_frame": round(fp32b["wall"] / (args.steps * B * args.streams) * 1e6, 2),
+            "psnr_y_db": round(psnr["fp32_batched"], 4),
+            "stage_us_per_frame": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in fp32b["stages"].items()}},
+        "fp16_math_reference_precision0": None if fp16m is None else {
+            "us_per_frame": round(fp16m["wall"] / (args.steps * B * args.streams) * 1e6, 2),
+            "psnr_y_db": round(psnr["fp16_math"], 4), "delta_psnr_db": round(psnr["fp32"] - psnr["fp16_math"], 4),
+            "stage_us_per_frame": {k: round(v[0] * 1e3 / max(v[1], 1), 2) for k, v in fp16m["stages"].items()}},
         "frames_1080p_us_per_frame": f1080,
         "single_1080p_fp16_us_per_frame": single,
         "uneven_batch_32_fp16_us_per_batch": uneven,
