@@ -58,7 +58,15 @@ typedef enum pyrowave_hip_encoder_flag_bits
 	 * reproduced, whose 8 Hillis-Steele steps leave entries at flat index
 	 * >= 1024 holding 256-thread windowed sums
 	 * (reference shaders/analyze_rate_control_finalize.comp:244). */
-	PYROWAVE_HIP_ENCODER_EXACT_PREFIX_BIT = 1u << 0
+	PYROWAVE_HIP_ENCODER_EXACT_PREFIX_BIT = 1u << 0,
+	/* The input planes hold 16-bit samples (R16_UNORM: u / 65535, the
+	 * reference's format for 10- to 16-bit y4m input rescaled to the full
+	 * range, encode.cpp:152 and yuv4mpeg.cpp:221-233).  Plane data must be
+	 * 2-byte aligned, row and pixel strides even (pixel stride 2 for planar
+	 * data, 4 for an interleaved P016 CbCr plane).  Such an encoder takes
+	 * device planes only (pyrowave_hip_encode_batch): the CPU-buffer formats
+	 * of pyrowave.h are 8-bit. */
+	PYROWAVE_HIP_ENCODER_UNORM16_INPUT_BIT = 1u << 1
 } pyrowave_hip_encoder_flag_bits;
 
 typedef struct pyrowave_hip_encoder_create_info
@@ -71,6 +79,15 @@ typedef struct pyrowave_hip_encoder_create_info
 	uint32_t flags; /* pyrowave_hip_encoder_flag_bits */
 } pyrowave_hip_encoder_create_info;
 
+typedef enum pyrowave_hip_decoder_flag_bits
+{
+	/* The decode targets take 16-bit samples (R16_UNORM,
+	 * round(clamp(v + 0.5) * 65535): the reference decodes 16-bit streams
+	 * into R16_UNORM images, decode.cpp:215).  Target data 2-byte aligned,
+	 * row strides even.  Device targets only, as for 16-bit encoders. */
+	PYROWAVE_HIP_DECODER_UNORM16_OUTPUT_BIT = 1u << 0
+} pyrowave_hip_decoder_flag_bits;
+
 typedef struct pyrowave_hip_decoder_create_info
 {
 	pyrowave_device device;
@@ -78,6 +95,7 @@ typedef struct pyrowave_hip_decoder_create_info
 	int height;
 	pyrowave_chroma_subsampling chroma;
 	pyrowave_hip_precision precision;
+	uint32_t flags; /* pyrowave_hip_decoder_flag_bits */
 } pyrowave_hip_decoder_create_info;
 
 /* One 8-bit plane in device memory.  pixel_stride_in_bytes is 1 for planar

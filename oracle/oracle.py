@@ -95,6 +95,13 @@ def load_library(path):
     for name in ("or_idwt_level_f64", "or_idwt_level_f32"):
         getattr(L, name).argtypes = [P, P, P, P, C.c_int, C.c_int, C.c_int, P, C.c_int]
     L.or_dwt_frame.argtypes = [C.POINTER(Layout), P, P, C.c_int, P, P]
+    L.or_unorm_dc.restype = C.c_float
+    L.or_unorm_dc.argtypes = [C.c_uint32, C.c_int, C.c_int]
+    L.or_dwt_frame_ex.argtypes = [C.POINTER(Layout), P, P, C.c_int, C.c_int, P, P]
+    L.or_encode_ex.restype = C.c_long
+    L.or_encode_ex.argtypes = [C.POINTER(Layout), P, P, C.c_int, C.c_int, C.c_size_t, C.c_int, C.c_int,
+                               P, C.c_long, P, P]
+    L.or_idwt_frame_ex.argtypes = [C.POINTER(Layout), P, C.c_int, P, P, C.c_int]
     L.or_quant_analyze.argtypes = [C.POINTER(Layout), P, P, P, P, P, P, P]
     L.or_quant_analyze_ex.argtypes = [C.POINTER(Layout), P, P, P, P, P, P, P, P]
     L.or_block8_stats.argtypes = [C.POINTER(Layout), P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -167,10 +174,14 @@ def plane_shape(L, c):
 
 
 def _plane_ptrs(planes):
-    planes = [np.ascontiguousarray(p, dtype=np.uint8) for p in planes]
+    """uint16 planes are R16_UNORM samples, anything else UNORM8; returns
+    (planes, pointers, pitches in samples, bits)."""
+    bits = 16 if all(np.asarray(p).dtype == np.uint16 for p in planes) else 8
+    dt = np.uint16 if bits == 16 else np.uint8
+    planes = [np.ascontiguousarray(p, dtype=dt) for p in planes]
     arr = (C.c_void_p * 3)(*[p.ctypes.data for p in planes])
     pitches = (C.c_int * 3)(*[p.shape[1] for p in planes])
-    return planes, arr, pitches
+    return planes, arr, pitches, bits
 
 
 BLOCK_DESC = np.dtype([("component", np.int16), ("level", np.int16), ("band", np.int16), ("pad", np.int16),
@@ -188,14 +199,20 @@ def band_view(L, coeffs, c, l, b):
     return coeffs[B.coeff_off:B.coeff_off + B.bw * B.bh].reshape(B.bh, B.bw)
 
 
+def unorm_dc(u, bits, precision=2):
+    """The oracle's level-0 input value of sample u (float)."""
+    return float(lib().or_unorm_dc(int(u), bits, precision))
+
+
 def dwt_frame(L, planes, precision=2, f64=False):
-    planes, arr, pitches = _plane_ptrs(planes)
+    """Forward transform of u8 (UNORM8) or uint16 (R16_UNORM) planes."""
+    planes, arr, pitches, bits = _plane_ptrs(planes)
     if f64:
         c64 = np.zeros(L.coeff_count, dtype=np.float64)
-        lib().or_dwt_frame(C.byref(L), arr, pitches, precision, None, ptr(c64))
+        lib().or_dwt_frame_ex(C.byref(L), arr, pitches, bits, precision, None, ptr(c64))
         return c64
     c32 = np.zeros(L.coeff_count, dtype=np.float32)
-    lib().or_dwt_frame(C.byref(L), arr, pitches, precision, ptr(c32), None)
+    lib().or_dwt_frame_ex(C.byref(L), arr, pitches, bits, precision, ptr(c32), None)
     return c32
 
 
@@ -269,13 +286,14 @@ def pack(L, coeffs, quant, sequence, capacity_words=None):
 
 
 def encode(L, planes, budget, sequence=1, precision=2, q1=True):
-    planes, arr, pitches = _plane_ptrs(planes)
+    """Encode u8 (UNORM8) or uint16 (R16_UNORM) planes."""
+    planes, arr, pitches, bits = _plane_ptrs(planes)
     cap = L.count32 * 600 + 16
     bs = np.zeros(cap, dtype=np.uint32)
     meta = np.zeros((L.count32, 2), dtype=np.uint32)
     quant = np.zeros(L.count32, dtype=np.int32)
-    n = lib().or_encode(C.byref(L), arr, pitches, precision, budget, sequence, 1 if q1 else 0,
-                        ptr(bs), cap, ptr(meta), ptr(quant))
+    n = lib().or_encode_ex(C.byref(L), arr, pitches, bits, precision, budget, sequence, 1 if q1 else 0,
+                           ptr(bs), cap, ptr(meta), ptr(quant))
     if n < 0:
         raise RuntimeError("bitstream capacity exceeded")
     return bs[:n].copy(), meta, quant
@@ -351,12 +369,13 @@ def dequant(L, payload, offsets):
     return coeffs
 
 
-def idwt_frame(L, coeffs, precision=2):
+def idwt_frame(L, coeffs, precision=2, bits=8):
+    """Inverse transform into u8 planes, or uint16 (R16_UNORM) with bits=16."""
     coeffs = np.ascontiguousarray(coeffs, np.float32)
-    planes = [np.zeros(plane_shape(L, c), dtype=np.uint8) for c in range(3)]
+    planes = [np.zeros(plane_shape(L, c), dtype=np.uint16 if bits == 16 else np.uint8) for c in range(3)]
     arr = (C.c_void_p * 3)(*[p.ctypes.data for p in planes])
     pitches = (C.c_int * 3)(*[p.shape[1] for p in planes])
-    lib().or_idwt_frame(C.byref(L), ptr(coeffs), precision, arr, pitches)
+    lib().or_idwt_frame_ex(C.byref(L), ptr(coeffs), precision, arr, pitches, bits)
     return planes
 
 
@@ -369,14 +388,15 @@ def idwt_frame_f64(L, coeffs64):
     return planes
 
 
-def decode(L, packets_bytes, precision=2):
-    """Push every packet, dequantize and inverse transform: u8 planes."""
+def decode(L, packets_bytes, precision=2, bits=8):
+    """Push every packet, dequantize and inverse transform: u8 planes (uint16
+    with bits=16)."""
     d = Decoder(L)
     for p in packets_bytes:
         if not d.push(p):
             raise RuntimeError("push_packet failed")
     coeffs = dequant(L, d.payload(), d.offsets())
-    return idwt_frame(L, coeffs, precision)
+    return idwt_frame(L, coeffs, precision, bits)
 
 
 def dwt_level(in_arr, R_x, R_y, A_x, A_y, f64=True):

@@ -760,6 +760,27 @@ static float unorm8_dc_h(uint8_t u)
 	return h_add(or_f16_to_f32(or_f32_to_f16((float)u / 255.0f)), -0.5f);
 }
 
+static float unorm16_dc(uint16_t u)
+{
+	/* R16_UNORM texel (u / 65535, the quotient rounded to float once) with
+	 * the level-0 DC shift: encode.cpp:152 samples 16-bit y4m input (rescaled
+	 * to the full 16-bit range, yuv4mpeg.cpp:221-233) through an R16_UNORM
+	 * image with the same dwt.comp:44-48 shift as UNORM8. */
+	return (float)u / 65535.0f - 0.5f;
+}
+
+static float unorm16_dc_h(uint16_t u)
+{
+	return h_add(or_f16_to_f32(or_f32_to_f16((float)u / 65535.0f)), -0.5f);
+}
+
+float or_unorm_dc(uint32_t u, int bits, int precision)
+{
+	if (bits == 16)
+		return precision == 0 ? unorm16_dc_h((uint16_t)u) : unorm16_dc((uint16_t)u);
+	return precision == 0 ? unorm8_dc_h((uint8_t)u) : unorm8_dc((uint8_t)u);
+}
+
 static void plane_dims(const or_layout *L, int c, int *w, int *h)
 {
 	*w = L->width;
@@ -773,6 +794,13 @@ static void plane_dims(const or_layout *L, int c, int *w, int *h)
 
 int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                  int precision, float *coeffs, double *coeffs64)
+{
+	const void *p[3] = {planes[0], planes[1], planes[2]};
+	return or_dwt_frame_ex(L, p, pitches, 8, precision, coeffs, coeffs64);
+}
+
+int or_dwt_frame_ex(const or_layout *L, const void *const planes[3], const int pitches[3], int bits,
+                    int precision, float *coeffs, double *coeffs64)
 {
 	int use_f64 = coeffs64 != NULL;
 	for (int c = 0; c < OR_COMPONENTS; c++)
@@ -799,13 +827,14 @@ int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int p
 				for (int y = 0; y < ph; y++)
 					for (int x = 0; x < pw; x++)
 					{
-						uint8_t u = planes[c][(size_t)y * pitches[c] + x];
+						const size_t at = (size_t)y * pitches[c] + x;
+						const size_t to = (size_t)y * pw + x;
+						const uint32_t u = bits == 16 ? ((const uint16_t *)planes[c])[at]
+						                              : ((const uint8_t *)planes[c])[at];
 						if (use_f64)
-							in64[(size_t)y * pw + x] = (double)u / 255.0 - 0.5;
-						else if (precision == 0)
-							in32[(size_t)y * pw + x] = unorm8_dc_h(u);
+							in64[to] = (double)u / (bits == 16 ? 65535.0 : 255.0) - 0.5;
 						else
-							in32[(size_t)y * pw + x] = unorm8_dc(u);
+							in32[to] = or_unorm_dc(u, bits, precision);
 					}
 			}
 			else
@@ -1475,6 +1504,15 @@ long or_encode(const or_layout *L, const uint8_t *const planes[3], const int pit
                int precision, size_t budget, int sequence, int q1,
                uint32_t *bitstream, long capacity_words, uint32_t *meta, int32_t *quant_out)
 {
+	const void *p[3] = {planes[0], planes[1], planes[2]};
+	return or_encode_ex(L, p, pitches, 8, precision, budget, sequence, q1, bitstream, capacity_words, meta,
+	                    quant_out);
+}
+
+long or_encode_ex(const or_layout *L, const void *const planes[3], const int pitches[3], int bits,
+                  int precision, size_t budget, int sequence, int q1,
+                  uint32_t *bitstream, long capacity_words, uint32_t *meta, int32_t *quant_out)
+{
 	float *coeffs = (float *)calloc((size_t)L->coeff_count, sizeof(float));
 	uint16_t *words = (uint16_t *)malloc(sizeof(uint16_t) * 16 * (size_t)L->count32);
 	uint8_t *bucket = (uint8_t *)malloc(16 * (size_t)L->count32);
@@ -1488,7 +1526,7 @@ long or_encode(const or_layout *L, const uint8_t *const planes[3], const int pit
 		L0.precision0 = 1;
 	L = &L0;
 
-	or_dwt_frame(L, planes, pitches, precision, coeffs, NULL);
+	or_dwt_frame_ex(L, planes, pitches, bits, precision, coeffs, NULL);
 	or_quant_analyze(L, coeffs, words, bucket, code8, np8, totals, &consumed);
 	or_finalize(totals, prefix, q1);
 	or_resolve(L, words, bucket, prefix, consumed, budget, quant);
@@ -1936,8 +1974,28 @@ static uint8_t unorm8_store(float v)
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
+static uint16_t unorm16_store(float v)
+{
+	/* R16_UNORM image store (idwt.comp:201-213 into an R16_UNORM target,
+	 * decode.cpp:215): +0.5 DC shift, clamp to [0,1], round to nearest (ties
+	 * up), as the UNORM8 store. */
+	v = v + 0.5f;
+	if (!(v > 0.0f))
+		v = 0.0f;
+	if (v > 1.0f)
+		v = 1.0f;
+	return (uint16_t)(int)fmaf(v, 65535.0f, 0.5f);
+}
+
 int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
                   uint8_t *const planes[3], const int pitches[3])
+{
+	void *p[3] = {planes[0], planes[1], planes[2]};
+	return or_idwt_frame_ex(L, coeffs_in, precision, p, pitches, 8);
+}
+
+int or_idwt_frame_ex(const or_layout *L, const float *coeffs_in, int precision,
+                     void *const planes[3], const int pitches[3], int bits)
 {
 	float *coeffs = (float *)malloc(sizeof(float) * (size_t)L->coeff_count);
 	memcpy(coeffs, coeffs_in, sizeof(float) * (size_t)L->coeff_count);
@@ -1971,7 +2029,14 @@ int or_idwt_frame(const or_layout *L, const float *coeffs_in, int precision,
 				      coeffs + B[3].coeff_off, bw, bw, bh, tmp, 2 * bw);
 				for (int y = 0; y < ph && y < 2 * bh; y++)
 					for (int x = 0; x < pw && x < 2 * bw; x++)
-						planes[c][(size_t)y * pitches[c] + x] = unorm8_store(tmp[(size_t)y * 2 * bw + x]);
+					{
+						const float v = tmp[(size_t)y * 2 * bw + x];
+						const size_t at = (size_t)y * pitches[c] + x;
+						if (bits == 16)
+							((uint16_t *)planes[c])[at] = unorm16_store(v);
+						else
+							((uint8_t *)planes[c])[at] = unorm8_store(v);
+					}
 				free(tmp);
 			}
 			else

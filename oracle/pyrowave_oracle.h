@@ -153,6 +153,16 @@ void or_idwt_level_h(const float *ll, const float *hl, const float *lh, const fl
  * use_f64 != 0: FP64 math; result written to coeffs64 (may be NULL otherwise). */
 int or_dwt_frame(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                  int precision, float *coeffs, double *coeffs64);
+/* As or_dwt_frame with input planes of `bits` 8 (UNORM8, u / 255) or 16
+ * (R16_UNORM, u / 65535: encode.cpp:152); pitches in samples.  A 16-bit
+ * sample 257 * u is the same rational as the 8-bit u, so such planes give
+ * the 8-bit transform exactly. */
+/* The level-0 input sample: UNORM8 u / 255 or R16_UNORM u / 65535 (bits
+ * 8 / 16), rounded to float, minus 0.5 in float; precision 0 rounds the
+ * texel to fp16 and subtracts in fp16. */
+float or_unorm_dc(uint32_t u, int bits, int precision);
+int or_dwt_frame_ex(const or_layout *L, const void *const planes[3], const int pitches[3], int bits,
+                    int precision, float *coeffs, double *coeffs64);
 
 /* Quantizer + analyze (wavelet_quant.comp, analyze_rate_control.comp).
  * Outputs, per 32x32 block b: words[b*16+q], bucket[b*16+q] (inclusive bucket,
@@ -200,6 +210,9 @@ long or_pack(const or_layout *L, const float *coeffs, const int32_t *quant, int 
 long or_encode(const or_layout *L, const uint8_t *const planes[3], const int pitches[3],
                int precision, size_t budget, int sequence, int q1,
                uint32_t *bitstream, long capacity_words, uint32_t *meta, int32_t *quant_out);
+long or_encode_ex(const or_layout *L, const void *const planes[3], const int pitches[3], int bits,
+                  int precision, size_t budget, int sequence, int q1,
+                  uint32_t *bitstream, long capacity_words, uint32_t *meta, int32_t *quant_out);
 
 /* Encoder::Impl::compute_num_packets / packetize (pyrowave_encoder.cpp:596-623, 1048-1115). */
 size_t or_compute_num_packets(const or_layout *L, const uint32_t *meta, size_t boundary);
@@ -235,6 +248,9 @@ int or_dequant(const or_layout *L, const uint32_t *payload, long payload_words,
  * precision 0 rounds every input band to fp16 and runs or_idwt_level_h. */
 int or_idwt_frame(const or_layout *L, const float *coeffs, int precision,
                   uint8_t *const planes[3], const int pitches[3]);
+/* bits 16: R16_UNORM output planes (decode.cpp:215), round(clamp(v + 0.5) * 65535). */
+int or_idwt_frame_ex(const or_layout *L, const float *coeffs, int precision,
+                     void *const planes[3], const int pitches[3], int bits);
 int or_idwt_frame_f64(const or_layout *L, const double *coeffs, double *const planes[3], const int pitches[3]);
 
 /* PSNR-HVS-M-H (psnr_hvs_m.c; reference shaders/psnr_hvs_m.comp, psnr.cpp). */

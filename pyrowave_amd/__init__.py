@@ -40,6 +40,8 @@ PRECISION_FP32 = 2
 PRECISION_FP16_LL = 3
 
 EXACT_PREFIX = 1
+UNORM16_INPUT = 2     # encoder flag: 16-bit (R16_UNORM) input planes
+UNORM16_OUTPUT = 1    # decoder flag: 16-bit (R16_UNORM) decode targets
 
 VK_QUEUE_GRAPHICS_BIT = 1
 VK_QUEUE_COMPUTE_BIT = 2
@@ -69,7 +71,7 @@ class HipEncoderCreateInfo(C.Structure):
 
 class HipDecoderCreateInfo(C.Structure):
     _fields_ = [("device", C.c_void_p), ("width", C.c_int), ("height", C.c_int), ("chroma", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("flags", C.c_uint32)]
 
 
 class CpuBuffer(C.Structure):
@@ -431,14 +433,14 @@ class Encoder:
 
 
 class Decoder:
-    def __init__(self, device, width, height, chroma=CHROMA_420, precision=None, fragment_path=False):
+    def __init__(self, device, width, height, chroma=CHROMA_420, precision=None, fragment_path=False, flags=0):
         self.device, self.width, self.height, self.chroma = device, width, height, chroma
         h = C.c_void_p()
         if precision is None:
             info = DecoderCreateInfo(device.handle if device else None, width, height, chroma, fragment_path)
             code = load().pyrowave_decoder_create(C.byref(info), C.byref(h))
         else:
-            info = HipDecoderCreateInfo(device.handle if device else None, width, height, chroma, precision)
+            info = HipDecoderCreateInfo(device.handle if device else None, width, height, chroma, precision, flags)
             code = load().pyrowave_hip_decoder_create(C.byref(info), C.byref(h))
         _check(code, "pyrowave_decoder_create")
         self.handle = h

@@ -61,10 +61,12 @@ void fill_encode_call(CallParams &F, Slot &S, const pyrowave_hip_frame &frame, s
 {
 	for (int c = 0; c < 3; c++)
 	{
+		// Pitch and step in samples.
 		const pyrowave_hip_plane &p = frame.planes[c];
+		const size_t sb = size_t(S.geom.sample_bytes);
 		F.src[c] = static_cast<const uint8_t *>(p.data);
-		F.src_pitch[c] = int32_t(p.row_stride_in_bytes);
-		F.src_step[c] = int32_t(p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes : 1);
+		F.src_pitch[c] = int32_t(p.row_stride_in_bytes / sb);
+		F.src_step[c] = int32_t(p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes / sb : 1);
 	}
 	set_encode_params(F, S, budget);
 }
@@ -81,7 +83,7 @@ void fill_decode_call(CallParams &F, Slot &S, const Slot *E, const pyrowave_hip_
 	for (int c = 0; c < 3; c++)
 	{
 		F.dst[c] = static_cast<uint8_t *>(target.planes[c].data);
-		F.dst_pitch[c] = int32_t(target.planes[c].row_stride_in_bytes);
+		F.dst_pitch[c] = int32_t(target.planes[c].row_stride_in_bytes / size_t(S.geom.sample_bytes));
 	}
 	S.decoded_frame = true;
 }
@@ -352,6 +354,9 @@ pyrowave_result validate_cpu_buffer(const Geometry &G, int width, int height, co
 	if (buffers->width != width || buffers->height != height)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	if (decode && buffers->format == PYROWAVE_CPU_BUFFER_FORMAT_NV12)
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	// The CPU-buffer formats are 8-bit.
+	if (G.sample_bytes != 1)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	if (G.chroma == PYROWAVE_CHROMA_SUBSAMPLING_420 && buffers->format == PYROWAVE_CPU_BUFFER_FORMAT_YUV444P)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
@@ -673,6 +678,7 @@ static pyrowave_result create_encoder(pyrowave_device device, int width, int hei
 	e->slot.exact_prefix = (flags & PYROWAVE_HIP_ENCODER_EXACT_PREFIX_BIT) != 0;
 	if (!e->slot.init(&device->dev, width, height, int(chroma), precision, true))
 		return PYROWAVE_ERROR_GENERIC;
+	e->slot.geom.sample_bytes = (flags & PYROWAVE_HIP_ENCODER_UNORM16_INPUT_BIT) ? 2 : 1;
 	device->dev.live_objects++;
 	*encoder = e.release();
 	return PYROWAVE_SUCCESS;
@@ -875,7 +881,7 @@ PYROWAVE_PUBLIC_API void pyrowave_encoder_destroy(pyrowave_encoder encoder)
 }
 
 static pyrowave_result create_decoder(pyrowave_device device, int width, int height, pyrowave_chroma_subsampling chroma,
-                                      int precision, pyrowave_decoder *decoder)
+                                      int precision, uint32_t flags, pyrowave_decoder *decoder)
 {
 	// reference pyrowave_c.cpp:1198-1226 (init failure is INVALID_ARGUMENT).
 	if (!device || !decoder)
@@ -893,6 +899,7 @@ static pyrowave_result create_decoder(pyrowave_device device, int width, int hei
 	d->device = device;
 	if (!d->slot.init(&device->dev, width, height, int(chroma), precision, false))
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	d->slot.geom.sample_bytes = (flags & PYROWAVE_HIP_DECODER_UNORM16_OUTPUT_BIT) ? 2 : 1;
 	// The single-frame launch tables up front, so that decoding one frame
 	// (pyrowave_hip_decode_packets_noalloc in particular) allocates nothing.
 	if (!get_plan(&device->dev, {&d->slot}, false))
@@ -913,7 +920,7 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_decoder_create(const pyrowave_decod
 {
 	if (!info)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	return create_decoder(info->device, info->width, info->height, info->chroma, default_precision(), decoder);
+	return create_decoder(info->device, info->width, info->height, info->chroma, default_precision(), 0, decoder);
 }
 
 PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decoder_create(const pyrowave_hip_decoder_create_info *info,
@@ -921,7 +928,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decoder_create(const pyrowave_h
 {
 	if (!info)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
-	return create_decoder(info->device, info->width, info->height, info->chroma, int(info->precision), decoder);
+	return create_decoder(info->device, info->width, info->height, info->chroma, int(info->precision), info->flags,
+	                      decoder);
 }
 
 PYROWAVE_PUBLIC_API void pyrowave_decoder_clear(pyrowave_decoder decoder)
@@ -1034,14 +1042,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encode_batch(pyrowave_encoder *
 		for (uint32_t j = 0; j < i; j++)
 			if (slots[j] == slots[i])
 				return PYROWAVE_ERROR_INVALID_ARGUMENT;
-		const Geometry &G = slots[i]->geom;
-		for (int c = 0; c < 3; c++)
-		{
-			const pyrowave_hip_plane &p = frames[i].planes[c];
-			size_t step = p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes : 1;
-			if (!p.data || p.row_stride_in_bytes < (size_t(G.plane_w(c)) - 1) * step + 1)
-				return PYROWAVE_ERROR_INVALID_ARGUMENT;
-		}
+		if (!frame_fits(slots[i]->geom, frames[i]))
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	}
 	Device &D = *slots[0]->dev;
 	std::lock_guard<std::recursive_mutex> g(D.lock);
@@ -1100,9 +1102,8 @@ static pyrowave_result decode_batch_common(pyrowave_decoder *decoders, pyrowave_
 			if (slots[j] == slots[i])
 				return PYROWAVE_ERROR_INVALID_ARGUMENT;
 		const Geometry &G = slots[i]->geom;
-		for (int c = 0; c < 3; c++)
-			if (!targets[i].planes[c].data || targets[i].planes[c].row_stride_in_bytes < size_t(G.plane_w(c)))
-				return PYROWAVE_ERROR_INVALID_ARGUMENT;
+		if (!target_fits(G, targets[i]))
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
 		if (encoders)
 		{
 			if (!encoders[i] || !encoders[i]->slot.have_result)
@@ -1260,9 +1261,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_batch(pyrowave_d
 			if (slots[j] == slots[i])
 				return PYROWAVE_ERROR_INVALID_ARGUMENT;
 		const Geometry &G = slots[i]->geom;
-		for (int c = 0; c < 3; c++)
-			if (!targets[i].planes[c].data || targets[i].planes[c].row_stride_in_bytes < size_t(G.plane_w(c)))
-				return PYROWAVE_ERROR_INVALID_ARGUMENT;
+		if (!target_fits(G, targets[i]))
+			return PYROWAVE_ERROR_INVALID_ARGUMENT;
 		const pyrowave_hip_packet_stream &P = packets[i];
 		if (!P.data || (reinterpret_cast<uintptr_t>(P.data) & 3) || P.data_bytes > UINT32_MAX || !P.packets ||
 		    !P.count)
@@ -1334,9 +1334,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_decode_packets_noalloc(pyrowave
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	Slot &S = decoder->slot;
 	const Geometry &G = S.geom;
-	for (int c = 0; c < 3; c++)
-		if (!target->planes[c].data || target->planes[c].row_stride_in_bytes < size_t(G.plane_w(c)))
-			return PYROWAVE_ERROR_INVALID_ARGUMENT;
+	if (!target_fits(G, *target))
+		return PYROWAVE_ERROR_INVALID_ARGUMENT;
 	const pyrowave_hip_packet_stream &P = *packets;
 	if (!P.data || (reinterpret_cast<uintptr_t>(P.data) & 3) || P.data_bytes > UINT32_MAX || !P.packets || !P.count)
 		return PYROWAVE_ERROR_INVALID_ARGUMENT;

@@ -70,7 +70,8 @@ enum StorageKind : int32_t
 {
 	kStoreF32 = 0,
 	kStoreF16 = 1,
-	kStoreU8 = 2
+	kStoreU8 = 2,
+	kStoreU16 = 3 // R16_UNORM planes (16-bit input / output)
 };
 
 // Per-(bucket, subdivision) resolve class.
@@ -115,14 +116,15 @@ struct FrameState
 	uint8_t *cls;           // [2048] RcClass
 	int32_t *required;      // [2048] savings still required at each entry
 	uint32_t *rc;           // [kRcCount] RcWord
-	// Per-call encoder inputs: u8 planes (device pointers), sequence counter
-	// and word budget for the coded blocks.
+	// Per-call encoder inputs: u8 or u16 planes (device pointers; pitch and
+	// step in samples), sequence counter and word budget for the coded blocks.
 	const uint8_t *src[3];
 	int32_t src_pitch[3];
 	int32_t src_step[3];
 	uint32_t sequence;
 	int32_t target_words;
-	// Per-call decoder outputs: u8 planes (device pointers).
+	// Per-call decoder outputs: u8 or u16 planes (device pointers; pitch in
+	// samples).
 	uint8_t *dst[3];
 	int32_t dst_pitch[3];
 	// Decoder input: offsets (u32 per block, or meta pairs) and payload.

@@ -24,6 +24,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <type_traits>
+
 #pragma clang fp contract(off)
 
 namespace pw
@@ -187,6 +189,11 @@ struct Raw4<__half>
 	uint2 w;
 };
 template <>
+struct Raw4<uint16_t>
+{
+	uint2 w;
+};
+template <>
 struct Raw4<float>
 {
 	float4 w;
@@ -215,6 +222,15 @@ __device__ __forceinline__ Raw4<T> load4_gather(const T *row, const int *xmap, i
 #pragma unroll
 		for (int i = 0; i < 4; i++)
 			r.w |= (uint32_t)row[__umul24((uint32_t)xmap[c + i], step)] << (8 * i);
+	}
+	else if constexpr (std::is_same<T, uint16_t>::value)
+	{
+		uint32_t u[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			u[i] = row[__umul24((uint32_t)xmap[c + i], step)];
+		r.w.x = u[0] | (u[1] << 16);
+		r.w.y = u[2] | (u[3] << 16);
 	}
 	else if constexpr (sizeof(T) == 2)
 	{
@@ -246,6 +262,21 @@ __device__ __forceinline__ float u8_dc(uint32_t u)
 	return fmaf(fu, hi, fu * lo) - 0.5f;
 }
 
+// u / 65535 - 0.5 the same way: hi + lo = 1/65535 split into two floats,
+// fmaf(u, hi, u * lo) rounds to float(u / 65535) for every u in 0..65535
+// (checked exhaustively with exact rationals), then the DC shift.
+__device__ __forceinline__ float u16_unorm(uint32_t u)
+{
+	const float fu = (float)u;
+	const float hi = __uint_as_float(0x37800080u), lo = __uint_as_float(0x27800080u);
+	return fmaf(fu, hi, fu * lo);
+}
+
+__device__ __forceinline__ float u16_dc(uint32_t u)
+{
+	return u16_unorm(u) - 0.5f;
+}
+
 template <typename T>
 __device__ __forceinline__ void unpack4(const Raw4<T> &r, const float *lut, float (&f)[4])
 {
@@ -254,6 +285,13 @@ __device__ __forceinline__ void unpack4(const Raw4<T> &r, const float *lut, floa
 #pragma unroll
 		for (int i = 0; i < 4; i++)
 			f[i] = lut ? lut[(r.w >> (8 * i)) & 0xffu] : u8_dc((r.w >> (8 * i)) & 0xffu);
+	}
+	else if constexpr (std::is_same<T, uint16_t>::value)
+	{
+		f[0] = u16_dc(r.w.x & 0xffffu);
+		f[1] = u16_dc(r.w.x >> 16);
+		f[2] = u16_dc(r.w.y & 0xffffu);
+		f[3] = u16_dc(r.w.y >> 16);
 	}
 	else if constexpr (sizeof(T) == 2)
 	{
@@ -354,6 +392,14 @@ __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict_
 		const bool vec_ok = step == 1 && ((reinterpret_cast<uintptr_t>(src) | (uintptr_t)pitch) & 3) == 0;
 		fwd_stage<S>(src, pitch, step, vec_ok, x0, J.R_x, s_ymap, s_xmap, s_lut, tid, sA);
 	}
+	else if (J.src_kind == kStoreU16)
+	{
+		const FrameState &F = frames[J.frame];
+		const uint16_t *src = reinterpret_cast<const uint16_t *>(F.src[J.component]);
+		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
+		const bool vec_ok = step == 1 && ((reinterpret_cast<uintptr_t>(src) | (2u * (uintptr_t)pitch)) & 7) == 0;
+		fwd_stage<S>(src, pitch, step, vec_ok, x0, J.R_x, s_ymap, s_xmap, s_lut, tid, sA);
+	}
 	else if (J.src_kind == kStoreF16)
 		fwd_stage<S>(static_cast<const __half *>(J.src), (uint32_t)J.src_pitch, 1u, true, x0, J.R_x, s_ymap, s_xmap,
 		             s_lut, tid, sA);
@@ -432,6 +478,41 @@ __device__ __forceinline__ uint8_t unorm8_store(float v)
 	return (uint8_t)(int)fmaf(v, 255.0f, 0.5f);
 }
 
+__device__ __forceinline__ uint16_t unorm16_store(float v)
+{
+	// The same into an R16_UNORM target (reference decode.cpp:215).
+	v = __builtin_amdgcn_fmed3f(v + 0.5f, 0.0f, 1.0f);
+	return (uint16_t)(int)fmaf(v, 65535.0f, 0.5f);
+}
+
+// Output planes of the last inverse level: u8 (R8_UNORM) or u16 (R16_UNORM).
+__device__ __forceinline__ bool plane_kind(int kind)
+{
+	return kind == kStoreU8 || kind == kStoreU16;
+}
+
+// Two horizontally adjacent plane samples (px, px + 1) of row py.
+__device__ __forceinline__ void store_plane_pair(uint8_t *plane, int kind, long dpitch, int py, int px, int R_x,
+                                                 float e, float o)
+{
+	if (kind == kStoreU16)
+	{
+		uint16_t *d = reinterpret_cast<uint16_t *>(plane) + (long)py * dpitch + px;
+		if (px < R_x)
+			d[0] = unorm16_store(e);
+		if (px + 1 < R_x)
+			d[1] = unorm16_store(o);
+	}
+	else
+	{
+		uint8_t *d = plane + (long)py * dpitch + px;
+		if (px < R_x)
+			d[0] = unorm8_store(e);
+		if (px + 1 < R_x)
+			d[1] = unorm8_store(o);
+	}
+}
+
 // Loads the four bands' (TW/2 + 4) x (TH/2 + 4) shares of the interleaved
 // (TW + 8) x (TH + 8) tile
 // (band b holds the positions whose (row, column) parity is (b >> 1, b & 1))
@@ -495,6 +576,30 @@ __device__ __forceinline__ void load_bands(const DwtJob &J, const int *ymap, con
 				const float x = S::kPrescale ? vals[b][k] * ((b >> 1) ? kInvK : kK) : vals[b][k];
 				sA[(2 * rr + (b >> 1)) * S::kPitch + 2 * cc + (b & 1)] = x;
 			}
+		}
+	}
+}
+
+template <typename S>
+__device__ __forceinline__ void store_u16_run(uint16_t *dst, int n_valid, const float *v)
+{
+	// v[0 .. kOut) to dst, 8-byte stores where aligned and whole.
+#pragma unroll
+	for (int c = 0; c < S::kOut / 4; c++)
+	{
+		uint16_t *d = dst + 4 * c;
+		if (4 * c + 4 <= n_valid && ((uintptr_t)d & 7) == 0)
+		{
+			const uint32_t lo = (uint32_t)unorm16_store(v[4 * c]) | ((uint32_t)unorm16_store(v[4 * c + 1]) << 16);
+			const uint32_t hi = (uint32_t)unorm16_store(v[4 * c + 2]) | ((uint32_t)unorm16_store(v[4 * c + 3]) << 16);
+			*reinterpret_cast<uint2 *>(d) = make_uint2(lo, hi);
+		}
+		else
+		{
+#pragma unroll
+			for (int j = 0; j < 4; j++)
+				if (4 * c + j < n_valid)
+					d[j] = unorm16_store(v[4 * c + j]);
 		}
 	}
 }
@@ -607,7 +712,7 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 	// Horizontal pass: TW / kOut lanes per row write TW consecutive outputs.
 	uint8_t *plane = nullptr;
 	long dpitch = J.dst_pitch;
-	if (J.dst_kind == kStoreU8)
+	if (plane_kind(J.dst_kind))
 	{
 		plane = frames[J.frame].dst[J.component];
 		dpitch = frames[J.frame].dst_pitch[J.component];
@@ -633,6 +738,11 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 		{
 			if (py < J.R_y)
 				store_u8_run<S>(plane + (long)py * dpitch + px0, J.R_x - px0, v + 4);
+		}
+		else if (J.dst_kind == kStoreU16)
+		{
+			if (py < J.R_y)
+				store_u16_run<S>(reinterpret_cast<uint16_t *>(plane) + (long)py * dpitch + px0, J.R_x - px0, v + 4);
 		}
 		else if (py < J.A_y)
 		{
@@ -745,7 +855,7 @@ __device__ __forceinline__ StreamSrc stream_src(const void *p)
 template <int KIND>
 constexpr uint32_t stream_esize()
 {
-	return KIND == kStoreU8 ? 1u : KIND == kStoreF16 ? 2u : 4u;
+	return KIND == kStoreU8 ? 1u : (KIND == kStoreF16 || KIND == kStoreU16) ? 2u : 4u;
 }
 
 // Every format is fetched as the aligned 32-bit word holding the sample (a
@@ -767,6 +877,8 @@ __device__ __forceinline__ float stream_value(uint32_t raw, const StreamSrc &S, 
 	const uint32_t a = S.lo + off * stream_esize<KIND>();
 	if constexpr (KIND == kStoreU8)
 		return u8_dc((raw >> (8 * (a & 3))) & 0xffu);
+	else if constexpr (KIND == kStoreU16)
+		return u16_dc((raw >> (8 * (a & 2))) & 0xffffu);
 	else if constexpr (KIND == kStoreF16)
 		return __half2float(__ushort_as_half((unsigned short)(raw >> (8 * (a & 2)))));
 	else
@@ -888,6 +1000,12 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const 
 		const FrameState &F = frames[J.frame];
 		forward_stream_tile<kStoreU8>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
 		                              (uint32_t)F.src_step[J.component], x0, y0, lane);
+	}
+	else if (J.src_kind == kStoreU16)
+	{
+		const FrameState &F = frames[J.frame];
+		forward_stream_tile<kStoreU16>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
+		                               (uint32_t)F.src_step[J.component], x0, y0, lane);
 	}
 	else if (J.src_kind == kStoreF16)
 		forward_stream_tile<kStoreF16>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
@@ -1144,7 +1262,7 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 
 	uint8_t *plane = nullptr;
 	long dpitch = J.dst_pitch;
-	if (J.dst_kind == kStoreU8)
+	if (plane_kind(J.dst_kind))
 	{
 		plane = frames[J.frame].dst[J.component];
 		dpitch = frames[J.frame].dst_pitch[J.component];
@@ -1178,16 +1296,10 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 	auto store_row = [&](int py, float e, float o) {
 		if (!lane_out)
 			return;
-		if (J.dst_kind == kStoreU8)
+		if (plane_kind(J.dst_kind))
 		{
 			if (py < J.R_y)
-			{
-				uint8_t *d = plane + (long)py * dpitch + px;
-				if (px < J.R_x)
-					d[0] = unorm8_store(e);
-				if (px + 1 < J.R_x)
-					d[1] = unorm8_store(o);
-			}
+				store_plane_pair(plane, J.dst_kind, dpitch, py, px, J.R_x, e, o);
 		}
 		else if (py < J.A_y)
 		{
@@ -1335,6 +1447,18 @@ __device__ __forceinline__ _Float16 stream_half(uint32_t raw, const StreamSrc &S
 	const uint32_t a = S.lo + off * stream_esize<KIND>();
 	if constexpr (KIND == kStoreU8)
 		return __builtin_bit_cast(_Float16, c_u8_dc_h[(raw >> (8 * (a & 3))) & 0xffu]);
+	else if constexpr (KIND == kStoreU16)
+	{
+		// fp16(fp16(float(u / 65535)) - 0.5): three roundings.  The barriers
+		// keep the compiler from merging the float quotient's fma with its
+		// conversion (v_fma_mix rounds once, straight to fp16), or the
+		// conversion with the fp16 subtraction.
+		float tf = u16_unorm((raw >> (8 * (a & 2))) & 0xffffu);
+		asm volatile("" : "+v"(tf));
+		_Float16 t = (_Float16)tf;
+		asm volatile("" : "+v"(t));
+		return t - (_Float16)0.5f;
+	}
 	else
 		return __builtin_bit_cast(_Float16, (uint16_t)(raw >> (8 * (a & 2))));
 }
@@ -1465,6 +1589,12 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream_h(cons
 		forward_stream_tile_h<kStoreU8>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
 		                                (uint32_t)F.src_step[J.component], x0, y0, lane);
 	}
+	else if (J.src_kind == kStoreU16)
+	{
+		const FrameState &F = frames[J.frame];
+		forward_stream_tile_h<kStoreU16>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
+		                                 (uint32_t)F.src_step[J.component], x0, y0, lane);
+	}
 	else
 		forward_stream_tile_h<kStoreF16>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 }
@@ -1513,7 +1643,7 @@ __device__ __forceinline__ void inverse_stream_tile_h(const DwtJob &J, const Fra
 
 	uint8_t *plane = nullptr;
 	long dpitch = J.dst_pitch;
-	if (J.dst_kind == kStoreU8)
+	if (plane_kind(J.dst_kind))
 	{
 		plane = frames[J.frame].dst[J.component];
 		dpitch = frames[J.frame].dst_pitch[J.component];
@@ -1536,16 +1666,10 @@ __device__ __forceinline__ void inverse_stream_tile_h(const DwtJob &J, const Fra
 	auto store_row = [&](int py, _Float16 e, _Float16 o) {
 		if (!lane_out)
 			return;
-		if (J.dst_kind == kStoreU8)
+		if (plane_kind(J.dst_kind))
 		{
 			if (py < J.R_y)
-			{
-				uint8_t *d = plane + (long)py * dpitch + px;
-				if (px < J.R_x)
-					d[0] = unorm8_store((float)e);
-				if (px + 1 < J.R_x)
-					d[1] = unorm8_store((float)o);
-			}
+				store_plane_pair(plane, J.dst_kind, dpitch, py, px, J.R_x, (float)e, (float)o);
 		}
 		else if (py < J.A_y)
 		{

@@ -505,7 +505,8 @@ bool Slot::decode_is_ready(bool allow_partial) const
 
 static bool fusable(const Geometry &G, int c)
 {
-	return G.plane_w(c) >= 13 && G.plane_h(c) >= 13;
+	// The fused launch reads 8-bit planes only.
+	return G.plane_w(c) >= 13 && G.plane_h(c) >= 13 && G.sample_bytes == 1;
 }
 
 bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *storage, hipStream_t stream)
@@ -573,7 +574,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 					if (l == first)
 					{
 						J.src = nullptr;
-						J.src_kind = kStoreU8;
+						J.src_kind = G.sample_bytes == 2 ? kStoreU16 : kStoreU8;
 						J.R_x = G.plane_w(c);
 						J.R_y = G.plane_h(c);
 					}
@@ -594,7 +595,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 					if (l == first)
 					{
 						J.dst = nullptr;
-						J.dst_kind = kStoreU8;
+						J.dst_kind = G.sample_bytes == 2 ? kStoreU16 : kStoreU8;
 						J.R_x = G.plane_w(c);
 						J.R_y = G.plane_h(c);
 					}

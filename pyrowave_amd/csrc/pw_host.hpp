@@ -55,6 +55,7 @@ struct Geometry
 	long flat_count = 0;
 	size_t store_bytes = 0;
 	int precision = PYROWAVE_HIP_PRECISION_FP32;
+	int sample_bytes = 1; // encoder input / decoder output planes: 1 (UNORM8) or 2 (R16_UNORM)
 	BandGeom band[kComponents][kLevels][kBands];
 
 	bool init(int w, int h, int chroma, int precision);
@@ -63,6 +64,42 @@ struct Geometry
 	int first_level(int c) const { return (c && chroma == 0) ? 1 : 0; }
 	uint32_t capacity_words() const;
 };
+
+// A caller's input plane c fits the encoder's geometry: data present, rows
+// long enough for the plane's width at the pixel stride (default: one
+// sample), and for 16-bit samples data and strides 2-byte aligned.
+inline bool plane_fits(const Geometry &G, int c, const pyrowave_hip_plane &p)
+{
+	const size_t sb = size_t(G.sample_bytes);
+	const size_t step = p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes : sb;
+	if (!p.data || step < sb || p.row_stride_in_bytes < (size_t(G.plane_w(c)) - 1) * step + sb)
+		return false;
+	return sb == 1 || ((reinterpret_cast<uintptr_t>(p.data) | p.row_stride_in_bytes | step) & 1u) == 0;
+}
+
+inline bool frame_fits(const Geometry &G, const pyrowave_hip_frame &f)
+{
+	for (int c = 0; c < 3; c++)
+		if (!plane_fits(G, c, f.planes[c]))
+			return false;
+	return true;
+}
+
+// A decode target's planes fit the decoder's geometry (rows of plane-width
+// samples; 16-bit targets 2-byte aligned).
+inline bool target_fits(const Geometry &G, const pyrowave_hip_decode_target &t)
+{
+	const size_t sb = size_t(G.sample_bytes);
+	for (int c = 0; c < 3; c++)
+	{
+		const pyrowave_hip_output_plane &p = t.planes[c];
+		if (!p.data || p.row_stride_in_bytes < size_t(G.plane_w(c)) * sb)
+			return false;
+		if (sb == 2 && ((reinterpret_cast<uintptr_t>(p.data) | p.row_stride_in_bytes) & 1u))
+			return false;
+	}
+	return true;
+}
 
 struct DeviceBuffer
 {

@@ -91,26 +91,6 @@ struct Finished
 	uint32_t bytes = 0, packets = 0;
 };
 
-bool frame_fits(const Geometry &G, const pyrowave_hip_frame &f)
-{
-	for (int c = 0; c < 3; c++)
-	{
-		const pyrowave_hip_plane &p = f.planes[c];
-		size_t step = p.pixel_stride_in_bytes ? p.pixel_stride_in_bytes : 1;
-		if (!p.data || p.row_stride_in_bytes < (size_t(G.plane_w(c)) - 1) * step + 1)
-			return false;
-	}
-	return true;
-}
-
-bool target_fits(const Geometry &G, const pyrowave_hip_decode_target &t)
-{
-	for (int c = 0; c < 3; c++)
-		if (!t.planes[c].data || t.planes[c].row_stride_in_bytes < size_t(G.plane_w(c)))
-			return false;
-	return true;
-}
-
 // Bytes of the download that covers a result at this budget (the encoder
 // never writes more than the budget's words).
 uint32_t download_words(size_t budget)
