@@ -67,3 +67,23 @@ def mask_unstored_ll(L, flat):
         B = L.band[c][first][0]
         out[B.coeff_off:B.coeff_off + B.bw * B.bh] = 0
     return out
+
+
+def dev_planes16(planes, pad=0):
+    """u16 device copies of 16-bit planes (row pitch padded by `pad` samples)
+    and their pyrowave_hip_frame; keep the tensors alive while it is used."""
+    import torch
+    ts = []
+    for p in planes:
+        t = torch.zeros((p.shape[0], p.shape[1] + pad), dtype=torch.int16, device="cuda")
+        t[:, :p.shape[1]] = torch.from_numpy(np.ascontiguousarray(p, np.uint16).view(np.int16)).cuda()
+        ts.append(t)
+    return ts, pw.hip_frame([(t.data_ptr(), 2 * t.stride(0), 2) for t in ts])
+
+
+def target16(w, h, chroma):
+    """Zeroed u16 device planes of a frame and their pyrowave_hip_decode_target."""
+    import torch
+    outs = [torch.zeros(chroma_shape(w, h, chroma) if c else (h, w), dtype=torch.int16, device="cuda")
+            for c in range(3)]
+    return outs, pw.hip_target([(o.data_ptr(), 2 * o.stride(0)) for o in outs])

@@ -22,7 +22,7 @@ import pyrowave_amd as pw
 from oracle import oracle as O
 from pyrowave_amd import synth
 
-from helpers import chroma_shape
+from helpers import dev_planes16 as _dev16, target16 as _target16
 
 pytestmark = pytest.mark.gpu
 
@@ -36,24 +36,6 @@ def planes16(w, h, chroma, seed):
     base = synth.frame(w, h, chroma == pw.CHROMA_420, seed=seed)
     return [(p.astype(np.int64) * 257 + rng.integers(-200, 201, p.shape)).clip(0, 65535).astype(np.uint16)
             for p in base]
-
-
-def _dev16(planes, pad=0):
-    """u16 device copies (row pitch padded by `pad` samples) and their frame."""
-    import torch
-    ts = []
-    for p in planes:
-        t = torch.zeros((p.shape[0], p.shape[1] + pad), dtype=torch.int16, device="cuda")
-        t[:, :p.shape[1]] = torch.from_numpy(p.view(np.int16)).cuda()
-        ts.append(t)
-    return ts, pw.hip_frame([(t.data_ptr(), 2 * t.stride(0), 2) for t in ts])
-
-
-def _target16(w, h, chroma):
-    import torch
-    outs = [torch.zeros(chroma_shape(w, h, chroma) if c else (h, w), dtype=torch.int16, device="cuda")
-            for c in range(3)]
-    return outs, pw.hip_target([(o.data_ptr(), 2 * o.stride(0)) for o in outs])
 
 
 def _encode16(dev, w, h, chroma, precision, frame, budget, boundary=1 << 30):
