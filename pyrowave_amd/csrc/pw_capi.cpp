@@ -495,6 +495,8 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_create_device_by_compat(uint32_t vi
 		d->dev.timing = strtol(env, nullptr, 0) != 0 ? (1u << pw::kStageCount) - 1 : 0u;
 	if (const char *env = getenv("PYROWAVE_FUSED"))
 		d->dev.fused = (int)strtol(env, nullptr, 0);
+	if (const char *env = getenv("PYROWAVE_FUSED_RESOLVE"))
+		d->dev.fused_resolve = strtol(env, nullptr, 0) != 0;
 	*device = d.release();
 	return PYROWAVE_SUCCESS;
 }

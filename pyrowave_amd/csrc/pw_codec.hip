@@ -706,6 +706,268 @@ __global__ __launch_bounds__(256) void k_rc_prepare(FrameState *__restrict__ fra
 		F.group_sum[g] = (uint32_t)total;
 }
 
+// Inclusive scan over a 1024-thread workgroup (16 waves); total: the sum.
+__device__ __forceinline__ int block_incl_scan1024(int v, int *s_wave, int &total)
+{
+	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+	const int incl = wave_incl_scan(v);
+	if (lane == 63)
+		s_wave[wave] = incl;
+	__syncthreads();
+	int base = 0;
+	total = 0;
+#pragma unroll
+	for (int w = 0; w < 16; w++)
+	{
+		const int x = s_wave[w];
+		base += w < wave ? x : 0;
+		total += x;
+	}
+	__syncthreads();
+	return base + incl;
+}
+
+// The three rate-control steps of one frame in one 1024-thread workgroup:
+// the finalize of k_rc_finalize (its 512-thread scan reproduced by threads
+// 0-511), the partial entries of k_rc_partial walked one after another by the
+// whole workgroup, and the per-block truncation and group offsets of
+// k_rc_prepare, with the classes in LDS: two launch boundaries fewer for a
+// lone frame of up to kResolveMaxBlocks blocks.
+__global__ __launch_bounds__(1024) void k_rc_resolve(FrameState *__restrict__ frames)
+{
+	__shared__ uint32_t s_scan[512];
+	__shared__ uint8_t s_cls[kFlat];
+	__shared__ uint16_t s_plist[kFlat];
+	__shared__ int s_preq[kFlat];
+	__shared__ uint32_t s_pq[kResolveMaxBlocks / 4]; // partial truncation per block, one byte each
+
+	__shared__ int s_wave[16];
+	__shared__ uint32_t s_consumed;
+	__shared__ int s_npartial;
+	FrameState &F = frames[blockIdx.x];
+	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+	const int n32 = F.n32;
+
+	// Every load that does not depend on the scan goes out first: the
+	// totals, the level-0 words and the first chunk's block records.
+	const bool scanner = t < 512;
+	uint32_t v[4] = {0u, 0u, 0u, 0u};
+	if (scanner)
+	{
+		const uint4 tv = reinterpret_cast<const uint4 *>(F.totals)[t];
+		v[0] = tv.x;
+		v[1] = tv.y;
+		v[2] = tv.z;
+		v[3] = tv.w;
+	}
+	// Thread t owns blocks t + 1024 i; their records (16 word counts, 16
+	// buckets) are held in registers from here on.
+	constexpr int kPre = 4;
+	static_assert(kResolveMaxBlocks <= kPre * 1024, "every block record is held in registers");
+	uint4 rw0[kPre], rw1[kPre], rbk[kPre];
+#pragma unroll
+	for (int i = 0; i < kPre; i++)
+	{
+		const int b = t + 1024 * i;
+		rw0[i] = rw1[i] = rbk[i] = make_uint4(0u, 0u, 0u, 0u);
+		if (b < n32)
+		{
+			const uint4 *wp = reinterpret_cast<const uint4 *>(F.words + (long)b * 16);
+			rw0[i] = wp[0];
+			rw1[i] = wp[1];
+			rbk[i] = *reinterpret_cast<const uint4 *>(F.bucket + (long)b * 16);
+		}
+	}
+	uint32_t acc = 0;
+	{
+		// Words at quant level 0, eight blocks per 16-byte load (the array is
+		// padded to whole groups of eight with zeros).
+		const uint4 *w = reinterpret_cast<const uint4 *>(F.words0);
+		const int groups = (n32 + 7) >> 3;
+		for (int g = t; g < groups; g += 1024)
+		{
+			const uint4 q = w[g];
+			acc += (q.x & 0xffffu) + (q.x >> 16) + (q.y & 0xffffu) + (q.y >> 16) + (q.z & 0xffffu) + (q.z >> 16) +
+			       (q.w & 0xffffu) + (q.w >> 16);
+		}
+	}
+	for (int i = t; i < (n32 + 3) / 4; i += 1024)
+		s_pq[i] = 0;
+	if (t == 0)
+	{
+		s_consumed = 0;
+		s_npartial = 0;
+	}
+	__syncthreads();
+	for (int d = 32; d >= 1; d >>= 1)
+		acc += __shfl_xor(acc, d, 64);
+	if (lane == 0)
+		atomicAdd(&s_consumed, acc);
+
+	// Finalize scan: threads 0-511 hold four entries each
+	// (reference shaders/analyze_rate_control_finalize.comp:14-38).
+	const uint32_t total4[4] = {v[0], v[1], v[2], v[3]};
+	if (scanner)
+	{
+		v[1] += v[0];
+		v[2] += v[1];
+		v[3] += v[2];
+		s_scan[t] = v[3];
+	}
+	__syncthreads();
+	const uint32_t limit = (F.flags & 1u) ? 512u : 256u;
+	for (uint32_t step = 1; step < limit; step *= 2)
+	{
+		__syncthreads();
+		const uint32_t up = scanner && t >= (int)step ? s_scan[t - step] : 0u;
+		__syncthreads();
+		if (scanner)
+		{
+#pragma unroll
+			for (int i = 0; i < 4; i++)
+				v[i] += up;
+			s_scan[t] = v[3];
+		}
+	}
+	__syncthreads();
+	if (scanner)
+	{
+		const int required_base = (int)s_consumed - F.target_words;
+		uint32_t prev = t ? s_scan[t - 1] : 0u;
+		uint8_t cls4[4];
+		int req4[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+		{
+			const int x = 4 * t + i;
+			const bool empty = x == 0 ? v[0] == 0 : v[i] == prev;
+			const int required = x == 0 ? required_base : required_base - (int)prev;
+			uint8_t cls = kRcSkip;
+			if (!empty && required > 0)
+				cls = (int)total4[i] <= required ? kRcFull : kRcPartial;
+			cls4[i] = cls;
+			req4[i] = required;
+			s_cls[x] = cls;
+			if (cls == kRcPartial)
+			{
+				const int k = atomicAdd(&s_npartial, 1);
+				s_plist[k] = (uint16_t)x;
+				s_preq[k] = required;
+			}
+			prev = v[i];
+		}
+		reinterpret_cast<uint4 *>(F.prefix)[t] = make_uint4(v[0], v[1], v[2], v[3]);
+		reinterpret_cast<int4 *>(F.required)[t] = make_int4(req4[0], req4[1], req4[2], req4[3]);
+		reinterpret_cast<uint32_t *>(F.cls)[t] = (uint32_t)cls4[0] | ((uint32_t)cls4[1] << 8) |
+		                                         ((uint32_t)cls4[2] << 16) | ((uint32_t)cls4[3] << 24);
+	}
+	if (t == 0)
+		F.rc[kRcConsumed] = s_consumed;
+	__syncthreads();
+
+	// Partially applied entries (reference shaders/resolve_rate_control.comp:57-79):
+	// the blocks of the entry's subdivision in order; a block whose step into
+	// the entry's bucket keeps the running saving before it under the
+	// requirement takes that step.
+	uint8_t *const pq = reinterpret_cast<uint8_t *>(s_pq);
+	const int npartial = s_npartial;
+	for (int k = 0; k < npartial; k++)
+	{
+		const int x = s_plist[k];
+		const int required = s_preq[k];
+		const uint32_t beta = (uint32_t)x >> 4;
+		const int base = (x & 15) * F.per_sub;
+		const int end = base + F.per_sub < n32 ? base + F.per_sub : n32;
+		int total_saved = 0;
+		// Chunks aligned to 1024 blocks, so that block b is the one its
+		// thread holds; blocks outside [base, end) save nothing.
+		auto step = [&](int ci, const uint4 &w0, const uint4 &w1, const uint4 &bk) {
+			const int b = (ci << 10) + t;
+			int saving = 0, q = 0;
+			if (b >= base && b < end)
+			{
+				const uint32_t bw[4] = {bk.x, bk.y, bk.z, bk.w};
+				const uint32_t wv[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+				for (int qq = 1; qq < 16; qq++)
+				{
+					if (((bw[qq >> 2] >> (8 * (qq & 3))) & 0xffu) == beta)
+					{
+						const int a = (int)((wv[(qq - 1) >> 1] >> (16 * ((qq - 1) & 1))) & 0xffffu);
+						const int c = (int)((wv[qq >> 1] >> (16 * (qq & 1))) & 0xffffu);
+						saving = a - c;
+						q = qq;
+						break;
+					}
+				}
+			}
+			int sum;
+			const int incl = block_incl_scan1024(saving, s_wave, sum);
+			// Blocks of one subdivision are distinct, and entries run one
+			// after another, so the byte's read-modify-write is race-free.
+			if (saving != 0 && total_saved + incl - saving < required && q > pq[b])
+				pq[b] = (uint8_t)q;
+			total_saved += sum;
+		};
+		const int c0 = base >> 10, c1 = (end + 1023) >> 10;
+#pragma unroll
+		for (int ci = 0; ci < kPre; ci++)
+			if (ci >= c0 && ci < c1 && total_saved < required)
+				step(ci, rw0[ci], rw1[ci], rbk[ci]);
+		__syncthreads();
+	}
+
+	// Final truncation per block and word offsets within 256-block groups
+	// (four groups per 1024-block chunk, four waves each).
+	auto truncate = [&](int ci, const uint4 &w0, const uint4 &w1, const uint4 &bk) {
+		const int b = (ci << 10) + t;
+		int wf = 0;
+		if (b < n32)
+		{
+			const uint32_t wv[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+			const uint32_t bw[4] = {bk.x, bk.y, bk.z, bk.w};
+			auto words = [&](int q) { return (wv[q >> 1] >> (16 * (q & 1))) & 0xffffu; };
+			int qf = pq[b];
+			const uint32_t sd = (uint32_t)b >> F.shamt;
+#pragma unroll
+			for (int q = 1; q < 16; q++)
+			{
+				if (words(q - 1) != words(q))
+				{
+					const uint32_t bq = (bw[q >> 2] >> (8 * (q & 3))) & 0xffu;
+					if (s_cls[bq * kSubdiv + sd] == kRcFull && q > qf)
+						qf = q;
+				}
+			}
+			F.quant[b] = (uint8_t)qf;
+			wf = (int)words(qf);
+		}
+		const int incl = wave_incl_scan(wf);
+		if (lane == 63)
+			s_wave[wave] = incl;
+		__syncthreads();
+		const int grp = wave >> 2;
+		int gbase = 0, gtotal = 0;
+#pragma unroll
+		for (int w = 0; w < 4; w++)
+		{
+			const int x = s_wave[4 * grp + w];
+			gbase += 4 * grp + w < wave ? x : 0;
+			gtotal += x;
+		}
+		if (b < n32)
+			F.excl[b] = (uint32_t)(gbase + incl - wf);
+		const int g = b >> 8;
+		if ((t & 255) == 0 && g < F.ngroups)
+			F.group_sum[g] = (uint32_t)gtotal;
+		__syncthreads();
+	};
+	const int chunks = (n32 + 1023) >> 10;
+#pragma unroll
+	for (int ci = 0; ci < kPre; ci++)
+		if (ci < chunks)
+			truncate(ci, rw0[ci], rw1[ci], rbk[ci]);
+}
+
 // reference shaders/block_packing.comp:107-142.
 __device__ __forceinline__ uint32_t quantize_code_word(uint32_t cw, int quant)
 {
@@ -1175,6 +1437,11 @@ void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nb
 void launch_rc_finalize(FrameState *frames, int nframes, hipStream_t stream)
 {
 	hipLaunchKernelGGL(k_rc_finalize, dim3(nframes), dim3(512), 0, stream, frames);
+}
+
+void launch_rc_resolve(FrameState *frames, int nframes, hipStream_t stream)
+{
+	hipLaunchKernelGGL(k_rc_resolve, dim3(nframes), dim3(1024), 0, stream, frames);
 }
 
 void launch_rc_partial(FrameState *frames, int nframes, hipStream_t stream)

@@ -941,9 +941,14 @@ void run_encode(Plan &P, hipStream_t stream, bool timed)
 	D.end_stage(stream, kStageQuant, n, ev);
 
 	D.begin_stage(stream, kStageResolve, n, &ev);
-	launch_rc_finalize(frames, n, stream);
-	launch_rc_partial(frames, n, stream);
-	launch_rc_prepare(frames, P.d_grp_frame, P.ngroups, P.d_frame_group_base, stream);
+	if (D.fused_resolve && n <= kResolveMaxFrames && P.max_n32 <= kResolveMaxBlocks)
+		launch_rc_resolve(frames, n, stream);
+	else
+	{
+		launch_rc_finalize(frames, n, stream);
+		launch_rc_partial(frames, n, stream);
+		launch_rc_prepare(frames, P.d_grp_frame, P.ngroups, P.d_frame_group_base, stream);
+	}
 	D.end_stage(stream, kStageResolve, n, ev);
 
 	D.begin_stage(stream, kStagePacking, n, &ev);

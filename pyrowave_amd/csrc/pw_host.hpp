@@ -142,6 +142,7 @@ struct Device
 	// variant 0); new plans of the same shape start from these.
 	std::map<uint64_t, int> tuned_variant;
 	int forced_variant = -1; // >= 0: every transform level uses this variant
+	bool fused_resolve = true; // rate control of a frame in one launch (k_rc_resolve) where it fits
 	int fused = 1;           // forward levels 0+1 in one launch (k_dwt_forward_fused): 0 never, 1 batches of kFusedMinTiles tiles or more, 2 always
 	int tune_reps = 0;       // > 0: the next eager encode / decode autotunes
 	// PSNR-HVS-M-H tables per image height (device copies).

@@ -172,10 +172,14 @@ def test_device_parse_replays_reference_parser(device, name):
         dec.close()
 
 
-def test_exact_prefix_matches_oracle(device):
+@pytest.mark.parametrize("merged", ["1", "0"])
+def test_exact_prefix_matches_oracle(merged, monkeypatch):
     """The exact-prefix finalize on the device equals the oracle's, differs
     from the reference's scan above flat index 1024 on this content, and its
-    bitstream equals the oracle's exact-prefix encode."""
+    bitstream equals the oracle's exact-prefix encode; through the
+    one-workgroup resolve and the three launches (PYROWAVE_FUSED_RESOLVE=0)."""
+    monkeypatch.setenv("PYROWAVE_FUSED_RESOLVE", merged)
+    device = pw.Device()
     w, h, budget = 640, 360, 1000
     planes = synth.frame(w, h, False, seed=5, checker=0.3)
     rng = np.random.default_rng(1)
@@ -189,6 +193,7 @@ def test_exact_prefix_matches_oracle(device):
         rc = enc.read_rate_control(L.count32)
     finally:
         enc.close()
+        device.close()
     exact = O.finalize(rc["totals"], q1=False)
     assert np.array_equal(rc["prefix"], exact)
     assert not np.array_equal(exact, O.finalize(rc["totals"], q1=True))

@@ -162,10 +162,16 @@ def test_fp16_math_batch_cannot_mix_precisions(device):
             e.close()
 
 
-def test_rate_control_state_matches_oracle(device):
+@pytest.mark.parametrize("merged", ["1", "0"])
+def test_rate_control_state_matches_oracle(merged, monkeypatch):
+    """Rate-control state against the oracle, through the one-workgroup
+    resolve of a lone 1080p frame (k_rc_resolve) and through the three
+    launches (PYROWAVE_FUSED_RESOLVE=0)."""
     w, h = 1920, 1080
     planes = synth.frame(w, h, True, seed=7)
     L = O.layout(w, h, 0)
+    monkeypatch.setenv("PYROWAVE_FUSED_RESOLVE", merged)
+    device = pw.Device()
     enc = pw.Encoder(device, w, h, pw.CHROMA_420, precision=pw.PRECISION_FP32)
     try:
         assert enc.encode_cpu(planes, 300000) == pw.SUCCESS
@@ -174,6 +180,7 @@ def test_rate_control_state_matches_oracle(device):
         rc = enc.read_rate_control(L.count32)
     finally:
         enc.close()
+        device.close()
     qa = O.quant_analyze(L, coeffs)
     assert np.array_equal(rc["words"], qa["words"])
     assert np.array_equal(rc["bucket"], qa["bucket"])
