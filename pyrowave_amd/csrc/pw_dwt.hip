@@ -885,6 +885,38 @@ __device__ __forceinline__ float stream_value(uint32_t raw, const StreamSrc &S, 
 		return __uint_as_float(raw);
 }
 
+// The same with the sample's bit offset within its word known.
+template <int KIND>
+__device__ __forceinline__ float stream_value_at(uint32_t raw, uint32_t shift)
+{
+	if constexpr (KIND == kStoreU8)
+		return u8_dc((raw >> shift) & 0xffu);
+	else if constexpr (KIND == kStoreU16)
+		return u16_dc((raw >> shift) & 0xffffu);
+	else if constexpr (KIND == kStoreF16)
+		return __half2float(__ushort_as_half((unsigned short)(raw >> shift)));
+	else
+		return __uint_as_float(raw);
+}
+
+// Element `col` of the row that starts `row` elements into a band: the row
+// address is formed once per wave (scalar registers) and the lane's byte
+// offset is a fixed 32-bit vector offset, so a store costs no vector
+// address arithmetic.
+__device__ __forceinline__ void stream_store_row(void *base, int kind, uint32_t row, uint32_t col, float v)
+{
+	typedef __attribute__((address_space(1))) char GChar;
+	GChar *const rb = (GChar *)base + (size_t)row * (kind == kStoreF16 ? 2u : 4u);
+	// Kept as a 32-bit value at the store (not a 64-bit offset hoisted out of
+	// the loop), so that the store takes the scalar base + vector offset form.
+	uint32_t off = col << (kind == kStoreF16 ? 1 : 2);
+	asm volatile("" : "+v"(off));
+	if (kind == kStoreF16)
+		*(__attribute__((address_space(1))) uint16_t *)(rb + off) = __half_as_ushort(__float2half_rn(v));
+	else
+		*(__attribute__((address_space(1))) float *)(rb + off) = v;
+}
+
 __device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx, float v)
 {
 	if (kind == kStoreF16)
@@ -894,8 +926,13 @@ __device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx,
 }
 
 // One tile of the streamed forward transform with source samples of format
-// KIND (fixed per wave, so the loads need no branches).
-template <int KIND>
+// KIND (fixed per wave, so the loads need no branches).  WORD_ROWS: every
+// source row starts on a 32-bit word (row pitch in bytes a multiple of 4:
+// band storage always, planes of such pitches), so a lane's word offset and
+// bit shift within a row are the same in every row, and a row's loads are a
+// scalar row base plus that fixed per-lane offset (no address arithmetic per
+// row on the vector unit).
+template <int KIND, bool WORD_ROWS>
 __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void *src_ptr, uint32_t pitch,
                                                     uint32_t step, int x0, int y0, int lane)
 {
@@ -915,23 +952,51 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 	// once (the extension's modulo would otherwise run per row pair); a row
 	// pair reads its two entries with readlane.
 	static_assert(2 * kPairs <= 64, "one window row per lane");
-	const uint32_t row_off =
+	constexpr uint32_t kEs = stream_esize<KIND>();
+	// Row offsets: elements, or words for WORD_ROWS (exact: the row's byte
+	// offset is a multiple of 4).
+	const uint32_t row_el =
 	    lane < 2 * kPairs ? __umul24((uint32_t)fwd_ext(y0 - kApron + lane, J.R_y, J.A_y), pitch) : 0u;
+	const uint32_t row_off = WORD_ROWS ? (row_el * kEs) >> 2 : row_el;
+	// WORD_ROWS: the even / odd sample's word within a row and bit shift.
+	const uint32_t be = src.lo + xe * kEs, bo = src.lo + xo * kEs;
+	const uint32_t we = be >> 2, wo = bo >> 2, se = 8u * (be & 3u), so = 8u * (bo & 3u);
 	auto rows = [&](int n, uint32_t &r0, uint32_t &r1) {
 		r0 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n);
 		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
 	};
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
-		r[0] = stream_raw<KIND>(src, r0 + xe);
-		r[1] = stream_raw<KIND>(src, r0 + xo);
-		r[2] = stream_raw<KIND>(src, r1 + xe);
-		r[3] = stream_raw<KIND>(src, r1 + xo);
+		if constexpr (WORD_ROWS)
+		{
+			const GU32 p0 = src.words + r0, p1 = src.words + r1;
+			r[0] = p0[we];
+			r[1] = p0[wo];
+			r[2] = p1[we];
+			r[3] = p1[wo];
+		}
+		else
+		{
+			r[0] = stream_raw<KIND>(src, r0 + xe);
+			r[1] = stream_raw<KIND>(src, r0 + xo);
+			r[2] = stream_raw<KIND>(src, r1 + xe);
+			r[3] = stream_raw<KIND>(src, r1 + xo);
+		}
 	};
 	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
-		v[0] = stream_value<KIND>(r[0], src, r0 + xe);
-		v[1] = stream_value<KIND>(r[1], src, r0 + xo);
-		v[2] = stream_value<KIND>(r[2], src, r1 + xe);
-		v[3] = stream_value<KIND>(r[3], src, r1 + xo);
+		if constexpr (WORD_ROWS)
+		{
+			v[0] = stream_value_at<KIND>(r[0], se);
+			v[1] = stream_value_at<KIND>(r[1], so);
+			v[2] = stream_value_at<KIND>(r[2], se);
+			v[3] = stream_value_at<KIND>(r[3], so);
+		}
+		else
+		{
+			v[0] = stream_value<KIND>(r[0], src, r0 + xe);
+			v[1] = stream_value<KIND>(r[1], src, r0 + xo);
+			v[2] = stream_value<KIND>(r[2], src, r1 + xe);
+			v[3] = stream_value<KIND>(r[3], src, r1 + xo);
+		}
 	};
 	VLine lo, hi; // columns of horizontal low / high outputs
 	auto lift_pair = [&](int n, float e0, float o0, float e1, float o1) {
@@ -943,13 +1008,11 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 		const int oy = (y0 >> 1) + n - kApron;
 		if (n >= kApron && store_x && oy < J.bh)
 		{
-			// 32-bit element offsets from the uniform band bases (bands stay
-			// below 4 GiB): scalar base, one vector offset.
-			const uint32_t idx = __umul24((uint32_t)oy, (uint32_t)J.band_pitch) + (uint32_t)ox;
-			stream_store(J.band[0], J.ll_kind, idx, ll);
-			stream_store(J.band[1], J.band_kind, idx, hl);
-			stream_store(J.band[2], J.band_kind, idx, lh);
-			stream_store(J.band[3], J.band_kind, idx, hh);
+			const uint32_t row = (uint32_t)oy * (uint32_t)J.band_pitch;
+			stream_store_row(J.band[0], J.ll_kind, row, (uint32_t)ox, ll);
+			stream_store_row(J.band[1], J.band_kind, row, (uint32_t)ox, hl);
+			stream_store_row(J.band[2], J.band_kind, row, (uint32_t)ox, lh);
+			stream_store_row(J.band[3], J.band_kind, row, (uint32_t)ox, hh);
 		}
 	};
 	// Both buffers are converted (their loads were issued an iteration ago),
@@ -961,7 +1024,7 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 	rows(1, b0, b1);
 	load_pair(a0, a1, ra);
 	load_pair(b0, b1, rb);
-#pragma unroll 1
+#pragma unroll 2
 	for (int n = 0; n < kPairs; n += 2)
 	{
 		float a[4], b[4];
@@ -988,29 +1051,40 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const 
 {
 	constexpr int TW = kFwdVariantTileW[kFwdStreamVariant], TH = kFwdVariantTileH[kFwdStreamVariant];
 	const int lane = threadIdx.x & 63;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
+	// everything derived from it stay in scalar registers).
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	                                                    kStreamWaves +
+	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
 		return; // whole waves: no wave-level or workgroup barrier follows
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
 	const int x0 = (local % J.tiles_x) * TW;
 	const int y0 = (local / J.tiles_x) * TH;
+	// Band storage pitches are multiples of 32 elements: word rows always.
 	if (J.src_kind == kStoreU8)
 	{
 		const FrameState &F = frames[J.frame];
-		forward_stream_tile<kStoreU8>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
-		                              (uint32_t)F.src_step[J.component], x0, y0, lane);
+		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
+		if ((pitch & 3u) == 0)
+			forward_stream_tile<kStoreU8, true>(J, F.src[J.component], pitch, step, x0, y0, lane);
+		else
+			forward_stream_tile<kStoreU8, false>(J, F.src[J.component], pitch, step, x0, y0, lane);
 	}
 	else if (J.src_kind == kStoreU16)
 	{
 		const FrameState &F = frames[J.frame];
-		forward_stream_tile<kStoreU16>(J, F.src[J.component], (uint32_t)F.src_pitch[J.component],
-		                               (uint32_t)F.src_step[J.component], x0, y0, lane);
+		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
+		if ((pitch & 1u) == 0)
+			forward_stream_tile<kStoreU16, true>(J, F.src[J.component], pitch, step, x0, y0, lane);
+		else
+			forward_stream_tile<kStoreU16, false>(J, F.src[J.component], pitch, step, x0, y0, lane);
 	}
 	else if (J.src_kind == kStoreF16)
-		forward_stream_tile<kStoreF16>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+		forward_stream_tile<kStoreF16, true>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 	else
-		forward_stream_tile<kStoreF32>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+		forward_stream_tile<kStoreF32, true>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 }
 
 // --- forward, levels l and l + 1 fused -------------------------------------
@@ -1178,7 +1252,11 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_fused(const F
                                                                          int tiles, int nxcd)
 {
 	const int lane = threadIdx.x & 63;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
+	// everything derived from it stay in scalar registers).
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	                                                    kStreamWaves +
+	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
 		return; // whole waves: no wave-level or workgroup barrier follows
 	const FusedJob &FJ = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
@@ -1356,7 +1434,11 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const 
 	constexpr int TW = kInvVariantTileW[kInvStreamVariant], TH = kInvVariantTileH[kInvStreamVariant];
 	static_assert(TW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
 	const int lane = threadIdx.x & 63;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
+	// everything derived from it stay in scalar registers).
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	                                                    kStreamWaves +
+	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
 		return; // whole waves: no wave-level or workgroup barrier follows
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
@@ -1576,7 +1658,11 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream_h(cons
 {
 	constexpr int TW = kFwdVariantTileW[kHalfStreamVariant], TH = kFwdVariantTileH[kHalfStreamVariant];
 	const int lane = threadIdx.x & 63;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
+	// everything derived from it stay in scalar registers).
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	                                                    kStreamWaves +
+	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
 		return; // whole waves: no wave-level or workgroup barrier follows
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
@@ -1729,7 +1815,11 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream_h(cons
 	constexpr int TW = kInvVariantTileW[kHalfStreamVariant], TH = kInvVariantTileH[kHalfStreamVariant];
 	static_assert(TW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
 	const int lane = threadIdx.x & 63;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kStreamWaves + (int)(threadIdx.x >> 6);
+	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
+	// everything derived from it stay in scalar registers).
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	                                                    kStreamWaves +
+	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
 		return; // whole waves: no wave-level or workgroup barrier follows
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
