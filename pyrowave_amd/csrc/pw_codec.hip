@@ -1380,6 +1380,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	{
 		__half *b16 = static_cast<__half *>(J.data);
 		uint2 r[2];
+		// Each value rounds to fp16 from its float (no merge with the
+		// product that made it into one mixed-precision rounding).
+#pragma unroll
+		for (int p = 0; p < 8; p++)
+			asm volatile("" : "+v"(v[p]));
 #pragma unroll
 		for (int row = 0; row < 2; row++)
 		{

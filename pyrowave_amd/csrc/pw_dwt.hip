@@ -129,7 +129,12 @@ __device__ __forceinline__ int inv_ext(int p, int N)
 __device__ __forceinline__ void store_kind(void *base, int kind, long idx, float v)
 {
 	if (kind == kStoreF16)
+	{
+		// Rounded to fp16 from the float value (no merge with the product
+		// that made it into one mixed-precision rounding).
+		asm volatile("" : "+v"(v));
 		static_cast<__half *>(base)[idx] = __float2half_rn(v);
+	}
 	else
 		static_cast<float *>(base)[idx] = v;
 }
@@ -489,6 +494,34 @@ __device__ __forceinline__ uint16_t unorm16_store(float v)
 __device__ __forceinline__ bool plane_kind(int kind)
 {
 	return kind == kStoreU8 || kind == kStoreU16;
+}
+
+// Two horizontally adjacent plane samples (px, px + 1) of the row that
+// starts `row` samples into the plane: a wave-uniform row address and a
+// fixed 32-bit per-lane offset (the store's scalar base + vector offset form).
+template <int kind>
+__device__ __forceinline__ void store_plane_pair_row(uint8_t *plane, uint32_t row, int px, int R_x, float e, float o)
+{
+	typedef __attribute__((address_space(1))) uint8_t GU8;
+	constexpr uint32_t sb = kind == kStoreU16 ? 2u : 1u;
+	GU8 *const rb = (GU8 *)plane + (size_t)row * sb;
+	uint32_t off = (uint32_t)px * sb;
+	asm volatile("" : "+v"(off));
+	if constexpr (kind == kStoreU16)
+	{
+		__attribute__((address_space(1))) uint16_t *d = (__attribute__((address_space(1))) uint16_t *)(rb + off);
+		if (px < R_x)
+			d[0] = unorm16_store(e);
+		if (px + 1 < R_x)
+			d[1] = unorm16_store(o);
+	}
+	else
+	{
+		if (px < R_x)
+			rb[off] = unorm8_store(e);
+		if (px + 1 < R_x)
+			rb[off + 1] = unorm8_store(o);
+	}
 }
 
 // Two horizontally adjacent plane samples (px, px + 1) of row py.
@@ -903,7 +936,8 @@ __device__ __forceinline__ float stream_value_at(uint32_t raw, uint32_t shift)
 // address is formed once per wave (scalar registers) and the lane's byte
 // offset is a fixed 32-bit vector offset, so a store costs no vector
 // address arithmetic.
-__device__ __forceinline__ void stream_store_row(void *base, int kind, uint32_t row, uint32_t col, float v)
+template <int kind>
+__device__ __forceinline__ void stream_store_row(void *base, uint32_t row, uint32_t col, float v)
 {
 	typedef __attribute__((address_space(1))) char GChar;
 	GChar *const rb = (GChar *)base + (size_t)row * (kind == kStoreF16 ? 2u : 4u);
@@ -911,18 +945,65 @@ __device__ __forceinline__ void stream_store_row(void *base, int kind, uint32_t 
 	// the loop), so that the store takes the scalar base + vector offset form.
 	uint32_t off = col << (kind == kStoreF16 ? 1 : 2);
 	asm volatile("" : "+v"(off));
-	if (kind == kStoreF16)
+	if constexpr (kind == kStoreF16)
+	{
+		// The barrier keeps the compiler from merging the product that made
+		// v with this conversion (v_fma_mix rounds once, straight to fp16).
+		asm volatile("" : "+v"(v));
 		*(__attribute__((address_space(1))) uint16_t *)(rb + off) = __half_as_ushort(__float2half_rn(v));
+	}
 	else
 		*(__attribute__((address_space(1))) float *)(rb + off) = v;
+}
+
+// Elements col and col + 1 (col even) of a band row as one store: fp32 pairs
+// are 8-byte and fp16 pairs 4-byte aligned (band pitches are multiples of 32).
+template <int kind>
+__device__ __forceinline__ void stream_store_pair_row(void *base, uint32_t row, uint32_t col, float e, float o)
+{
+	typedef __attribute__((address_space(1))) char GChar;
+	GChar *const rb = (GChar *)base + (size_t)row * (kind == kStoreF16 ? 2u : 4u);
+	uint32_t off = col << (kind == kStoreF16 ? 1 : 2);
+	asm volatile("" : "+v"(off));
+	if constexpr (kind == kStoreF16)
+	{
+		asm volatile("" : "+v"(e), "+v"(o)); // two roundings, as in stream_store_row
+		*(__attribute__((address_space(1))) uint32_t *)(rb + off) =
+		    (uint32_t)__half_as_ushort(__float2half_rn(e)) | ((uint32_t)__half_as_ushort(__float2half_rn(o)) << 16);
+	}
+	else
+		*(__attribute__((address_space(1))) uint64_t *)(rb + off) =
+		    (uint64_t)__float_as_uint(e) | ((uint64_t)__float_as_uint(o) << 32);
 }
 
 __device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx, float v)
 {
 	if (kind == kStoreF16)
+	{
+		asm volatile("" : "+v"(v)); // as in store_kind
 		((__attribute__((address_space(1))) uint16_t *)base)[idx] = __half_as_ushort(__float2half_rn(v));
+	}
 	else
 		((__attribute__((address_space(1))) float *)base)[idx] = v;
+}
+
+template <int KIND, bool WORD_ROWS, int LLK, int HK>
+__device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void *src_ptr, uint32_t pitch,
+                                                    uint32_t step, int x0, int y0, int lane);
+
+// The tile with the job's output storage kinds as template arguments (LL
+// FP32 with fp16 or FP32 high bands, or all four fp16), so no store in the
+// loop branches on them.
+template <int KIND, bool WORD_ROWS>
+__device__ __forceinline__ void forward_stream_kinds(const DwtJob &J, const void *src_ptr, uint32_t pitch,
+                                                     uint32_t step, int x0, int y0, int lane)
+{
+	if (J.ll_kind == kStoreF16)
+		forward_stream_tile<KIND, WORD_ROWS, kStoreF16, kStoreF16>(J, src_ptr, pitch, step, x0, y0, lane);
+	else if (J.band_kind == kStoreF16)
+		forward_stream_tile<KIND, WORD_ROWS, kStoreF32, kStoreF16>(J, src_ptr, pitch, step, x0, y0, lane);
+	else
+		forward_stream_tile<KIND, WORD_ROWS, kStoreF32, kStoreF32>(J, src_ptr, pitch, step, x0, y0, lane);
 }
 
 // One tile of the streamed forward transform with source samples of format
@@ -932,7 +1013,7 @@ __device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx,
 // bit shift within a row are the same in every row, and a row's loads are a
 // scalar row base plus that fixed per-lane offset (no address arithmetic per
 // row on the vector unit).
-template <int KIND, bool WORD_ROWS>
+template <int KIND, bool WORD_ROWS, int LLK, int HK>
 __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void *src_ptr, uint32_t pitch,
                                                     uint32_t step, int x0, int y0, int lane)
 {
@@ -1009,10 +1090,10 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 		if (n >= kApron && store_x && oy < J.bh)
 		{
 			const uint32_t row = (uint32_t)oy * (uint32_t)J.band_pitch;
-			stream_store_row(J.band[0], J.ll_kind, row, (uint32_t)ox, ll);
-			stream_store_row(J.band[1], J.band_kind, row, (uint32_t)ox, hl);
-			stream_store_row(J.band[2], J.band_kind, row, (uint32_t)ox, lh);
-			stream_store_row(J.band[3], J.band_kind, row, (uint32_t)ox, hh);
+			stream_store_row<LLK>(J.band[0], row, (uint32_t)ox, ll);
+			stream_store_row<HK>(J.band[1], row, (uint32_t)ox, hl);
+			stream_store_row<HK>(J.band[2], row, (uint32_t)ox, lh);
+			stream_store_row<HK>(J.band[3], row, (uint32_t)ox, hh);
 		}
 	};
 	// Both buffers are converted (their loads were issued an iteration ago),
@@ -1068,23 +1149,23 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const 
 		const FrameState &F = frames[J.frame];
 		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
 		if ((pitch & 3u) == 0)
-			forward_stream_tile<kStoreU8, true>(J, F.src[J.component], pitch, step, x0, y0, lane);
+			forward_stream_kinds<kStoreU8, true>(J, F.src[J.component], pitch, step, x0, y0, lane);
 		else
-			forward_stream_tile<kStoreU8, false>(J, F.src[J.component], pitch, step, x0, y0, lane);
+			forward_stream_kinds<kStoreU8, false>(J, F.src[J.component], pitch, step, x0, y0, lane);
 	}
 	else if (J.src_kind == kStoreU16)
 	{
 		const FrameState &F = frames[J.frame];
 		const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
 		if ((pitch & 1u) == 0)
-			forward_stream_tile<kStoreU16, true>(J, F.src[J.component], pitch, step, x0, y0, lane);
+			forward_stream_kinds<kStoreU16, true>(J, F.src[J.component], pitch, step, x0, y0, lane);
 		else
-			forward_stream_tile<kStoreU16, false>(J, F.src[J.component], pitch, step, x0, y0, lane);
+			forward_stream_kinds<kStoreU16, false>(J, F.src[J.component], pitch, step, x0, y0, lane);
 	}
 	else if (J.src_kind == kStoreF16)
-		forward_stream_tile<kStoreF16, true>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+		forward_stream_kinds<kStoreF16, true>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 	else
-		forward_stream_tile<kStoreF32, true>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
+		forward_stream_kinds<kStoreF32, true>(J, J.src, (uint32_t)J.src_pitch, 1u, x0, y0, lane);
 }
 
 // --- forward, levels l and l + 1 fused -------------------------------------
@@ -1324,7 +1405,7 @@ __device__ __forceinline__ void hlift_inv_row(float &e, float &o)
 }
 
 // One tile with the LL band stored as LKIND and the high bands as HKIND.
-template <int LKIND, int HKIND>
+template <int LKIND, int HKIND, int DKIND>
 __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const FrameState *frames, int ox0, int oy0,
                                                     int lane)
 {
@@ -1355,38 +1436,39 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 		r0 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n);
 		r1 = (uint32_t)__builtin_amdgcn_readlane((int)row_off, 2 * n + 1);
 	};
-	// Raw words: LL and HL of the even row, LH and HH of the odd row.
-	const StreamSrc ll = stream_src(J.band[0]), hl = stream_src(J.band[1]), lh = stream_src(J.band[2]),
-	                hh = stream_src(J.band[3]);
+	// Raw words: LL and HL of the even row, LH and HH of the odd row.  Band
+	// storage is 256-byte aligned with pitches of 32 elements, so every row
+	// starts on a word: a row's loads are a scalar row base (in words) plus
+	// the lane's fixed word offset, and the bit shift within the word is the
+	// lane's too.
+	constexpr uint32_t kEl = stream_esize<LKIND>(), kEh = stream_esize<HKIND>();
+	const GU32 ll = (GU32)J.band[0], hl = (GU32)J.band[1], lh = (GU32)J.band[2], hh = (GU32)J.band[3];
+	const uint32_t wle = (ce * kEl) >> 2, whe = (ce * kEh) >> 2, who = (co * kEh) >> 2;
+	const uint32_t sle = 8u * ((ce * kEl) & 3u), she = 8u * ((ce * kEh) & 3u), sho = 8u * ((co * kEh) & 3u);
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
-		r[0] = stream_raw<LKIND>(ll, r0 + ce);
-		r[1] = stream_raw<HKIND>(hl, r0 + co);
-		r[2] = stream_raw<HKIND>(lh, r1 + ce);
-		r[3] = stream_raw<HKIND>(hh, r1 + co);
+		r[0] = (ll + ((r0 * kEl) >> 2))[wle];
+		r[1] = (hl + ((r0 * kEh) >> 2))[who];
+		r[2] = (lh + ((r1 * kEh) >> 2))[whe];
+		r[3] = (hh + ((r1 * kEh) >> 2))[who];
 	};
 	// Scaled for the vertical pass: even rows by K, odd rows by 1/K.
 	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
-		v[0] = stream_value<LKIND>(r[0], ll, r0 + ce) * kK;
-		v[1] = stream_value<HKIND>(r[1], hl, r0 + co) * kK;
-		v[2] = stream_value<HKIND>(r[2], lh, r1 + ce) * kInvK;
-		v[3] = stream_value<HKIND>(r[3], hh, r1 + co) * kInvK;
+		v[0] = stream_value_at<LKIND>(r[0], sle) * kK;
+		v[1] = stream_value_at<HKIND>(r[1], sho) * kK;
+		v[2] = stream_value_at<HKIND>(r[2], she) * kInvK;
+		v[3] = stream_value_at<HKIND>(r[3], sho) * kInvK;
 	};
 	auto store_row = [&](int py, float e, float o) {
 		if (!lane_out)
 			return;
-		if (plane_kind(J.dst_kind))
+		if constexpr (DKIND == kStoreU8 || DKIND == kStoreU16)
 		{
 			if (py < J.R_y)
-				store_plane_pair(plane, J.dst_kind, dpitch, py, px, J.R_x, e, o);
+				store_plane_pair_row<DKIND>(plane, (uint32_t)py * (uint32_t)dpitch, px, J.R_x, e, o);
 		}
-		else if (py < J.A_y)
-		{
-			const uint32_t idx = __umul24((uint32_t)py, (uint32_t)dpitch) + (uint32_t)px;
-			if (px < J.A_x)
-				stream_store(J.dst, J.dst_kind, idx, e);
-			if (px + 1 < J.A_x)
-				stream_store(J.dst, J.dst_kind, idx + 1, o);
-		}
+		else if (py < J.A_y && px < J.A_x)
+			// px is even and A_x = 2 * bw: px + 1 < A_x too.
+			stream_store_pair_row<DKIND>(J.dst, (uint32_t)py * (uint32_t)dpitch, (uint32_t)px, e, o);
 	};
 	VLineInv ce_line, co_line; // the even and the odd column
 	auto lift_pair = [&](int n, const float (&v)[4]) {
@@ -1406,7 +1488,7 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 	rows(1, b0, b1);
 	load_pair(a0, a1, ra);
 	load_pair(b0, b1, rb);
-#pragma unroll 1
+#pragma unroll 2
 	for (int n = 0; n < kPairs; n += 2)
 	{
 		float a[4], b[4];
@@ -1445,12 +1527,35 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const 
 	const int local = tile - J.tile_base;
 	const int ox0 = (local % J.tiles_x) * TW;
 	const int oy0 = (local / J.tiles_x) * TH;
+	// Band and output storage kinds as template arguments; the pairs that
+	// occur: fp16 LL and bands (FP16_LL levels 0-1) into a plane or an fp16
+	// LL; FP32 LL with fp16 bands (FP16 levels 0-1) into a plane or an FP32
+	// LL; FP32 throughout into a plane or an FP32 or fp16 LL (level 2 of
+	// FP16_LL writes the fp16 LL of level 1).
+	auto with_dst = [&](auto lk, auto hk) {
+		constexpr int L = decltype(lk)::value, H = decltype(hk)::value;
+		switch (J.dst_kind)
+		{
+		case kStoreU8:
+			inverse_stream_tile<L, H, kStoreU8>(J, frames, ox0, oy0, lane);
+			break;
+		case kStoreU16:
+			inverse_stream_tile<L, H, kStoreU16>(J, frames, ox0, oy0, lane);
+			break;
+		case kStoreF16:
+			inverse_stream_tile<L, H, kStoreF16>(J, frames, ox0, oy0, lane);
+			break;
+		default:
+			inverse_stream_tile<L, H, kStoreF32>(J, frames, ox0, oy0, lane);
+			break;
+		}
+	};
 	if (J.ll_kind == kStoreF16)
-		inverse_stream_tile<kStoreF16, kStoreF16>(J, frames, ox0, oy0, lane);
+		with_dst(std::integral_constant<int, kStoreF16>(), std::integral_constant<int, kStoreF16>());
 	else if (J.band_kind == kStoreF16)
-		inverse_stream_tile<kStoreF32, kStoreF16>(J, frames, ox0, oy0, lane);
+		with_dst(std::integral_constant<int, kStoreF32>(), std::integral_constant<int, kStoreF16>());
 	else
-		inverse_stream_tile<kStoreF32, kStoreF32>(J, frames, ox0, oy0, lane);
+		with_dst(std::integral_constant<int, kStoreF32>(), std::integral_constant<int, kStoreF32>());
 }
 
 // --- packed FP16 arithmetic, streamed (shape 5) ------------------------------
