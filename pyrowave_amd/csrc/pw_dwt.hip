@@ -1190,7 +1190,11 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const 
 // the mirrored rows.  LL_l values are rounded to the storage the separate
 // launches would keep them in (fp16 for FP16_LL), so the coefficients equal
 // those of the level-by-level launches bit for bit.
-template <int KIND, bool LL16>
+//
+// As in k_dwt_forward_stream: WORD_ROWS loads are a scalar row base plus a
+// fixed per-lane word offset, and the storage kinds (JH: level l's high
+// bands, KL / KH: level l + 1's LL and high bands) are template arguments.
+template <int KIND, bool WORD_ROWS, bool LL16, int JH, int KL, int KH>
 __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob &K, const void *src_ptr,
                                                    uint32_t pitch, uint32_t step, int ox1, int oy1, int lane)
 {
@@ -1216,17 +1220,33 @@ __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob
 	const bool store_x1 = lane >= 2 && lane < 2 + kFusedTileW1 && oc < K.bw;
 	const int B1 = K.A_y; // LL_l rows (= J.bh)
 
+	constexpr uint32_t kEs = stream_esize<KIND>();
+	const uint32_t be = src.lo + xe * kEs, bo = src.lo + xo * kEs;
+	const uint32_t we = be >> 2, wo = bo >> 2, se_b = 8u * (be & 3u), so_b = 8u * (bo & 3u);
+	// Row offsets: elements, or words for WORD_ROWS (exact: rows start on words).
 	auto row_base = [&](int r) __attribute__((always_inline)) {
-		return __umul24((uint32_t)fwd_ext(r, J.R_y, J.A_y), pitch);
+		const uint32_t el = __umul24((uint32_t)fwd_ext(r, J.R_y, J.A_y), pitch);
+		return WORD_ROWS ? (el * kEs) >> 2 : el;
 	};
 	auto load_pair = [&](int n, uint32_t (&r)[4], uint32_t &r0, uint32_t &r1) __attribute__((always_inline)) {
 		const int row = y0 - kApron + 2 * n;
 		r0 = row_base(row);
 		r1 = row_base(row + 1);
-		r[0] = stream_raw<KIND>(src, r0 + xe);
-		r[1] = stream_raw<KIND>(src, r0 + xo);
-		r[2] = stream_raw<KIND>(src, r1 + xe);
-		r[3] = stream_raw<KIND>(src, r1 + xo);
+		if constexpr (WORD_ROWS)
+		{
+			const GU32 p0 = src.words + r0, p1 = src.words + r1;
+			r[0] = p0[we];
+			r[1] = p0[wo];
+			r[2] = p1[we];
+			r[3] = p1[wo];
+		}
+		else
+		{
+			r[0] = stream_raw<KIND>(src, r0 + xe);
+			r[1] = stream_raw<KIND>(src, r0 + xo);
+			r[2] = stream_raw<KIND>(src, r1 + xe);
+			r[3] = stream_raw<KIND>(src, r1 + xo);
+		}
 	};
 	VLine lo, hi;   // level l
 	VLine lo1, hi1; // level l + 1
@@ -1280,16 +1300,29 @@ __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob
 		const int oy2 = oy1 + m - kApron;
 		if (m >= kApron && store_x1 && oy2 < K.bh)
 		{
-			const uint32_t idx = __umul24((uint32_t)oy2, (uint32_t)K.band_pitch) + (uint32_t)oc;
-			stream_store(K.band[0], K.ll_kind, idx, a_ll);
-			stream_store(K.band[1], K.band_kind, idx, a_hl);
-			stream_store(K.band[2], K.band_kind, idx, a_lh);
-			stream_store(K.band[3], K.band_kind, idx, a_hh);
+			const uint32_t row = (uint32_t)oy2 * (uint32_t)K.band_pitch;
+			stream_store_row<KL>(K.band[0], row, (uint32_t)oc, a_ll);
+			stream_store_row<KH>(K.band[1], row, (uint32_t)oc, a_hl);
+			stream_store_row<KH>(K.band[2], row, (uint32_t)oc, a_lh);
+			stream_store_row<KH>(K.band[3], row, (uint32_t)oc, a_hh);
 		}
 	};
 	auto lift_pair = [&](int n, uint32_t r0, uint32_t r1, const uint32_t (&r)[4]) __attribute__((always_inline)) {
-		float e0 = stream_value<KIND>(r[0], src, r0 + xe), o0 = stream_value<KIND>(r[1], src, r0 + xo);
-		float e1 = stream_value<KIND>(r[2], src, r1 + xe), o1 = stream_value<KIND>(r[3], src, r1 + xo);
+		float e0, o0, e1, o1;
+		if constexpr (WORD_ROWS)
+		{
+			e0 = stream_value_at<KIND>(r[0], se_b);
+			o0 = stream_value_at<KIND>(r[1], so_b);
+			e1 = stream_value_at<KIND>(r[2], se_b);
+			o1 = stream_value_at<KIND>(r[3], so_b);
+		}
+		else
+		{
+			e0 = stream_value<KIND>(r[0], src, r0 + xe);
+			o0 = stream_value<KIND>(r[1], src, r0 + xo);
+			e1 = stream_value<KIND>(r[2], src, r1 + xe);
+			o1 = stream_value<KIND>(r[3], src, r1 + xo);
+		}
 		hlift_row(e0, o0);
 		hlift_row(e1, o1);
 		float ll, lh, hl, hh;
@@ -1300,10 +1333,10 @@ __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob
 		const int j = 2 * oy1 - 8 + n;
 		if (n >= 8 && n < 8 + 2 * H1 && store_x && j < J.bh)
 		{
-			const uint32_t idx = __umul24((uint32_t)j, (uint32_t)J.band_pitch) + (uint32_t)ox;
-			stream_store(J.band[1], J.band_kind, idx, hl);
-			stream_store(J.band[2], J.band_kind, idx, lh);
-			stream_store(J.band[3], J.band_kind, idx, hh);
+			const uint32_t row = (uint32_t)j * (uint32_t)J.band_pitch;
+			stream_store_row<JH>(J.band[1], row, (uint32_t)ox, hl);
+			stream_store_row<JH>(J.band[2], row, (uint32_t)ox, lh);
+			stream_store_row<JH>(J.band[3], row, (uint32_t)ox, hh);
 		}
 		feed(n, ll);
 	};
@@ -1349,10 +1382,35 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_fused(const F
 	const FrameState &F = frames[J.frame];
 	const void *p = F.src[J.component];
 	const uint32_t pitch = (uint32_t)F.src_pitch[J.component], step = (uint32_t)F.src_step[J.component];
-	if (J.ll_kind == kStoreF16)
-		forward_fused_tile<kStoreU8, true>(J, K, p, pitch, step, ox1, oy1, lane);
+	// The storage-kind combinations that occur: FP32 throughout; FP16 (fp16
+	// high bands of levels 0-1, FP32 LL) fusing levels 0+1 or 1+2 (4:2:0
+	// chroma: level 2 all FP32); FP16_LL (all four bands of levels 0-1 fp16,
+	// so the carried LL rounds to fp16) fusing 0+1 or 1+2.
+	auto go = [&](auto wr) {
+		constexpr bool WR = decltype(wr)::value;
+		if (J.band_kind == kStoreF32)
+			forward_fused_tile<kStoreU8, WR, false, kStoreF32, kStoreF32, kStoreF32>(J, K, p, pitch, step, ox1, oy1,
+			                                                                          lane);
+		else if (J.ll_kind == kStoreF16)
+		{
+			if (K.ll_kind == kStoreF16)
+				forward_fused_tile<kStoreU8, WR, true, kStoreF16, kStoreF16, kStoreF16>(J, K, p, pitch, step, ox1,
+				                                                                         oy1, lane);
+			else
+				forward_fused_tile<kStoreU8, WR, true, kStoreF16, kStoreF32, kStoreF32>(J, K, p, pitch, step, ox1,
+				                                                                         oy1, lane);
+		}
+		else if (K.band_kind == kStoreF16)
+			forward_fused_tile<kStoreU8, WR, false, kStoreF16, kStoreF32, kStoreF16>(J, K, p, pitch, step, ox1, oy1,
+			                                                                          lane);
+		else
+			forward_fused_tile<kStoreU8, WR, false, kStoreF16, kStoreF32, kStoreF32>(J, K, p, pitch, step, ox1, oy1,
+			                                                                          lane);
+	};
+	if ((pitch & 3u) == 0)
+		go(std::true_type());
 	else
-		forward_fused_tile<kStoreU8, false>(J, K, p, pitch, step, ox1, oy1, lane);
+		go(std::false_type());
 }
 
 // --- inverse, streamed (shape 4) ------------------------------------------
