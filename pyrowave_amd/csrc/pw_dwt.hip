@@ -103,6 +103,10 @@ __device__ __forceinline__ int mirror_repeat(int t, int R)
 // gather footprint and a MIRRORED_REPEAT sampler).  The identity on [0, R).
 __device__ __forceinline__ int fwd_ext(int p, int R, int A)
 {
+	// Inside the valid input (0 <= p <= R - 2; A >= R) the extension is the
+	// identity: the common case, without the mirror arithmetic.
+	if ((unsigned)p < (unsigned)(R - 1))
+		return p;
 	int c = p & ~1;
 	int coord = c < 0 ? c : c + 1;
 	if (coord >= 2 * A - R)
@@ -1343,7 +1347,7 @@ __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob
 	uint32_t ra[4], rb[4], a0, a1, b0, b1;
 	load_pair(0, ra, a0, a1);
 	load_pair(1, rb, b0, b1);
-#pragma unroll 1
+#pragma unroll 2
 	for (int n = 0; n < kSteps; n += 2)
 	{
 		const uint32_t pa0 = a0, pa1 = a1, pb0 = b0, pb1 = b1;
