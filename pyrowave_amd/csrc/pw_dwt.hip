@@ -500,31 +500,42 @@ __device__ __forceinline__ bool plane_kind(int kind)
 	return kind == kStoreU8 || kind == kStoreU16;
 }
 
+// Raw buffer access for the streamed kernels: a buffer resource over a
+// band or plane base (scalar registers), a wave-uniform byte offset (the
+// row) and a fixed per-lane byte offset (the column), so a load or store
+// needs no address arithmetic on the vector unit.  The resource spans the
+// whole address range (every offset is in bounds by construction; a load
+// of the aligned word holding a sample may reach up to three bytes past a
+// plane's last sample, inside the same page).
+typedef unsigned int v2u32 __attribute__((ext_vector_type(2)));
+constexpr int kBufWord3 = 0x00020000; // raw buffer, 32-bit data format (gfx9 family)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_res(const void *p)
+{
+	return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, -1, kBufWord3);
+}
+
 // Two horizontally adjacent plane samples (px, px + 1) of the row that
-// starts `row` samples into the plane: a wave-uniform row address and a
-// fixed 32-bit per-lane offset (the store's scalar base + vector offset form).
+// starts `row` samples into the plane.
 template <int kind>
 __device__ __forceinline__ void store_plane_pair_row(uint8_t *plane, uint32_t row, int px, int R_x, float e, float o)
 {
-	typedef __attribute__((address_space(1))) uint8_t GU8;
 	constexpr uint32_t sb = kind == kStoreU16 ? 2u : 1u;
-	GU8 *const rb = (GU8 *)plane + (size_t)row * sb;
-	uint32_t off = (uint32_t)px * sb;
-	asm volatile("" : "+v"(off));
+	const __amdgpu_buffer_rsrc_t r = buf_res(plane);
+	const uint32_t voff = (uint32_t)px * sb, soff = row * sb;
 	if constexpr (kind == kStoreU16)
 	{
-		__attribute__((address_space(1))) uint16_t *d = (__attribute__((address_space(1))) uint16_t *)(rb + off);
 		if (px < R_x)
-			d[0] = unorm16_store(e);
+			__builtin_amdgcn_raw_buffer_store_b16(unorm16_store(e), r, voff, soff, 0);
 		if (px + 1 < R_x)
-			d[1] = unorm16_store(o);
+			__builtin_amdgcn_raw_buffer_store_b16(unorm16_store(o), r, voff, soff + 2u, 0);
 	}
 	else
 	{
 		if (px < R_x)
-			rb[off] = unorm8_store(e);
+			__builtin_amdgcn_raw_buffer_store_b8(unorm8_store(e), r, voff, soff, 0);
 		if (px + 1 < R_x)
-			rb[off + 1] = unorm8_store(o);
+			__builtin_amdgcn_raw_buffer_store_b8(unorm8_store(o), r, voff, soff + 1u, 0);
 	}
 }
 
@@ -895,6 +906,12 @@ constexpr uint32_t stream_esize()
 	return KIND == kStoreU8 ? 1u : (KIND == kStoreF16 || KIND == kStoreU16) ? 2u : 4u;
 }
 
+// Word `w` of the row that starts `row_w` words into a word-aligned source.
+__device__ __forceinline__ uint32_t load_row_word(GU32 base, uint32_t row_w, uint32_t w)
+{
+	return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(buf_res((const void *)base), w << 2, row_w << 2, 0);
+}
+
 // Every format is fetched as the aligned 32-bit word holding the sample (a
 // byte or half is extracted when it is converted): a narrower load's
 // zero-extension would be placed right after it by the compiler and wait for
@@ -943,21 +960,17 @@ __device__ __forceinline__ float stream_value_at(uint32_t raw, uint32_t shift)
 template <int kind>
 __device__ __forceinline__ void stream_store_row(void *base, uint32_t row, uint32_t col, float v)
 {
-	typedef __attribute__((address_space(1))) char GChar;
-	GChar *const rb = (GChar *)base + (size_t)row * (kind == kStoreF16 ? 2u : 4u);
-	// Kept as a 32-bit value at the store (not a 64-bit offset hoisted out of
-	// the loop), so that the store takes the scalar base + vector offset form.
-	uint32_t off = col << (kind == kStoreF16 ? 1 : 2);
-	asm volatile("" : "+v"(off));
+	constexpr uint32_t es = kind == kStoreF16 ? 2u : 4u;
+	const __amdgpu_buffer_rsrc_t r = buf_res(base);
 	if constexpr (kind == kStoreF16)
 	{
 		// The barrier keeps the compiler from merging the product that made
 		// v with this conversion (v_fma_mix rounds once, straight to fp16).
 		asm volatile("" : "+v"(v));
-		*(__attribute__((address_space(1))) uint16_t *)(rb + off) = __half_as_ushort(__float2half_rn(v));
+		__builtin_amdgcn_raw_buffer_store_b16(__half_as_ushort(__float2half_rn(v)), r, col * es, row * es, 0);
 	}
 	else
-		*(__attribute__((address_space(1))) float *)(rb + off) = v;
+		__builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, col * es, row * es, 0);
 }
 
 // Elements col and col + 1 (col even) of a band row as one store: fp32 pairs
@@ -965,19 +978,20 @@ __device__ __forceinline__ void stream_store_row(void *base, uint32_t row, uint3
 template <int kind>
 __device__ __forceinline__ void stream_store_pair_row(void *base, uint32_t row, uint32_t col, float e, float o)
 {
-	typedef __attribute__((address_space(1))) char GChar;
-	GChar *const rb = (GChar *)base + (size_t)row * (kind == kStoreF16 ? 2u : 4u);
-	uint32_t off = col << (kind == kStoreF16 ? 1 : 2);
-	asm volatile("" : "+v"(off));
+	constexpr uint32_t es = kind == kStoreF16 ? 2u : 4u;
+	const __amdgpu_buffer_rsrc_t r = buf_res(base);
 	if constexpr (kind == kStoreF16)
 	{
 		asm volatile("" : "+v"(e), "+v"(o)); // two roundings, as in stream_store_row
-		*(__attribute__((address_space(1))) uint32_t *)(rb + off) =
+		const uint32_t packed =
 		    (uint32_t)__half_as_ushort(__float2half_rn(e)) | ((uint32_t)__half_as_ushort(__float2half_rn(o)) << 16);
+		__builtin_amdgcn_raw_buffer_store_b32(packed, r, col * es, row * es, 0);
 	}
 	else
-		*(__attribute__((address_space(1))) uint64_t *)(rb + off) =
-		    (uint64_t)__float_as_uint(e) | ((uint64_t)__float_as_uint(o) << 32);
+	{
+		const v2u32 pair = {__float_as_uint(e), __float_as_uint(o)};
+		__builtin_amdgcn_raw_buffer_store_b64(pair, r, col * es, row * es, 0);
+	}
 }
 
 __device__ __forceinline__ void stream_store(void *base, int kind, uint32_t idx, float v)
@@ -1053,11 +1067,10 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
 		if constexpr (WORD_ROWS)
 		{
-			const GU32 p0 = src.words + r0, p1 = src.words + r1;
-			r[0] = p0[we];
-			r[1] = p0[wo];
-			r[2] = p1[we];
-			r[3] = p1[wo];
+			r[0] = load_row_word(src.words, r0, we);
+			r[1] = load_row_word(src.words, r0, wo);
+			r[2] = load_row_word(src.words, r1, we);
+			r[3] = load_row_word(src.words, r1, wo);
 		}
 		else
 		{
@@ -1238,11 +1251,10 @@ __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob
 		r1 = row_base(row + 1);
 		if constexpr (WORD_ROWS)
 		{
-			const GU32 p0 = src.words + r0, p1 = src.words + r1;
-			r[0] = p0[we];
-			r[1] = p0[wo];
-			r[2] = p1[we];
-			r[3] = p1[wo];
+			r[0] = load_row_word(src.words, r0, we);
+			r[1] = load_row_word(src.words, r0, wo);
+			r[2] = load_row_word(src.words, r1, we);
+			r[3] = load_row_word(src.words, r1, wo);
 		}
 		else
 		{
@@ -1508,10 +1520,10 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 	const uint32_t wle = (ce * kEl) >> 2, whe = (ce * kEh) >> 2, who = (co * kEh) >> 2;
 	const uint32_t sle = 8u * ((ce * kEl) & 3u), she = 8u * ((ce * kEh) & 3u), sho = 8u * ((co * kEh) & 3u);
 	auto load_pair = [&](uint32_t r0, uint32_t r1, uint32_t (&r)[4]) {
-		r[0] = (ll + ((r0 * kEl) >> 2))[wle];
-		r[1] = (hl + ((r0 * kEh) >> 2))[who];
-		r[2] = (lh + ((r1 * kEh) >> 2))[whe];
-		r[3] = (hh + ((r1 * kEh) >> 2))[who];
+		r[0] = load_row_word(ll, (r0 * kEl) >> 2, wle);
+		r[1] = load_row_word(hl, (r0 * kEh) >> 2, who);
+		r[2] = load_row_word(lh, (r1 * kEh) >> 2, whe);
+		r[3] = load_row_word(hh, (r1 * kEh) >> 2, who);
 	};
 	// Scaled for the vertical pass: even rows by K, odd rows by 1/K.
 	auto values = [&](uint32_t r0, uint32_t r1, const uint32_t (&r)[4], float (&v)[4]) {
