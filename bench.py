@@ -80,25 +80,47 @@ def traffic_profile(batch):
     return os.path.join(ROOT, "profiles", f"traffic_4k420_fp16_b{batch}.txt")
 
 
-def measured_traffic(direction, shapes, batch):
+def measured_traffic(direction, shapes, batch, fused):
     """HBM bytes per launch of one transform direction ("forward" or
     "inverse") with the given tile shape per level, at this batch size: the
-    mean over the five levels of the committed rocprofv3 PMC figures (lines
-    `traffic_level <direction> <level> <shape> <bytes>`), or None when the
-    profile of that batch size, or a level's figure, is missing."""
+    mean over the stage's launches of the committed rocprofv3 PMC figures
+    (lines `traffic_level <direction> <level> <shape> <bytes>`; with the
+    fused forward, `traffic_fused forward <bytes>` for the launch of levels
+    0+1 and `traffic_fused_level forward 2 <shape> <bytes>` for the level-2
+    launch, which then holds luma alone), or None when the profile of that
+    batch size, or a launch's figure, is missing."""
     table = {}
     try:
         with open(traffic_profile(batch)) as f:
             for line in f:
                 parts = line.split()
-                if len(parts) == 5 and parts[0] == "traffic_level" and parts[1] == direction:
-                    table[(int(parts[2]), int(parts[3]))] = int(parts[4])
+                if len(parts) == 5 and parts[0] in ("traffic_level", "traffic_fused_level") and parts[1] == direction:
+                    table[(parts[0], int(parts[2]), int(parts[3]))] = int(parts[4])
+                elif len(parts) == 3 and parts[0] == "traffic_fused" and parts[1] == direction:
+                    table[("traffic_fused",)] = int(parts[2])
     except OSError:
         return None
-    per_level = [table.get((level, shape)) for level, shape in enumerate(shapes)]
-    if not per_level or None in per_level:
+    if fused:
+        launches = [table.get(("traffic_fused",)), table.get(("traffic_fused_level", 2, shapes[2]))]
+        launches += [table.get(("traffic_level", level, shapes[level])) for level in (3, 4)]
+    else:
+        launches = [table.get(("traffic_level", level, shape)) for level, shape in enumerate(shapes)]
+    if not launches or None in launches:
         return None
-    return int(round(sum(per_level) / len(per_level)))
+    return int(round(sum(launches) / len(launches)))
+
+
+# Fused forward tiles (kFusedTileW1 x kFusedTileH1 = 26 x 32 level-(l+1)
+# coefficients) of one 3840x2160 4:2:0 frame: luma levels 0+1 (960 x 544 per
+# band at level 1), each chroma plane levels 1+2 (480 x 272).
+FUSED_TILES_4K420 = -(-960 // 26) * -(-544 // 32) + 2 * (-(-480 // 26) * -(-272 // 32))
+FUSED_MIN_TILES = 16384  # the library fuses launches of at least this many tiles
+
+
+def fused_forward(batch):
+    """Whether a batch of 4K 4:2:0 frames takes the fused levels-0+1 launch."""
+    mode = int(os.environ.get("PYROWAVE_FUSED", "1"), 0)
+    return mode >= 2 or (mode == 1 and FUSED_TILES_4K420 * batch >= FUSED_MIN_TILES)
 
 
 def dwt_algorithmic_bytes(fp16):
@@ -239,6 +261,23 @@ def pmc_summary(fetch_csv, write_csv):
                 wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
                 print(f"traffic_level {direction} {level} {shape} {int(round(fv + wv, -3))}")
 
+    def row(key):
+        fv = sum(fagg[key]) / len(fagg[key]) * ff * 1024
+        wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
+        return int(round(fv + wv, -3))
+
+    # The fused forward schedule of the headline batch: levels 0+1 in one
+    # launch, then luma level 2 alone, then levels 3 and 4 as above.
+    key = ("k_dwt_forward_fused", -(-FUSED_TILES_4K420 * 32 // 4))
+    if key in fagg:
+        print(f"traffic_fused forward {row(key)}")
+    else:
+        print(f"# missing fused forward ({key[0]}, {key[1]} workgroups)")
+    for shape, (kernel, tw, th, per_wg) in enumerate(TRANSFORM_SHAPES["forward"]):
+        key = (kernel, -(-transform_tiles(2, tw, th, 32, planes=1) // per_wg))
+        if key in fagg:
+            print(f"traffic_fused_level forward 2 {shape} {row(key)}")
+
 
 # Transform tile shapes by index: (kernel, tile width, tile height, tiles per workgroup).
 TRANSFORM_SHAPES = {
@@ -251,12 +290,13 @@ TRANSFORM_SHAPES = {
 }
 
 
-def transform_tiles(level, tw, th, batch):
+def transform_tiles(level, tw, th, batch, planes=None):
     """Tiles of one transform launch over a batch of 4K 4:2:0 frames: level 0
     is the luma plane alone; launch l > 0 runs luma level l with chroma level
-    l - 1, three planes of the same (aligned) size."""
+    l - 1, three planes of the same (aligned) size (`planes` overrides)."""
     w, h = 3840 >> level, 2176 >> level
-    planes = 1 if level == 0 else 3
+    if planes is None:
+        planes = 1 if level == 0 else 3
     return planes * -(-w // tw) * -(-h // th) * batch
 
 
@@ -557,25 +597,33 @@ def main():
         psnr[name] = float(np.mean(vals[:4]))
 
     # Rooflines of the transform kernels from the events of the timed region.
-    # One batch runs LEVELS launches of each kernel; a launch's algorithmic
-    # bytes are the batch's compulsory bytes / LEVELS, its duration the
-    # stage time / LEVELS, so achieved = compulsory bytes / stage time.
-    def kernels_of(base, shapes):
-        """The kernels one transform stage launches: shape 4 is the streamed kernel."""
-        return "+".join(sorted({base + "_stream" if v == 4 else base for v in shapes}))
+    # One batch runs one launch per level (the inverse, and the forward
+    # level by level) or, with the fused forward, four: levels 0+1, luma
+    # level 2, levels 3 and 4.  A launch's algorithmic bytes are the batch's
+    # compulsory bytes / launches, its duration the stage time / launches,
+    # so achieved = compulsory bytes / stage time.
+    fused = fused_forward(B)
 
-    def roofline(stage, kernel, direction, shapes):
+    def kernels_of(base, shapes, fused_first=False):
+        """The kernels one transform stage launches: shape 4 is the streamed kernel."""
+        names = {base + "_stream" if v == 4 else base for v in (shapes[2:] if fused_first else shapes)}
+        if fused_first:
+            names.add(base + "_fused")
+        return "+".join(sorted(names))
+
+    def roofline(stage, kernel, direction, shapes, fused_first=False):
         ms, frames = fp16["roof"][stage]
-        launches = args.steps * LEVELS
+        per_batch = LEVELS - 1 if fused_first else LEVELS
+        launches = args.steps * per_batch
         per_frame = dwt_algorithmic_bytes(True)
         bytes_per_launch = per_frame * frames / max(launches, 1)
         us_per_launch = ms * 1e3 / max(launches, 1)
         achieved = bytes_per_launch / (us_per_launch * 1e-6) / 1e9
         return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(direction, shapes, args.batch),
+                "traffic": measured_traffic(direction, shapes, args.batch, fused_first),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
-                "launches_timed": launches}
+                "launches_timed": launches, "launches_per_batch": per_batch}
 
     # Launch-bound case: one 1920x1080 4:2:0 frame per call (crop of the
     # resident sequence), FP16, eager launches vs a replayed hipGraph.
@@ -749,8 +797,8 @@ def main():
         "stage_us_per_frame_fp16": stage_us,
         "stage_us_per_frame_fp32_single": stage_us32,
         "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1]), "inverse", fp16["tuned"][1]),
-        "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0]), "forward",
-                                     fp16["tuned"][0]),
+        "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0], fused), "forward",
+                                     fp16["tuned"][0], fused),
         "fp16_ll_reference_storage": None if fp16_ll is None else {
             "us_per_frame": round(fp16_ll["wall"] / (args.steps * B * args.streams) * 1e6, 2),
             "psnr_y_db": round(psnr["fp16_ll"], 4), "delta_psnr_db": round(psnr["fp32"] - psnr["fp16_ll"], 4),

@@ -1068,28 +1068,22 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	const uint32_t out_planes = active ? ((cw >> (2 * s)) & 3u) + out_q : 0u;
 	// Bit planes, MSB first: plane byte bit p is bit `bit` of magnitude p.
 	// Magnitudes p and p+4 share a register (low / high half), so one plane
-	// of all eight is four shift-and-mask steps and a fold.
+	// of all eight is four shift-and-mask steps and a fold.  The planes kept
+	// are bits in_planes - 1 down to in_planes - out_planes, so a magnitude
+	// is significant (its sign is sent) when it has a bit at or above
+	// in_planes - out_planes; the plane bytes themselves are extracted once
+	// their LDS position is known, after the scan below.
 	uint32_t pair[4];
 #pragma unroll
 	for (int i = 0; i < 4; i++)
 		pair[i] = (uint32_t)Q.n[i] | ((uint32_t)Q.n[i + 4] << 16);
 	uint32_t sig = 0;
-	uint32_t plane_bytes_lane[14];
-#pragma unroll
-	for (int pi = 0; pi < 14; pi++)
+	if (out_planes)
 	{
-		uint32_t byte = 0;
-		if ((uint32_t)pi < out_planes)
-		{
-			const int bit = (int)in_planes - 1 - pi;
-			uint32_t c = 0;
+		const uint32_t drop = in_planes - out_planes;
 #pragma unroll
-			for (int i = 0; i < 4; i++)
-				c |= ((pair[i] >> bit) & 0x00010001u) << i;
-			byte = (c & 0xfu) | ((c >> 12) & 0xf0u);
-		}
-		plane_bytes_lane[pi] = byte;
-		sig |= byte;
+		for (int p = 0; p < 8; p++)
+			sig |= ((uint32_t)Q.n[p] >> drop) != 0u ? 1u << p : 0u;
 	}
 	const int nsig = __popc(sig);
 	const int cost8 = cluster_sum_i(8 * (int)out_planes + nsig);
@@ -1127,10 +1121,15 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 		lds_store_byte(s_buf, 9 + 2 * j, (cw >> 8) & 0xffu);
 		lds_store_byte(s_buf, 8 + 2 * N + j, (cw >> 16) & 0xffu);
 	}
+	for (uint32_t pi = 0; pi < out_planes; pi++)
+	{
+		const int bit = (int)in_planes - 1 - (int)pi;
+		uint32_t c = 0;
 #pragma unroll
-	for (int pi = 0; pi < 14; pi++)
-		if ((uint32_t)pi < out_planes)
-			lds_store_byte(s_buf, plane_base + (uint32_t)plane_excl + pi, plane_bytes_lane[pi]);
+		for (int i = 0; i < 4; i++)
+			c |= ((pair[i] >> bit) & 0x00010001u) << i;
+		lds_store_byte(s_buf, plane_base + (uint32_t)plane_excl + pi, (c & 0xfu) | ((c >> 12) & 0xf0u));
+	}
 	if (sig)
 	{
 		uint32_t bits = 0;

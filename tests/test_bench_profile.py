@@ -3,7 +3,9 @@
 bench.py reports `traffic` for whatever tile shapes its autotuning run keeps,
 so the profile of the default batch size must hold a figure for every
 (direction, level, shape) the autotuner can choose; an average over the
-shapes a run kept is then always available.
+shapes a run kept is then always available.  The fused forward schedule
+(levels 0+1 in one launch, then luma level 2 alone) has its own figures for
+the fused launch and the level-2 shapes the profiled run kept.
 """
 import os
 import sys
@@ -18,12 +20,26 @@ def test_traffic_profile_covers_every_shape():
     assert os.path.exists(bench.traffic_profile(batch))
     for direction in ("forward", "inverse"):
         for shape in range(5):
-            t = bench.measured_traffic(direction, [shape] * 5, batch)
+            t = bench.measured_traffic(direction, [shape] * 5, batch, False)
             assert t is not None, (direction, shape)
             # At least the compulsory bytes of one launch, at most a few times them.
             algorithmic = bench.dwt_algorithmic_bytes(True) * batch / bench.LEVELS
             assert algorithmic <= t < 4 * algorithmic, (direction, shape, t)
     # A mix per level, as an autotuned run keeps.
-    assert bench.measured_traffic("inverse", [4, 4, 4, 2, 1], batch) is not None
+    assert bench.measured_traffic("inverse", [4, 4, 4, 2, 1], batch, False) is not None
     # No profile for another batch size: no figure.
-    assert bench.measured_traffic("inverse", [0] * 5, 7) is None
+    assert bench.measured_traffic("inverse", [0] * 5, 7, False) is None
+
+
+def test_traffic_profile_fused_forward():
+    batch = 32
+    assert bench.fused_forward(batch)
+    with open(bench.traffic_profile(batch)) as f:
+        lines = [l.split() for l in f]
+    assert any(p[:2] == ["traffic_fused", "forward"] for p in lines)
+    kept = [int(p[3]) for p in lines if p[:3] == ["traffic_fused_level", "forward", "2"]]
+    assert kept
+    for shape in kept:
+        t = bench.measured_traffic("forward", [0, 0, shape, shape, shape], batch, True)
+        algorithmic = bench.dwt_algorithmic_bytes(True) * batch / (bench.LEVELS - 1)
+        assert t is not None and algorithmic <= t < 4 * algorithmic, (shape, t)
