@@ -19,6 +19,15 @@
  *                                  band orientation, brute-force convolution
  *                                  with the textbook 9/7 analysis filters
  *   FP32 DWT / iDWT .............. error bound against the FP64 transform
+ *   16-bit (R16_UNORM) input ..... planes of 257 * u give the 8-bit transform
+ *                                  and bitstream exactly; the level-0 sample
+ *                                  against exact rationals
+ *   16-bit output ................ round(clamp(v) * 65535) of the FP64
+ *                                  inverse within one unit
+ *                                  (tests/test_oracle_unorm16.py)
+ *   precision 0 (fp16 math) ...... numpy float16 arithmetic step by step
+ *                                  and the FP64 transform within fp16 bounds
+ *                                  (tests/test_oracle_fp16.py)
  *   fp16 conversion .............. exhaustive comparison with numpy float16
  *   quant scale codes ............ exact-rational minimal code keeping the
  *                                  block peak under 2^e - 1/4
