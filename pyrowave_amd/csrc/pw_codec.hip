@@ -360,59 +360,83 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 		s_msb[k8] = msb;
 		s_disc[k8] = msb >= 0 ? round_f16_nonneg(fminf(dd, 60000.0f)) : 0.0f;
 	}
-	__syncthreads();
-	int qmax = -1;
-#pragma unroll
-	for (int k = 0; k < 16; k++)
-		qmax = s_msb[k] > qmax ? s_msb[k] : qmax;
+	// Each wave runs the levels of its own eight 8x8 blocks (the analyze below
+	// reads level q of a block only for q <= its msb): the largest msb of the
+	// wave, equal within each 8-lane cluster.
+	int qmax = msb;
+	qmax = max(qmax, __shfl_xor(qmax, 8, 64));
+	qmax = max(qmax, __shfl_xor(qmax, 16, 64));
+	qmax = max(qmax, __shfl_xor(qmax, 32, 64));
+	qmax = __builtin_amdgcn_readfirstlane(qmax);
 
 	float a[8];
 #pragma unroll
 	for (int p = 0; p < 8; p++)
 		a[p] = fabsf(Q.t[p]);
-	for (int q = 0; q <= qmax; q++)
+	// Early / late plane count of the lane's subblock at quant level q.
+	auto late_planes = [&](int q) {
+		int b = Q.b4max >> q;
+		int early = b > 0 ? 1 : 0;
+		const int m = msb - q;
+		if (m >= 3)
+		{
+			const int qp = m - 2;
+			early = qp + 1;
+			b >>= qp;
+		}
+		early += find_msb((uint32_t)b) + 1;
+		return early - 1 > 0 ? early - 1 : 0;
+	};
+	// Level 0: cost only (its distortion is stored as 0, as the reference does).
+	{
+		int packed = 0;
+		if (msb >= 0)
+		{
+			int nsig = 0;
+#pragma unroll
+			for (int p = 0; p < 8; p++)
+				nsig += Q.n[p] != 0 ? 1 : 0;
+			packed = late_planes(0) | (nsig << 16);
+		}
+		const int sums = cluster_sum_i(packed);
+		if (s == 0 && msb >= 0)
+		{
+			s_cost[0][k8] = 8 * (sums & 0xffff) + (sums >> 16);
+			s_dist[0][k8] = 0.0f;
+		}
+	}
+	for (int q = 1; q <= qmax; q++)
 	{
 		// compute_quant_stats (reference shaders/wavelet_quant.comp:113-165).
 		float se_lane = 0.0f;
 		int packed = 0; // late planes | significant count << 16
 		if (q <= msb)
 		{
+			// A magnitude n is significant at level q when n >= 2^q; its
+			// reconstruction (floor(n / 2^q) + 1/2) * 2^q is the integer
+			// (n with the low q bits cleared) | 2^(q-1), exact in float.
+			const uint32_t lim = 1u << q, keep = ~(lim - 1u), half_bit = lim >> 1;
 			float err[8];
 			int nsig = 0;
-			const float half_step = q ? (float)(1 << (q - 1)) : 0.0f;
 #pragma unroll
 			for (int p = 0; p < 8; p++)
 			{
-				int iv = Q.n[p] >> q;
-				float recon = 0.0f;
-				if (iv != 0)
-				{
-					nsig++;
-					// (iv + 0.5) * 2^q, or iv at q = 0
-					recon = (float)(iv << q) + half_step;
-				}
+				const uint32_t n = (uint32_t)Q.n[p];
+				const bool nz = n >= lim;
+				nsig += nz ? 1 : 0;
+				const float recon = nz ? (float)((n & keep) | half_bit) : 0.0f;
 				err[p] = a[p] - recon;
 			}
 			se_lane = ((dot4_sq(err[0], err[1], err[2], err[3]) + dot4_sq(err[4], err[5], err[6], err[7])) *
 			           J.rdo_scale) * Q.iq2;
-			int b = Q.b4max >> q;
-			int early = b > 0 ? 1 : 0;
-			int m = msb - q;
-			if (m >= 3)
-			{
-				int qp = m - 2;
-				early = qp + 1;
-				b >>= qp;
-			}
-			early += find_msb((uint32_t)b) + 1;
-			packed = (early - 1 > 0 ? early - 1 : 0) | (nsig << 16);
+			packed = late_planes(q) | (nsig << 16);
 		}
 		float se = cluster_sum_f(se_lane);
 		int sums = cluster_sum_i(packed);
 		if (s == 0 && q <= msb)
 		{
 			s_cost[q][k8] = 8 * (sums & 0xffff) + (sums >> 16);
-			s_dist[q][k8] = q == 0 ? 0.0f : round_f16_nonneg(fminf(se, 60000.0f));
+			s_dist[q][k8] = round_f16_nonneg(fminf(se, 60000.0f));
 		}
 	}
 	__syncthreads();
