@@ -206,6 +206,12 @@ constexpr int kFusedTileH1 = 32;
 // pays only when there are waves enough to hide that, i.e. batches of about
 // seventeen 4K frames or more; smaller launches run level by level.
 constexpr int kFusedMinTiles = 16384;
+// The inverse counterpart (k_dwt_inverse_fused): a FusedJob's a is level l
+// (writes the component's plane), b level l + 1 (reads its four bands); a
+// tile is kInvFusedTileW x kInvFusedTileH plane samples.
+constexpr int kInvFusedTileW = 120;
+constexpr int kInvFusedTileH = 64;
+constexpr int kInvFusedMinTiles = 0;
 
 // One transmitted band of one frame, for the per-32x32-block kernels.
 struct BandJob
@@ -305,6 +311,8 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                         int variant, hipStream_t stream);
 void launch_dwt_forward_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                              hipStream_t stream);
+void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                               hipStream_t stream);
 // blk_band: two words per block (band job index, bx | by << 16).
 void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,

@@ -1632,6 +1632,246 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const 
 		with_dst(std::integral_constant<int, kStoreF32>(), std::integral_constant<int, kStoreF32>());
 }
 
+// --- inverse, levels l + 1 and l fused ---------------------------------------
+//
+// The inverse counterpart of k_dwt_forward_fused: one wave per tile of
+// kInvFusedTileW x kInvFusedTileH samples of a component's plane (level l,
+// the component's first level), with two delay lines.  The level-(l+1) line
+// walks the level-(l+1) bands under the tile's LL_l columns (lanes 0 .. 35;
+// lanes 36 .. 63 repeat lane 35's loads) and finishes two LL_l rows per row
+// pair; each finished LL_l row is gathered (ds_bpermute) from the lane that
+// holds each column to the level-l lane that reads it, and enters level l's
+// line with that level's HL / LH / HH rows, as the LL_l rows the separate
+// launch would read back.  So LL_l is never written nor read.
+//
+// Extension of LL_l as level l reads it (the interleaved signal mirrored
+// about 0 and A - 1: LL_l rows and columns mirror about 0 and about
+// bw - 1/2): left and right, the gather takes the mirrored column; above the
+// plane, the LL_l rows the level-(l+1) line computes from its own mirrored
+// input are already the mirror images (whole-sample symmetry about 0
+// survives every lifting step, down to the order of each float addition);
+// past the bottom, where the level-(l+1) signal mirrors about bh - 1
+// instead, the last LL_l row pair stands in for the next one in reverse
+// order (rows bh and bh + 1 are the only ones past the bottom that reach a
+// plane sample).  LL_l values are rounded to the storage the separate
+// launches keep them in (fp16 for FP16_LL), so the planes equal those of the
+// level-by-level launches bit for bit.
+__device__ __forceinline__ int inv_ext_fast(int p, int N)
+{
+	return (unsigned)p < (unsigned)N ? p : inv_ext(p, N);
+}
+
+template <int KL, int KH, bool LL16, int JH>
+__device__ __forceinline__ void inverse_fused_tile(const DwtJob &J, const DwtJob &K, const FrameState *frames,
+                                                   int ox0, int oy0, int lane)
+{
+	constexpr int TH = kInvFusedTileH;
+	constexpr int kPairs0 = TH / 2 + kApron; // level-l row pairs, as a streamed tile's
+	static_assert(TH % 4 == 0, "LL_l row pairs start on even rows");
+	static_assert(kInvFusedTileW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
+	static_assert((kPairs0 / 2) % 2 == 0, "the main loop is unrolled by two");
+	const uint32_t pitch0 = (uint32_t)J.band_pitch, pitch1 = (uint32_t)K.band_pitch;
+	const int c0 = ox0 >> 1; // the tile's first LL_l column (even)
+	// Level l: lane k holds the interleaved column pair ox0 - 4 + 2k, +1.
+	const int xe0 = inv_ext(ox0 - kApron + 2 * lane, J.A_x), xo0 = inv_ext(ox0 - kApron + 2 * lane + 1, J.A_x);
+	const uint32_t ce0 = (uint32_t)(xe0 >> 1), co0 = (uint32_t)(xo0 >> 1);
+	const int px = ox0 - kApron + 2 * lane;
+	const bool lane_out = lane >= kApron / 2 && lane < 64 - kApron / 2;
+	// Level l + 1: lane j holds the interleaved column pair c0 - 6 + 2j, +1,
+	// whose horizontal inverse is LL_l columns c0 - 6 + 2j, +1; lanes 2 .. 33
+	// are complete, LL_l columns c0 - 2 .. c0 + 61.
+	const int j1 = lane < 36 ? lane : 35;
+	const int xe1 = inv_ext(c0 - 6 + 2 * j1, K.A_x), xo1 = inv_ext(c0 - 5 + 2 * j1, K.A_x);
+	const uint32_t ce1 = (uint32_t)(xe1 >> 1), co1 = (uint32_t)(xo1 >> 1);
+	// Level-l lane k reads LL_l column ce0 (c0 - 2 + k extended), which lane
+	// (ce0 - c0 + 6) / 2 holds as its even (ce0 even) or odd column.  For
+	// every lane whose values reach a stored sample that lane is in 2 .. 33;
+	// the clamp only keeps the others' addresses in the wave.
+	const int gsrc = ((int)ce0 - c0 + 6) >> 1;
+	const int gaddr = (gsrc < 0 ? 0 : gsrc > 63 ? 63 : gsrc) << 2;
+	const bool godd = (ce0 & 1u) != 0;
+
+	uint8_t *plane = frames[J.frame].dst[J.component];
+	const uint32_t dpitch = (uint32_t)frames[J.frame].dst_pitch[J.component];
+	constexpr uint32_t kEl1 = stream_esize<KL>(), kEh1 = stream_esize<KH>(), kEh0 = stream_esize<JH>();
+	const GU32 ll1 = (GU32)K.band[0], hl1 = (GU32)K.band[1], lh1 = (GU32)K.band[2], hh1 = (GU32)K.band[3];
+	const GU32 hl0 = (GU32)J.band[1], lh0 = (GU32)J.band[2], hh0 = (GU32)J.band[3];
+	const uint32_t wle1 = (ce1 * kEl1) >> 2, whe1 = (ce1 * kEh1) >> 2, who1 = (co1 * kEh1) >> 2;
+	const uint32_t sle1 = 8u * ((ce1 * kEl1) & 3u), she1 = 8u * ((ce1 * kEh1) & 3u),
+	               sho1 = 8u * ((co1 * kEh1) & 3u);
+	const uint32_t whe0 = (ce0 * kEh0) >> 2, who0 = (co0 * kEh0) >> 2;
+	const uint32_t she0 = 8u * ((ce0 * kEh0) & 3u), sho0 = 8u * ((co0 * kEh0) & 3u);
+
+	// Band-row offsets (elements) of the interleaved rows y (even) and y + 1;
+	// wave-uniform, so scalar.
+	auto rows = [&](int y, int A, uint32_t pitch, uint32_t &r0, uint32_t &r1) __attribute__((always_inline)) {
+		r0 = (uint32_t)(inv_ext_fast(y, A) >> 1) * pitch;
+		r1 = (uint32_t)(inv_ext_fast(y + 1, A) >> 1) * pitch;
+	};
+	// Level-(l+1) row pair m: interleaved rows oy0 / 2 - 6 + 2m, +1 (LL, HL;
+	// LH, HH).  Level-l row pair n: rows oy0 - 4 + 2n, +1 (HL; LH, HH; its LL
+	// comes from the gather).
+	auto load1 = [&](int m, uint32_t (&r)[4]) __attribute__((always_inline)) {
+		uint32_t r0, r1;
+		rows((oy0 >> 1) - 6 + 2 * m, K.A_y, pitch1, r0, r1);
+		r[0] = load_row_word(ll1, (r0 * kEl1) >> 2, wle1);
+		r[1] = load_row_word(hl1, (r0 * kEh1) >> 2, who1);
+		r[2] = load_row_word(lh1, (r1 * kEh1) >> 2, whe1);
+		r[3] = load_row_word(hh1, (r1 * kEh1) >> 2, who1);
+	};
+	auto load0 = [&](int n, uint32_t (&r)[3]) __attribute__((always_inline)) {
+		uint32_t r0, r1;
+		rows(oy0 - kApron + 2 * n, J.A_y, pitch0, r0, r1);
+		r[0] = load_row_word(hl0, (r0 * kEh0) >> 2, who0);
+		r[1] = load_row_word(lh0, (r1 * kEh0) >> 2, whe0);
+		r[2] = load_row_word(hh0, (r1 * kEh0) >> 2, who0);
+	};
+
+	VLineInv e1_line, o1_line; // level l + 1: the even and the odd column
+	VLineInv e0_line, o0_line; // level l
+	// Level-(l+1) row pair: the vertical delay line, and once it yields
+	// finished rows (OUT), their horizontal inverse and the gather.
+	auto step1 = [&](auto out, const uint32_t (&r)[4], float &g0, float &g1) __attribute__((always_inline)) {
+		const float v0 = stream_value_at<KL>(r[0], sle1) * kK;
+		const float v1 = stream_value_at<KH>(r[1], sho1) * kK;
+		const float v2 = stream_value_at<KH>(r[2], she1) * kInvK;
+		const float v3 = stream_value_at<KH>(r[3], sho1) * kInvK;
+		float a0, a1, b0, b1;
+		e1_line.push(v0, v2, a0, a1);
+		o1_line.push(v1, v3, b0, b1);
+		if constexpr (decltype(out)::value)
+		{
+			hlift_inv_row(a0, b0);
+			hlift_inv_row(a1, b1);
+			const float ge0 = __int_as_float(__builtin_amdgcn_ds_bpermute(gaddr, __float_as_int(a0)));
+			const float go0 = __int_as_float(__builtin_amdgcn_ds_bpermute(gaddr, __float_as_int(b0)));
+			const float ge1 = __int_as_float(__builtin_amdgcn_ds_bpermute(gaddr, __float_as_int(a1)));
+			const float go1 = __int_as_float(__builtin_amdgcn_ds_bpermute(gaddr, __float_as_int(b1)));
+			g0 = godd ? go0 : ge0;
+			g1 = godd ? go1 : ge1;
+			if constexpr (LL16)
+			{
+				g0 = __half2float(__float2half_rn(g0));
+				g1 = __half2float(__float2half_rn(g1));
+			}
+		}
+	};
+	auto store_row = [&](int py, float e, float o) __attribute__((always_inline)) {
+		if (lane_out && py < J.R_y)
+			store_plane_pair_row<kStoreU8>(plane, (uint32_t)py * dpitch, px, J.R_x, e, o);
+	};
+	// Level-l row pair n with its LL_l row ll; from n = kApron on it
+	// finishes plane rows.
+	auto step0 = [&](auto st, int n, float ll, const uint32_t (&r)[3]) __attribute__((always_inline)) {
+		const float v0 = ll * kK;
+		const float v1 = stream_value_at<JH>(r[0], sho0) * kK;
+		const float v2 = stream_value_at<JH>(r[1], she0) * kInvK;
+		const float v3 = stream_value_at<JH>(r[2], sho0) * kInvK;
+		float e0, e1, o0, o1;
+		e0_line.push(v0, v2, e0, e1);
+		o0_line.push(v1, v3, o0, o1);
+		if constexpr (decltype(st)::value)
+		{
+			hlift_inv_row(e0, o0);
+			hlift_inv_row(e1, o1);
+			const int py = oy0 - kApron + 2 * (n - 2);
+			store_row(py, e0, o0);
+			store_row(py + 1, e1, o1);
+		}
+	};
+
+	// Warm-up: level-(l+1) pairs 0 .. 3 (their outputs precede the first
+	// LL_l row the tile needs); the last one also fetches level l's first two
+	// pairs.
+	uint32_t q1[4], qa[3], qb[3];
+	float g0 = 0.0f, g1 = 0.0f;
+	load1(0, q1);
+#pragma unroll
+	for (int m = 0; m < kApron; m++)
+	{
+		const uint32_t c1[4] = {q1[0], q1[1], q1[2], q1[3]};
+		load1(m + 1, q1);
+		if (m == kApron - 1)
+		{
+			load0(0, qa);
+			load0(1, qb);
+		}
+		__builtin_amdgcn_sched_barrier(0);
+		step1(std::false_type(), c1, g0, g1);
+	}
+	// Main loop: iteration t runs level-(l+1) pair t + 4, which finishes LL_l
+	// rows oy0 / 2 - 2 + 2t, +1, then level-l pairs 2t, 2t + 1, which take
+	// them.
+	const int bh0 = J.bh;
+	float p0 = 0.0f, p1 = 0.0f; // the previous LL_l row pair (bottom mirror)
+	auto iteration = [&](auto st, int t) __attribute__((always_inline)) {
+		const uint32_t c1[4] = {q1[0], q1[1], q1[2], q1[3]};
+		const uint32_t ca[3] = {qa[0], qa[1], qa[2]}, cb[3] = {qb[0], qb[1], qb[2]};
+		if (t + 1 < kPairs0 / 2)
+		{
+			load1(t + kApron + 1, q1);
+			load0(2 * t + 2, qa);
+			load0(2 * t + 3, qb);
+		}
+		// The scheduler would otherwise sink the loads towards their use.
+		__builtin_amdgcn_sched_barrier(0);
+		step1(std::true_type(), c1, g0, g1);
+		if ((oy0 >> 1) - 2 + 2 * t == bh0)
+		{
+			// LL_l rows bh, bh + 1 mirror to bh - 1, bh - 2.
+			g0 = p1;
+			g1 = p0;
+		}
+		p0 = g0;
+		p1 = g1;
+		step0(st, 2 * t, g0, ca);
+		step0(st, 2 * t + 1, g1, cb);
+	};
+	static_assert(kApron == 4, "the first two iterations finish no plane row");
+	iteration(std::false_type(), 0);
+	iteration(std::false_type(), 1);
+#pragma unroll 2
+	for (int t = 2; t < kPairs0 / 2; t++)
+		iteration(std::true_type(), t);
+}
+
+__global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_fused(const FusedJob *__restrict__ jobs,
+                                                                         const uint16_t *__restrict__ tile_job,
+                                                                         const FrameState *__restrict__ frames,
+                                                                         int tiles, int nxcd)
+{
+	const int lane = threadIdx.x & 63;
+	// Wave-uniform (readfirstlane tells the compiler so).
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	                                                    kStreamWaves +
+	                                                (int)(threadIdx.x >> 6));
+	if (tile >= tiles)
+		return; // whole waves: no wave-level or workgroup barrier follows
+	const FusedJob &FJ = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
+	const DwtJob J = FJ.a, K = FJ.b;
+	const int local = tile - FJ.tile_base;
+	const int ox0 = (local % FJ.tiles_x) * kInvFusedTileW;
+	const int oy0 = (local / FJ.tiles_x) * kInvFusedTileH;
+	// The storage-kind combinations that occur (level l + 1's LL and high
+	// bands, whether LL_l rounds to fp16, level l's high bands): FP32
+	// throughout; FP16 (fp16 high bands of levels 0-1, FP32 LL) fusing 1+0 or
+	// 2+1 (4:2:0 chroma: level 2 all FP32); FP16_LL (all four bands of levels
+	// 0-1 fp16) fusing 1+0 or 2+1.
+	if (J.band_kind == kStoreF32)
+		inverse_fused_tile<kStoreF32, kStoreF32, false, kStoreF32>(J, K, frames, ox0, oy0, lane);
+	else if (J.ll_kind == kStoreF16)
+	{
+		if (K.ll_kind == kStoreF16)
+			inverse_fused_tile<kStoreF16, kStoreF16, true, kStoreF16>(J, K, frames, ox0, oy0, lane);
+		else
+			inverse_fused_tile<kStoreF32, kStoreF32, true, kStoreF16>(J, K, frames, ox0, oy0, lane);
+	}
+	else if (K.band_kind == kStoreF16)
+		inverse_fused_tile<kStoreF32, kStoreF16, false, kStoreF16>(J, K, frames, ox0, oy0, lane);
+	else
+		inverse_fused_tile<kStoreF32, kStoreF32, false, kStoreF16>(J, K, frames, ox0, oy0, lane);
+}
+
 // --- packed FP16 arithmetic, streamed (shape 5) ------------------------------
 //
 // The reference's precision 0 (shaders/dwt_common.h:29-34: FLOAT =
@@ -2039,6 +2279,14 @@ void launch_dwt_forward_fused(const FusedJob *jobs, const uint16_t *tile_job, in
 {
 	if (tiles > 0)
 		hipLaunchKernelGGL(k_dwt_forward_fused, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
+		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+}
+
+void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
+                              hipStream_t stream)
+{
+	if (tiles > 0)
+		hipLaunchKernelGGL(k_dwt_inverse_fused, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
 		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
 }
 

@@ -505,7 +505,7 @@ bool Slot::decode_is_ready(bool allow_partial) const
 
 static bool fusable(const Geometry &G, int c)
 {
-	// The fused launch reads 8-bit planes only.
+	// The fused launches read and write 8-bit planes only.
 	return G.plane_w(c) >= 13 && G.plane_h(c) >= 13 && G.sample_bytes == 1;
 }
 
@@ -533,7 +533,9 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	for (Slot *S : slots)
 		if ((S->geom.precision == PYROWAVE_HIP_PRECISION_FP16_MATH) != half_math)
 			return false;
-	has_fused = encoder && !half_math;
+	// Both directions fuse a component's first two levels (FP32-arithmetic
+	// frames with 8-bit planes).
+	has_fused = !half_math;
 
 	for (int f = 0; f < n; f++)
 	{
@@ -551,10 +553,12 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 				const int first = G.first_level(c);
 				if (l < first)
 					continue;
-				// The fused launch takes the LL values it computes above and left
-				// of the frame as the mirror images of those inside, which holds
-				// when the plane's extension mirrors the 12 rows and columns its
-				// window reaches past the edge: planes of at least 13 samples.
+				// The fused forward launch takes the LL values it computes above
+				// and left of the frame as the mirror images of those inside,
+				// which holds when the plane's extension mirrors the 12 rows and
+				// columns its window reaches past the edge: planes of at least 13
+				// samples.  (The fused inverse needs no such bound: its band
+				// dimensions are even and at least 16.)
 				const bool in_fused = has_fused && l <= first + 1 && fusable(G, c);
 				const BandGeom &B0 = G.band[c][l][0];
 				DwtJob J = {};
@@ -642,8 +646,12 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 				FusedJob FJ = {};
 				FJ.a = first_two[c][0];
 				FJ.b = first_two[c][1];
-				FJ.tiles_x = (FJ.b.bw + kFusedTileW1 - 1) / kFusedTileW1;
-				const int tiles_y = (FJ.b.bh + kFusedTileH1 - 1) / kFusedTileH1;
+				// Forward tiles cover level l + 1's bands; inverse tiles the
+				// plane (only the plane is written).
+				FJ.tiles_x = encoder ? (FJ.b.bw + kFusedTileW1 - 1) / kFusedTileW1
+				                     : (FJ.a.R_x + kInvFusedTileW - 1) / kInvFusedTileW;
+				const int tiles_y = encoder ? (FJ.b.bh + kFusedTileH1 - 1) / kFusedTileH1
+				                            : (FJ.a.R_y + kInvFusedTileH - 1) / kInvFusedTileH;
 				FJ.tile_base = int(ftile_map.size());
 				const uint16_t idx = uint16_t(fjobs.size());
 				fjobs.push_back(FJ);
@@ -973,16 +981,29 @@ void run_decode(Plan &P, hipStream_t stream, bool timed)
 
 	const int reps = timed ? D.tune_reps : 0;
 	D.begin_stage(stream, kStageIDWT, n, &ev);
-	for (int l = kLevels - 1; l >= 0; l--)
+	if (P.has_fused && D.forced_variant < 0 && reps == 0 &&
+	    (D.fused >= 2 || (D.fused == 1 && P.fused_tiles >= kInvFusedMinTiles)))
 	{
-		auto launch = [&](int v) {
-			launch_dwt_inverse(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
-		};
-		if (reps > 0 && !P.half_math)
-			tune_level(P, false, l, reps, stream, launch);
-		else
-			launch(level_variant(P, l));
+		// The levels above each component's first two with their tuned
+		// shapes, then those two in one launch.
+		for (int l = kLevels - 1; l >= 0; l--)
+		{
+			const int v = P.variant[l];
+			launch_dwt_inverse(P.d_rest_jobs[l][v], P.d_rest_tile_job[l][v], P.rest_tiles[l][v], frames, v, stream);
+		}
+		launch_dwt_inverse_fused(P.d_fused_jobs, P.d_fused_tile_job, P.fused_tiles, frames, stream);
 	}
+	else
+		for (int l = kLevels - 1; l >= 0; l--)
+		{
+			auto launch = [&](int v) {
+				launch_dwt_inverse(P.d_dwt_jobs[l][v], P.d_tile_job[l][v], P.tiles[l][v], frames, v, stream);
+			};
+			if (reps > 0 && !P.half_math)
+				tune_level(P, false, l, reps, stream, launch);
+			else
+				launch(level_variant(P, l));
+		}
 	D.end_stage(stream, kStageIDWT, n, ev);
 	D.timing = saved_timing;
 }

@@ -143,7 +143,10 @@ struct Device
 	std::map<uint64_t, int> tuned_variant;
 	int forced_variant = -1; // >= 0: every transform level uses this variant
 	bool fused_resolve = true; // rate control of a frame in one launch (k_rc_resolve) where it fits
-	int fused = 1;           // forward levels 0+1 in one launch (k_dwt_forward_fused): 0 never, 1 batches of kFusedMinTiles tiles or more, 2 always
+	// A component's first two transform levels in one launch, forward
+	// (k_dwt_forward_fused) and inverse (k_dwt_inverse_fused): 0 never, 1 for
+	// launches of at least kFusedMinTiles / kInvFusedMinTiles tiles, 2 always.
+	int fused = 1;
 	int tune_reps = 0;       // > 0: the next eager encode / decode autotunes
 	// PSNR-HVS-M-H tables per image height (device copies).
 	std::map<int, std::unique_ptr<DeviceBuffer>> psnr_tables;
@@ -247,8 +250,8 @@ struct Plan
 	uint16_t *d_tile_job[kLevels][kDwtVariants] = {};
 	int tiles[kLevels][kDwtVariants] = {};
 	int variant[kLevels] = {};
-	// Encoder plans of FP32-arithmetic frames: the fused first two levels of
-	// every component, and the per-level tables of the levels that remain.
+	// Plans of FP32-arithmetic frames: the fused first two levels of every
+	// component, and the per-level tables of the levels that remain.
 	bool has_fused = false;
 	FusedJob *d_fused_jobs = nullptr;
 	uint16_t *d_fused_tile_job = nullptr;

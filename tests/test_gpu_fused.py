@@ -1,8 +1,9 @@
-"""The fused first two forward levels (k_dwt_forward_fused, the forward path
-of large batches, forced here for every launch with PYROWAVE_FUSED=2) against
-the level-by-level launches (GPU): every stored coefficient and the bitstream
-are identical, on sizes that leave partial tiles at the right and bottom
-edges, 4:2:0 and 4:4:4, in every precision with FP32 arithmetic."""
+"""The fused first two transform levels against the level-by-level launches
+(GPU), forced for every launch with PYROWAVE_FUSED=2: forward
+(k_dwt_forward_fused), every stored coefficient and the bitstream are
+identical; inverse (k_dwt_inverse_fused), every decoded plane sample is.  On
+sizes that leave partial tiles at the right and bottom edges, 4:2:0 and
+4:4:4, in every precision with FP32 arithmetic."""
 import numpy as np
 import pytest
 
@@ -10,7 +11,7 @@ import pyrowave_amd as pw
 from oracle import oracle as O
 from pyrowave_amd import synth
 
-from helpers import mask_unstored_ll
+from helpers import decode_bytes, mask_unstored_ll
 
 pytestmark = pytest.mark.gpu
 
@@ -49,5 +50,41 @@ def test_fused_forward_equals_level_by_level(w, h, c, precision, monkeypatch):
             mism = np.flatnonzero(fused[1].view(np.uint32) != ref[1].view(np.uint32))
             assert mism.size == 0, (shape, mism.size, mism[:8])
             assert np.array_equal(fused[0], ref[0]), shape
+    finally:
+        dev.close()
+
+
+def _decode(dev, w, h, c, precision, data, packets, force):
+    dev.force_tile_variant(force)
+    dec = pw.Decoder(dev, w, h, c, precision=precision)
+    try:
+        return decode_bytes(dec, data, packets)
+    finally:
+        dec.close()
+
+
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP32, pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
+@pytest.mark.parametrize("w,h,c", SIZES)
+def test_fused_inverse_equals_level_by_level(w, h, c, precision, monkeypatch):
+    planes = synth.frame(w, h, c == pw.CHROMA_420, seed=w * h) if w * h >= 4096 else None
+    if planes is None:
+        from helpers import ramp_planes
+        planes = ramp_planes(w, h, c)
+    budget = max(2000, w * h // 2)
+    monkeypatch.setenv("PYROWAVE_FUSED", "2")
+    dev = pw.Device()
+    try:
+        enc = pw.Encoder(dev, w, h, c, precision=precision)
+        try:
+            assert enc.encode_cpu(planes, budget) == pw.SUCCESS
+            out, pk = enc.packetize(1 << 30, budget + 64)
+        finally:
+            enc.close()
+        fused = _decode(dev, w, h, c, precision, out, pk, -1)
+        for shape in (0, 4):
+            ref = _decode(dev, w, h, c, precision, out, pk, shape)
+            for p in range(3):
+                mism = np.flatnonzero(fused[p] != ref[p])
+                assert mism.size == 0, (shape, p, mism.size, mism[:8])
     finally:
         dev.close()
