@@ -1635,7 +1635,7 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const 
 // --- inverse, levels l + 1 and l fused ---------------------------------------
 //
 // The inverse counterpart of k_dwt_forward_fused: one wave per tile of
-// kInvFusedTileW x kInvFusedTileH samples of a component's plane (level l,
+// kInvFusedTileW x TH samples of a component's plane (level l,
 // the component's first level), with two delay lines.  The level-(l+1) line
 // walks the level-(l+1) bands under the tile's LL_l columns (lanes 0 .. 35;
 // lanes 36 .. 63 repeat lane 35's loads) and finishes two LL_l rows per row
@@ -1661,11 +1661,10 @@ __device__ __forceinline__ int inv_ext_fast(int p, int N)
 	return (unsigned)p < (unsigned)N ? p : inv_ext(p, N);
 }
 
-template <int KL, int KH, bool LL16, int JH>
+template <int TH, int KL, int KH, bool LL16, int JH>
 __device__ __forceinline__ void inverse_fused_tile(const DwtJob &J, const DwtJob &K, const FrameState *frames,
                                                    int ox0, int oy0, int lane)
 {
-	constexpr int TH = kInvFusedTileH;
 	constexpr int kPairs0 = TH / 2 + kApron; // level-l row pairs, as a streamed tile's
 	static_assert(TH % 4 == 0, "LL_l row pairs start on even rows");
 	static_assert(kInvFusedTileW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
@@ -1835,6 +1834,7 @@ __device__ __forceinline__ void inverse_fused_tile(const DwtJob &J, const DwtJob
 		iteration(std::true_type(), t);
 }
 
+template <int TH>
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_fused(const FusedJob *__restrict__ jobs,
                                                                          const uint16_t *__restrict__ tile_job,
                                                                          const FrameState *__restrict__ frames,
@@ -1851,25 +1851,25 @@ __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_fused(const F
 	const DwtJob J = FJ.a, K = FJ.b;
 	const int local = tile - FJ.tile_base;
 	const int ox0 = (local % FJ.tiles_x) * kInvFusedTileW;
-	const int oy0 = (local / FJ.tiles_x) * kInvFusedTileH;
+	const int oy0 = (local / FJ.tiles_x) * TH;
 	// The storage-kind combinations that occur (level l + 1's LL and high
 	// bands, whether LL_l rounds to fp16, level l's high bands): FP32
 	// throughout; FP16 (fp16 high bands of levels 0-1, FP32 LL) fusing 1+0 or
 	// 2+1 (4:2:0 chroma: level 2 all FP32); FP16_LL (all four bands of levels
 	// 0-1 fp16) fusing 1+0 or 2+1.
 	if (J.band_kind == kStoreF32)
-		inverse_fused_tile<kStoreF32, kStoreF32, false, kStoreF32>(J, K, frames, ox0, oy0, lane);
+		inverse_fused_tile<TH, kStoreF32, kStoreF32, false, kStoreF32>(J, K, frames, ox0, oy0, lane);
 	else if (J.ll_kind == kStoreF16)
 	{
 		if (K.ll_kind == kStoreF16)
-			inverse_fused_tile<kStoreF16, kStoreF16, true, kStoreF16>(J, K, frames, ox0, oy0, lane);
+			inverse_fused_tile<TH, kStoreF16, kStoreF16, true, kStoreF16>(J, K, frames, ox0, oy0, lane);
 		else
-			inverse_fused_tile<kStoreF32, kStoreF32, true, kStoreF16>(J, K, frames, ox0, oy0, lane);
+			inverse_fused_tile<TH, kStoreF32, kStoreF32, true, kStoreF16>(J, K, frames, ox0, oy0, lane);
 	}
 	else if (K.band_kind == kStoreF16)
-		inverse_fused_tile<kStoreF32, kStoreF16, false, kStoreF16>(J, K, frames, ox0, oy0, lane);
+		inverse_fused_tile<TH, kStoreF32, kStoreF16, false, kStoreF16>(J, K, frames, ox0, oy0, lane);
 	else
-		inverse_fused_tile<kStoreF32, kStoreF32, false, kStoreF16>(J, K, frames, ox0, oy0, lane);
+		inverse_fused_tile<TH, kStoreF32, kStoreF32, false, kStoreF16>(J, K, frames, ox0, oy0, lane);
 }
 
 // --- packed FP16 arithmetic, streamed (shape 5) ------------------------------
@@ -2282,12 +2282,18 @@ void launch_dwt_forward_fused(const FusedJob *jobs, const uint16_t *tile_job, in
 		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
 }
 
-void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
-                              hipStream_t stream)
+void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, int tile_h,
+                              const FrameState *frames, hipStream_t stream)
 {
-	if (tiles > 0)
-		hipLaunchKernelGGL(k_dwt_inverse_fused, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
-		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+	if (tiles <= 0)
+		return;
+	const dim3 grid((tiles + kStreamWaves - 1) / kStreamWaves), block(64 * kStreamWaves);
+	if (tile_h == kInvFusedTallH)
+		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedTallH>, grid, block, 0, stream, jobs, tile_job, frames,
+		                   tiles, xcd_count());
+	else
+		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedShortH>, grid, block, 0, stream, jobs, tile_job, frames,
+		                   tiles, xcd_count());
 }
 
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,

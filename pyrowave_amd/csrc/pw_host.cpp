@@ -536,6 +536,19 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	// Both directions fuse a component's first two levels (FP32-arithmetic
 	// frames with 8-bit planes).
 	has_fused = !half_math;
+	// Inverse tile height: tall tiles where the launch has waves enough.
+	inv_fused_tile_h = kInvFusedShortH;
+	if (has_fused && !encoder)
+	{
+		long tall = 0;
+		for (Slot *S : slots)
+			for (int c = 0; c < kComponents; c++)
+				if (fusable(S->geom, c))
+					tall += long((S->geom.plane_w(c) + kInvFusedTileW - 1) / kInvFusedTileW) *
+					        ((S->geom.plane_h(c) + kInvFusedTallH - 1) / kInvFusedTallH);
+		if (tall >= kInvFusedTallTiles)
+			inv_fused_tile_h = kInvFusedTallH;
+	}
 
 	for (int f = 0; f < n; f++)
 	{
@@ -651,7 +664,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 				FJ.tiles_x = encoder ? (FJ.b.bw + kFusedTileW1 - 1) / kFusedTileW1
 				                     : (FJ.a.R_x + kInvFusedTileW - 1) / kInvFusedTileW;
 				const int tiles_y = encoder ? (FJ.b.bh + kFusedTileH1 - 1) / kFusedTileH1
-				                            : (FJ.a.R_y + kInvFusedTileH - 1) / kInvFusedTileH;
+				                            : (FJ.a.R_y + inv_fused_tile_h - 1) / inv_fused_tile_h;
 				FJ.tile_base = int(ftile_map.size());
 				const uint16_t idx = uint16_t(fjobs.size());
 				fjobs.push_back(FJ);
@@ -991,7 +1004,8 @@ void run_decode(Plan &P, hipStream_t stream, bool timed)
 			const int v = P.variant[l];
 			launch_dwt_inverse(P.d_rest_jobs[l][v], P.d_rest_tile_job[l][v], P.rest_tiles[l][v], frames, v, stream);
 		}
-		launch_dwt_inverse_fused(P.d_fused_jobs, P.d_fused_tile_job, P.fused_tiles, frames, stream);
+		launch_dwt_inverse_fused(P.d_fused_jobs, P.d_fused_tile_job, P.fused_tiles, P.inv_fused_tile_h, frames,
+		                         stream);
 	}
 	else
 		for (int l = kLevels - 1; l >= 0; l--)

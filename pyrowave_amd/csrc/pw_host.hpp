@@ -256,6 +256,7 @@ struct Plan
 	FusedJob *d_fused_jobs = nullptr;
 	uint16_t *d_fused_tile_job = nullptr;
 	int fused_tiles = 0;
+	int inv_fused_tile_h = kInvFusedShortH; // decoder plans: kInvFusedTallH or kInvFusedShortH
 	DwtJob *d_rest_jobs[kLevels][kDwtVariants] = {};
 	uint16_t *d_rest_tile_job[kLevels][kDwtVariants] = {};
 	int rest_tiles[kLevels][kDwtVariants] = {};

@@ -88,3 +88,44 @@ def test_fused_inverse_equals_level_by_level(w, h, c, precision, monkeypatch):
                 assert mism.size == 0, (shape, p, mism.size, mism[:8])
     finally:
         dev.close()
+
+
+@pytest.mark.parametrize("precision", [pw.PRECISION_FP16, pw.PRECISION_FP16_LL])
+def test_fused_inverse_tall_tiles_batch(device, precision):
+    """A batch decode large enough for the tall fused inverse tiles (nine
+    2990x1690 4:4:4 frames: 3 x 25 x 27 = 2025 tiles of 120x64 each, 18225
+    in all; partial tiles at the right and bottom) equals each frame decoded
+    on its own level by level."""
+    import torch
+    w, h, c, n = 2990, 1690, pw.CHROMA_444, 9
+    budget = 600000
+    streams = []
+    for i in range(n):
+        enc = pw.Encoder(device, w, h, c, precision=precision)
+        try:
+            assert enc.encode_cpu(synth.frame(w, h, False, seed=100 + i), budget) == pw.SUCCESS
+            out, pk = enc.packetize(1 << 30, budget + 64)
+            streams.append((out, pk))
+        finally:
+            enc.close()
+    decs = [pw.Decoder(device, w, h, c, precision=precision) for _ in range(n)]
+    try:
+        outs = [[torch.zeros((h, w), dtype=torch.uint8, device="cuda") for _ in range(3)] for _ in range(n)]
+        tgs = [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in oo]) for oo in outs]
+        for d, (data, pk) in zip(decs, streams):
+            for o, z in pk:
+                assert d.push_packet(data[o:o + z]) == pw.SUCCESS
+        pw.decode_batch(decs, tgs)
+        torch.cuda.synchronize()
+        got = [[o.cpu().numpy() for o in oo] for oo in outs]
+    finally:
+        for d in decs:
+            d.close()
+    device.force_tile_variant(4)
+    try:
+        for i, (data, pk) in enumerate(streams):
+            ref = _decode(device, w, h, c, precision, data, pk, 4)
+            for p in range(3):
+                assert np.array_equal(got[i][p], ref[p]), (i, p)
+    finally:
+        device.force_tile_variant(-1)
