@@ -84,11 +84,12 @@ def measured_traffic(direction, shapes, batch, fused):
     """HBM bytes per launch of one transform direction ("forward" or
     "inverse") with the given tile shape per level, at this batch size: the
     mean over the stage's launches of the committed rocprofv3 PMC figures
-    (lines `traffic_level <direction> <level> <shape> <bytes>`; with the
-    fused forward, `traffic_fused forward <bytes>` for the launch of levels
-    0+1 and `traffic_fused_level forward 2 <shape> <bytes>` for the level-2
-    launch, which then holds luma alone), or None when the profile of that
-    batch size, or a launch's figure, is missing."""
+    (lines `traffic_level <direction> <level> <shape> <bytes>`; with a
+    fused schedule, `traffic_fused <direction> <bytes>` for the launch of
+    each component's first two levels and `traffic_fused_level <direction>
+    2 <shape> <bytes>` for the level-2 launch, which then holds luma alone),
+    or None when the profile of that batch size, or a launch's figure, is
+    missing."""
     table = {}
     try:
         with open(traffic_profile(batch)) as f:
@@ -121,6 +122,31 @@ def fused_forward(batch):
     """Whether a batch of 4K 4:2:0 frames takes the fused levels-0+1 launch."""
     mode = int(os.environ.get("PYROWAVE_FUSED", "1"), 0)
     return mode >= 2 or (mode == 1 and FUSED_TILES_4K420 * batch >= FUSED_MIN_TILES)
+
+
+# Fused inverse tiles of one 3840x2160 4:2:0 frame: 120 plane samples wide,
+# 64 rows high in launches of at least 16384 such tiles, else 32 rows; the
+# library runs launches of fewer than 512 (32-row) tiles level by level.
+INV_FUSED_TILE_W = 120
+INV_FUSED_TALL_H, INV_FUSED_SHORT_H = 64, 32
+INV_FUSED_TALL_TILES, INV_FUSED_MIN_TILES = 16384, 512
+
+
+def inv_fused_tiles(batch, tile_h):
+    per_frame = sum(-(-pw_ // INV_FUSED_TILE_W) * -(-ph // tile_h) for pw_, ph in ((W, H), (W // 2, H // 2),
+                                                                                  (W // 2, H // 2)))
+    return per_frame * batch
+
+
+def inv_fused_tile_h(batch):
+    return INV_FUSED_TALL_H if inv_fused_tiles(batch, INV_FUSED_TALL_H) >= INV_FUSED_TALL_TILES else INV_FUSED_SHORT_H
+
+
+def fused_inverse(batch):
+    """Whether a batch of 4K 4:2:0 frames takes the fused inverse launch of
+    each component's first two levels."""
+    mode = int(os.environ.get("PYROWAVE_FUSED", "1"), 0)
+    return mode >= 2 or (mode == 1 and inv_fused_tiles(batch, INV_FUSED_SHORT_H) >= INV_FUSED_MIN_TILES)
 
 
 def dwt_algorithmic_bytes(fp16):
@@ -266,17 +292,20 @@ def pmc_summary(fetch_csv, write_csv):
         wv = sum(wagg.get(key, [0])) / max(len(wagg.get(key, [0])), 1) * wf * 1024
         return int(round(fv + wv, -3))
 
-    # The fused forward schedule of the headline batch: levels 0+1 in one
-    # launch, then luma level 2 alone, then levels 3 and 4 as above.
-    key = ("k_dwt_forward_fused", -(-FUSED_TILES_4K420 * 32 // 4))
-    if key in fagg:
-        print(f"traffic_fused forward {row(key)}")
-    else:
-        print(f"# missing fused forward ({key[0]}, {key[1]} workgroups)")
-    for shape, (kernel, tw, th, per_wg) in enumerate(TRANSFORM_SHAPES["forward"]):
-        key = (kernel, -(-transform_tiles(2, tw, th, 32, planes=1) // per_wg))
+    # The fused schedules of the headline batch: each component's first two
+    # levels in one launch, luma level 2 alone, then levels 3 and 4 as above.
+    th = inv_fused_tile_h(32)
+    fused_keys = {"forward": ("k_dwt_forward_fused", -(-FUSED_TILES_4K420 * 32 // 4)),
+                  "inverse": (f"k_dwt_inverse_fused<{th}>", -(-inv_fused_tiles(32, th) // 4))}
+    for direction, key in fused_keys.items():
         if key in fagg:
-            print(f"traffic_fused_level forward 2 {shape} {row(key)}")
+            print(f"traffic_fused {direction} {row(key)}")
+        else:
+            print(f"# missing fused {direction} ({key[0]}, {key[1]} workgroups)")
+        for shape, (kernel, tw, th_, per_wg) in enumerate(TRANSFORM_SHAPES[direction]):
+            key = (kernel, -(-transform_tiles(2, tw, th_, 32, planes=1) // per_wg))
+            if key in fagg:
+                print(f"traffic_fused_level {direction} 2 {shape} {row(key)}")
 
 
 # Transform tile shapes by index: (kernel, tile width, tile height, tiles per workgroup).
@@ -597,12 +626,13 @@ def main():
         psnr[name] = float(np.mean(vals[:4]))
 
     # Rooflines of the transform kernels from the events of the timed region.
-    # One batch runs one launch per level (the inverse, and the forward
-    # level by level) or, with the fused forward, four: levels 0+1, luma
-    # level 2, levels 3 and 4.  A launch's algorithmic bytes are the batch's
+    # One batch runs one launch per level (level by level) or, with a fused
+    # schedule, four: each component's first two levels, luma level 2,
+    # levels 3 and 4.  A launch's algorithmic bytes are the batch's
     # compulsory bytes / launches, its duration the stage time / launches,
     # so achieved = compulsory bytes / stage time.
     fused = fused_forward(B)
+    fused_inv = fused_inverse(B)
 
     def kernels_of(base, shapes, fused_first=False):
         """The kernels one transform stage launches: shape 4 is the streamed kernel."""
@@ -799,7 +829,8 @@ def main():
                              "p99": round(float(np.percentile(lat, 99)), 1)},
         "stage_us_per_frame_fp16": stage_us,
         "stage_us_per_frame_fp32_single": stage_us32,
-        "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1]), "inverse", fp16["tuned"][1]),
+        "roofline": roofline("iDWT", kernels_of("k_dwt_inverse", fp16["tuned"][1], fused_inv), "inverse",
+                             fp16["tuned"][1], fused_inv),
         "roofline_forward": roofline("DWT", kernels_of("k_dwt_forward", fp16["tuned"][0], fused), "forward",
                                      fp16["tuned"][0], fused),
         "fp16_ll_reference_storage": None if fp16_ll is None else {
