@@ -136,12 +136,16 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
 	                  __builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48));
 }
 
-__device__ __forceinline__ float cluster_max_f(float v)
+// Maximum over an 8-lane cluster of non-negative, non-NaN floats: their bit
+// patterns order as unsigned integers, so the integer maximum (one DPP
+// instruction a step, no float canonicalization) returns the same float.
+__device__ __forceinline__ float cluster_max_nonneg_f(float v)
 {
-	v = fmaxf(v, dpp_f<kDppXor1>(v));
-	v = fmaxf(v, dpp_f<kDppXor2>(v));
-	v = fmaxf(v, dpp_f<kDppHalfMirror>(v));
-	return v;
+	uint32_t b = __float_as_uint(v);
+	b = max(b, (uint32_t)dpp_i<kDppXor1>((int)b));
+	b = max(b, (uint32_t)dpp_i<kDppXor2>((int)b));
+	b = max(b, (uint32_t)dpp_i<kDppHalfMirror>((int)b));
+	return __uint_as_float(b);
 }
 
 __device__ __forceinline__ uint32_t cluster_or(uint32_t v)
@@ -152,17 +156,19 @@ __device__ __forceinline__ uint32_t cluster_or(uint32_t v)
 	return v;
 }
 
-// fp16 rounding (nearest even) of 0 <= x <= 60000, returned as float: the
-// value float(half(x)).  Normal range: round the float's significand to 10
-// bits; subnormal range: round to a multiple of 2^-24.
+// fp16 rounding (nearest even, subnormals kept) of 0 <= x <= 60000, returned
+// as float: the value float(half(x)), two hardware conversions.  The empty asm
+// keeps the backend from merging the conversion with the arithmetic that
+// produced x into one mixed-precision rounding.
 __device__ __forceinline__ float round_f16_nonneg(float x)
 {
-	if (x < 0x1p-14f)
-		return rintf(x * 0x1p24f) * 0x1p-24f;
-	uint32_t b = __float_as_uint(x);
-	b = (b + 0xfffu + ((b >> 13) & 1u)) & ~0x1fffu;
-	return __uint_as_float(b);
+	asm volatile("" : "+v"(x));
+	return __half2float(__float2half_rn(x));
 }
+
+// Two 16-bit lanes of one register: the quantized magnitudes (at most 16383)
+// of two texels, so masks, minima and sums run on both at once.
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 
 // Inclusive scan over the 64 lanes of a wave with DPP (all lanes active):
 // row_shr 1, 2, 4, 8 scan each 16-lane row (lanes without a source read 0),
@@ -248,7 +254,8 @@ __device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, 
 		Q.t[p] = t;
 		ms = fmaxf(ms, fabsf(t));
 	}
-	float M = cluster_max_f(ms);
+	// ms >= 0 and never NaN (t is clamped with fminf / fmaxf).
+	float M = cluster_max_nonneg_f(ms);
 
 	float quant_scale;
 	if (M < 1.0f)
@@ -373,20 +380,20 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 #pragma unroll
 	for (int p = 0; p < 8; p++)
 		a[p] = fabsf(Q.t[p]);
-	// Early / late plane count of the lane's subblock at quant level q.
-	auto late_planes = [&](int q) {
-		int b = Q.b4max >> q;
-		int early = b > 0 ? 1 : 0;
-		const int m = msb - q;
-		if (m >= 3)
-		{
-			const int qp = m - 2;
-			early = qp + 1;
-			b >>= qp;
-		}
-		early += find_msb((uint32_t)b) + 1;
-		return early - 1 > 0 ? early - 1 : 0;
-	};
+	u16x2 npair[4];
+#pragma unroll
+	for (int i = 0; i < 4; i++)
+		npair[i] = u16x2{(unsigned short)Q.n[2 * i], (unsigned short)Q.n[2 * i + 1]};
+	// Late plane count of the lane's subblock at quant level q, max(early - 1,
+	// 0) of reference shaders/wavelet_quant.comp:139-163.  With m = msb - q
+	// >= 3 the early count is (m - 2) quality planes + a sign plane + the
+	// planes of (b4max >> q) >> (m - 2) = b4max >> (msb - 2), which does not
+	// depend on q; otherwise it is 1 + the planes of b4max >> q when that is
+	// nonzero, else 0.  So both cases are a subtraction from a per-lane
+	// constant (checked for every msb <= 13, b4max < 2^(msb+1), q <= msb).
+	const int m4 = find_msb((uint32_t)Q.b4max);
+	const int fq = msb >= 2 ? find_msb((uint32_t)Q.b4max >> (msb - 2)) : 0;
+	auto late_planes = [&](int q) { return q <= msb - 3 ? msb - q + fq - 1 : max(m4 - q + 1, 0); };
 	// Level 0: cost only (its distortion is stored as 0, as the reference does).
 	{
 		int packed = 0;
@@ -414,19 +421,26 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 		{
 			// A magnitude n is significant at level q when n >= 2^q; its
 			// reconstruction (floor(n / 2^q) + 1/2) * 2^q is the integer
-			// (n with the low q bits cleared) | 2^(q-1), exact in float.
-			const uint32_t lim = 1u << q, keep = ~(lim - 1u), half_bit = lim >> 1;
+			// r + 2^(q-1), r = n with the low q bits cleared, exact in float.
+			// r is 0 or at least 2^q, so min(r, 2^(q-1)) is 2^(q-1) exactly
+			// for the significant magnitudes and 0 for the others: r plus it
+			// is the reconstruction (0 when not significant), and its sum
+			// over the eight is 2^(q-1) times their significant count.
+			const unsigned short keep = (unsigned short)(0xffffu << q), half_bit = (unsigned short)(1u << (q - 1));
+			const u16x2 keep2 = {keep, keep}, half2 = {half_bit, half_bit};
+			u16x2 msum = {0, 0};
 			float err[8];
-			int nsig = 0;
 #pragma unroll
-			for (int p = 0; p < 8; p++)
+			for (int i = 0; i < 4; i++)
 			{
-				const uint32_t n = (uint32_t)Q.n[p];
-				const bool nz = n >= lim;
-				nsig += nz ? 1 : 0;
-				const float recon = nz ? (float)((n & keep) | half_bit) : 0.0f;
-				err[p] = a[p] - recon;
+				const u16x2 r = npair[i] & keep2;
+				const u16x2 m = __builtin_elementwise_min(r, half2);
+				const u16x2 recon = r + m;
+				msum += m;
+				err[2 * i] = a[2 * i] - (float)recon.x;
+				err[2 * i + 1] = a[2 * i + 1] - (float)recon.y;
 			}
+			const int nsig = (int)(((uint32_t)msum.x + (uint32_t)msum.y) >> (q - 1));
 			se_lane = ((dot4_sq(err[0], err[1], err[2], err[3]) + dot4_sq(err[4], err[5], err[6], err[7])) *
 			           J.rdo_scale) * Q.iq2;
 			packed = late_planes(q) | (nsig << 16);
