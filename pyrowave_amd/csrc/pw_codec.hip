@@ -217,7 +217,9 @@ __device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, 
 	const int y = by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
 	// Band storage is padded to whole 32x32 blocks (pitch and rows multiples
 	// of 32, 256-byte aligned), so both rows of the 4x2 subblock load as one
-	// aligned vector each; samples outside the band read as 0.
+	// aligned vector each.  The padding is zero: encoder bands are cleared
+	// when the slot is made and every forward transform stores only inside
+	// its band (x < bw, y < bh), so samples outside the band read as 0.
 	float row[2][4];
 	const long i0 = (long)y * J.pitch + x;
 	if (J.kind == kStoreF16)
@@ -245,8 +247,7 @@ __device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, 
 #pragma unroll
 	for (int p = 0; p < 8; p++)
 	{
-		int px = x + (p >> 1), py = y + (p & 1);
-		float v = (px < J.bw && py < J.bh) ? row[p & 1][p >> 1] : 0.0f;
+		const float v = row[p & 1][p >> 1];
 		// Magnitudes beyond 16383.75 would overflow the 14 planes a block can
 		// describe; they are clamped (reachable only far outside the nominal
 		// coefficient range).
@@ -1006,34 +1007,26 @@ __global__ __launch_bounds__(1024) void k_rc_resolve(FrameState *__restrict__ fr
 			truncate(ci, rw0[ci], rw1[ci], rbk[ci]);
 }
 
-// reference shaders/block_packing.comp:107-142.
+// reference shaders/block_packing.comp:107-142: the block's 8x8 code word
+// truncated by `quant` levels.  The q bits absorb the first levels; each
+// level beyond them lowers every 4x2 subblock's 2-bit plane code c to
+// max(c - 1, 0), which the reference does by shifting planes lo / hi / both
+// (c >= 1, c >= 2, c == 3) down; written here without its loop: one level
+// left gives codes (hi & ~lo) | both << 1, two give both, three or more 0
+// (checked against the reference loop on 3.2 million code words).
 __device__ __forceinline__ uint32_t quantize_code_word(uint32_t cw, int quant)
 {
-	if (quant != 0 && cw != 0)
-	{
-		int q_bits = (int)((cw >> 16) & 0xf);
-		int sub = q_bits < quant ? q_bits : quant;
-		q_bits -= sub;
-		quant -= sub;
-		if (quant != 0)
-		{
-			quant = quant < 3 ? quant : 3;
-			uint32_t plane0 = cw & 0x5555u;
-			uint32_t plane1 = (cw & 0xaaaau) >> 1;
-			uint32_t plane2 = plane0 & plane1;
-			do
-			{
-				plane0 = plane1;
-				plane1 = plane2;
-				plane2 = 0;
-				quant--;
-			} while (quant != 0);
-			plane0 &= ~plane1;
-			cw = (cw & ~0xffffu) | ((plane0 | (plane1 << 1)) & 0xffffu);
-		}
-		cw = (cw & ~(0xfu << 16)) | ((uint32_t)q_bits << 16);
-	}
-	return cw;
+	if (cw == 0)
+		return 0;
+	const int q_bits = (int)((cw >> 16) & 0xf);
+	const int sub = q_bits < quant ? q_bits : quant;
+	const int rest = quant - sub;
+	const uint32_t lo = cw & 0x5555u, hi = (cw >> 1) & 0x5555u, both = lo & hi;
+	uint32_t planes = cw & 0xffffu;
+	planes = rest == 1 ? (hi & ~lo) | (both << 1) : planes;
+	planes = rest == 2 ? both : planes;
+	planes = rest >= 3 ? 0u : planes;
+	return (cw & ~0xfffffu) | ((uint32_t)(q_bits - sub) << 16) | planes;
 }
 
 __device__ __forceinline__ void lds_store_byte(uint32_t *buf, uint32_t byte_pos, uint32_t value)
@@ -1046,10 +1039,8 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
                                                         FrameState *__restrict__ frames,
                                                         int nxcd)
 {
-	__shared__ uint32_t s_buf[kMaxBlockWords];
-	__shared__ int s_tmp[2];
-	__shared__ uint32_t s_ballot;
-	__shared__ uint32_t s_bits;
+	__shared__ __align__(16) uint32_t s_buf[kMaxBlockWords];
+	__shared__ uint32_t s_wave[2][3]; // per wave: scan partial, coded 8x8 mask, coded bits
 	__shared__ uint32_t s_group_base;
 
 	const BlockCoord C = locate_block(bands, blk_band, nxcd);
@@ -1074,13 +1065,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 		return;
 	}
 
-	for (int i = tid; i < kMaxBlockWords; i += kBlockThreads)
-		s_buf[i] = 0;
-	if (tid == 0)
-	{
-		s_ballot = 0;
-		s_bits = 0;
-	}
+	// One 16-byte store per lane clears the block buffer.
+	static_assert(kMaxBlockWords == 4 * kBlockThreads, "one uint4 of the block buffer per lane");
+	reinterpret_cast<uint4 *>(s_buf)[tid] = make_uint4(0u, 0u, 0u, 0u);
 	if (tid < 64)
 	{
 		// Start of this block's 256-block group: sum of the earlier groups.
@@ -1111,40 +1098,60 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	// is significant (its sign is sent) when it has a bit at or above
 	// in_planes - out_planes; the plane bytes themselves are extracted once
 	// their LDS position is known, after the scan below.
+	u16x2 npair[4];
 	uint32_t pair[4];
 #pragma unroll
 	for (int i = 0; i < 4; i++)
-		pair[i] = (uint32_t)Q.n[i] | ((uint32_t)Q.n[i + 4] << 16);
+	{
+		npair[i] = u16x2{(unsigned short)Q.n[i], (unsigned short)Q.n[i + 4]};
+		pair[i] = __builtin_bit_cast(uint32_t, npair[i]);
+	}
+	// Significance of both halves at once: min(n >> drop, 1) is 0 or 1 in
+	// each 16-bit lane; bit i of the low half is magnitude i, of the high
+	// half magnitude i + 4.
 	uint32_t sig = 0;
 	if (out_planes)
 	{
-		const uint32_t drop = in_planes - out_planes;
+		const unsigned short drop = (unsigned short)(in_planes - out_planes);
+		const u16x2 drop2 = {drop, drop}, one2 = {1, 1};
+		uint32_t t = 0;
 #pragma unroll
-		for (int p = 0; p < 8; p++)
-			sig |= ((uint32_t)Q.n[p] >> drop) != 0u ? 1u << p : 0u;
+		for (int i = 0; i < 4; i++)
+			t |= __builtin_bit_cast(uint32_t, __builtin_elementwise_min(npair[i] >> drop2, one2)) << i;
+		sig = (t & 0xfu) | ((t >> 12) & 0xf0u);
 	}
 	const int nsig = __popc(sig);
 	const int cost8 = cluster_sum_i(8 * (int)out_planes + nsig);
-	__syncthreads();
 
-	if (s == 0)
+	// Each wave holds eight 8x8 blocks (lane 8k is block k's first lane, and
+	// every lane of a block has the same code word and cost): its coded-block
+	// mask from a ballot, its coded bits (cost + 24 per coded 8x8) as a wave
+	// sum, and its part of one scan of (planes << 16 | significant count)
+	// (both sums stay below 2^16: 128 lanes x 14 planes, 1024 coefficients)
+	// go to LDS together, so one barrier publishes all three.
+	const int lane = tid & 63, wave = tid >> 6;
+	const uint64_t act = __ballot(s == 0 && active);
+	uint32_t act8 = 0;
+#pragma unroll
+	for (int k = 0; k < 8; k++)
+		act8 |= (uint32_t)((act >> (8 * k)) & 1u) << k;
+	const uint32_t wbits = wave_sum_u32(s == 0 && cost8 ? (uint32_t)cost8 + 24u : 0u);
+	const int scan_v = ((int)out_planes << 16) | nsig;
+	const int scan_incl = wave_incl_scan(scan_v);
+	if (lane == 63)
 	{
-		if (active)
-			atomicOr(&s_ballot, 1u << k8);
-		if (cost8)
-			atomicAdd(&s_bits, (uint32_t)cost8 + 24u);
+		s_wave[wave][0] = (uint32_t)scan_incl;
+		s_wave[wave][1] = act8;
+		s_wave[wave][2] = wbits;
 	}
 	__syncthreads();
-	const uint32_t ballot = s_ballot;
+	const uint32_t ballot = s_wave[0][1] | (s_wave[1][1] << 8);
 	const uint32_t N = __popc(ballot);
-	uint32_t wcount = (s_bits + 31) / 32;
+	uint32_t wcount = (s_wave[0][2] + s_wave[1][2] + 31) / 32;
 	if (wcount)
 		wcount += 2;
-
-	// One scan of (planes << 16 | significant count): both sums stay below
-	// 2^16 (128 lanes x 14 planes, 1024 coefficients).
-	int packed_total;
-	const int packed_excl = block_excl_scan128(((int)out_planes << 16) | nsig, s_tmp, packed_total);
+	const int packed_excl = (wave ? (int)s_wave[0][0] : 0) + scan_incl - scan_v;
+	const int packed_total = (int)(s_wave[0][0] + s_wave[1][0]);
 	const int plane_excl = packed_excl >> 16, sig_excl = packed_excl & 0xffff;
 	const int planes_total = packed_total >> 16;
 
