@@ -324,10 +324,10 @@ struct BlockCoord
 
 // blk_band holds two words per block of the launch: the band job index and
 // the block position bx | by << 16 within its band.
-__device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const uint32_t *blk_band, int nxcd)
+__device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const uint32_t *blk_band, XcdMap xm)
 {
 	BlockCoord C;
-	const int g = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
+	const int g = xcd_swizzle((int)blockIdx.x, xm);
 	const uint2 d = reinterpret_cast<const uint2 *>(blk_band)[g];
 	C.J = bands[__builtin_amdgcn_readfirstlane(d.x)];
 	const uint32_t pos = __builtin_amdgcn_readfirstlane(d.y);
@@ -340,7 +340,7 @@ __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const u
 __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
                                                                  const uint32_t *__restrict__ blk_band,
                                                                  const FrameState *__restrict__ frames,
-                                                                 int nxcd)
+                                                                 XcdMap xm)
 {
 	__shared__ int s_cost[16][16];
 	__shared__ float s_dist[16][16];
@@ -350,7 +350,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 	__shared__ float s_D[16];
 	__shared__ uint32_t s_v[16];
 
-	const BlockCoord C = locate_block(bands, blk_band, nxcd);
+	const BlockCoord C = locate_block(bands, blk_band, xm);
 	const BandJob &J = C.J;
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
 	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
@@ -1037,13 +1037,13 @@ __device__ __forceinline__ void lds_store_byte(uint32_t *buf, uint32_t byte_pos,
 __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restrict__ bands,
                                                         const uint32_t *__restrict__ blk_band,
                                                         FrameState *__restrict__ frames,
-                                                        int nxcd)
+                                                        XcdMap xm)
 {
 	__shared__ __align__(16) uint32_t s_buf[kMaxBlockWords];
 	__shared__ uint32_t s_wave[2][3]; // per wave: scan partial, coded 8x8 mask, coded bits
 	__shared__ uint32_t s_group_base;
 
-	const BlockCoord C = locate_block(bands, blk_band, nxcd);
+	const BlockCoord C = locate_block(bands, blk_band, xm);
 	const BandJob &J = C.J;
 	FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
@@ -1223,25 +1223,6 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	}
 }
 
-// Byte `pos` of a coded block: from the LDS copy of its first `nbytes`
-// bytes, else (only a malformed block reads past its own length) from the
-// payload itself, and zero past the payload's end.
-struct BlockBytes
-{
-	const uint32_t *lds;
-	const uint32_t *global;
-	uint32_t nbytes, avail;
-
-	__device__ __forceinline__ uint32_t operator()(uint32_t pos) const
-	{
-		if (pos < nbytes)
-			return (lds[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
-		if (pos < avail)
-			return (global[pos >> 2] >> (8 * (pos & 3))) & 0xffu;
-		return 0;
-	}
-};
-
 // reference shaders/wavelet_dequant.comp:100-160 and 220-391.  The coded
 // block (at most kMaxBlockWords words for any block an encoder can produce) is
 // first copied to LDS with coalesced loads and parsed from there.  A malformed
@@ -1250,7 +1231,7 @@ struct BlockBytes
 __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
                                                            const uint32_t *__restrict__ blk_band,
                                                            const FrameState *__restrict__ frames,
-                                                           int nxcd)
+                                                           XcdMap xm)
 {
 	__shared__ uint32_t s_pay[kMaxBlockWords];
 	__shared__ uint32_t s_poff[16];
@@ -1259,7 +1240,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	__shared__ uint32_t s_sign_base;
 	__shared__ int s_tmp[2];
 
-	const BlockCoord C = locate_block(bands, blk_band, nxcd);
+	const BlockCoord C = locate_block(bands, blk_band, xm);
 	const BandJob &J = C.J;
 	const FrameState &F = frames[J.frame];
 	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
@@ -1304,7 +1285,19 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		for (uint32_t i = tid + kBlockThreads; i < nwords; i += kBlockThreads)
 			s_pay[i] = payload[i];
 		__syncthreads();
-		const BlockBytes byte_at{s_pay, payload, nwords * 4, (payload_words - off) * 4};
+		// Byte `pos` of the coded block.  A well-formed block reads only its
+		// own nwords words, all in LDS; past them (only a malformed block
+		// does) come the payload's following bytes, and zero past its end.
+		// The two sources are separate loads, never one load through a
+		// pointer chosen per lane (which would be a generic FLAT access).
+		const uint32_t nbytes = nwords * 4, avail = (payload_words - off) * 4;
+		auto byte_at = [&](uint32_t pos) -> uint32_t {
+			const uint32_t w = s_pay[min(pos >> 2, (uint32_t)kMaxBlockWords - 1u)];
+			uint32_t b = (w >> (8 * (pos & 3))) & 0xffu;
+			if (pos >= nbytes)
+				b = pos < avail ? (payload[pos >> 2] >> (8 * (pos & 3))) & 0xffu : 0u;
+			return b;
+		};
 		const uint32_t ballot = w0 & 0xffffu;
 		const uint32_t q_code = byte_at(4);
 		const uint32_t N = __popc(ballot);
@@ -1479,7 +1472,7 @@ void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nb
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_quant_analyze, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   xcd_count());
+		                   xcd_map(dim3(nblocks)));
 }
 
 
@@ -1511,7 +1504,7 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   xcd_count());
+		                   xcd_map(dim3(nblocks)));
 }
 
 void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
@@ -1519,6 +1512,6 @@ void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks,
 {
 	if (nblocks > 0)
 		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
-		                   xcd_count());
+		                   xcd_map(dim3(nblocks)));
 }
 } // namespace pw

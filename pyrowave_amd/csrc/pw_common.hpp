@@ -238,21 +238,58 @@ struct BandJob
 };
 
 // Workgroups are dispatched round-robin over the XCDs (workgroup i runs on
-// XCD i % nxcd), and each XCD has its own L2.  This maps the dispatch order
-// to a contiguous range of work items per XCD, so that neighbouring tiles and
-// blocks, which share cache lines and filter aprons, meet in one L2.  A
-// bijection on [0, n) for any n.
-__host__ __device__ inline int xcd_swizzle(int b, int n, int nxcd)
+// XCD i % nxcd), and each XCD has its own L2.  xcd_swizzle maps the dispatch
+// order to a contiguous range of work items per XCD, so that neighbouring
+// tiles and blocks, which share cache lines and filter aprons, meet in one L2:
+// a bijection on [0, n) for any n.  The quotient and remainder of n / nxcd
+// come with the launch (XcdMap, made on the host for the launch's grid), and
+// a power-of-two XCD count divides by shifting, so a workgroup spends a few
+// scalar instructions on it instead of two integer divisions.
+struct XcdMap
 {
-	if (nxcd <= 1 || n < nxcd)
+	int nxcd;  // 1: identity (one XCD, or fewer workgroups than XCDs)
+	int shift; // log2(nxcd) when nxcd is a power of two, else -1
+	int q, r;  // n / nxcd, n % nxcd
+};
+
+__host__ __device__ inline int xcd_swizzle(int b, const XcdMap &m)
+{
+	if (m.nxcd <= 1)
 		return b;
-	const int q = n / nxcd, r = n % nxcd;
-	const int x = b % nxcd, i = b / nxcd;
-	return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+	int x, i;
+	if (m.shift >= 0)
+	{
+		x = b & (m.nxcd - 1);
+		i = b >> m.shift;
+	}
+	else
+	{
+		x = b % m.nxcd;
+		i = b / m.nxcd;
+	}
+	return x < m.r ? x * (m.q + 1) + i : m.r * (m.q + 1) + (x - m.r) * m.q + i;
+}
+
+__host__ __device__ inline XcdMap make_xcd_map(int n, int nxcd)
+{
+	XcdMap m = {1, 0, n, 0};
+	if (nxcd <= 1 || n < nxcd)
+		return m;
+	m.nxcd = nxcd;
+	m.shift = -1;
+	for (int k = 0; k < 31; k++)
+		if ((1 << k) == nxcd)
+			m.shift = k;
+	m.q = n / nxcd;
+	m.r = n % nxcd;
+	return m;
 }
 
 // Number of XCDs of the current device (1 where the runtime cannot tell).
 int xcd_count();
+
+// The XcdMap of a launch of `grid` workgroups on the current device.
+XcdMap xcd_map(const dim3 &grid);
 
 // One frame of a device packetize launch (reference
 // pyrowave_encoder.cpp:596-623, 1048-1115): the sequence header goes in the

@@ -364,7 +364,7 @@ __device__ __forceinline__ void fwd_stage(const T *src, uint32_t pitch, uint32_t
 template <int W, int H, bool LUT>
 __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict__ jobs,
                                                        const uint16_t *__restrict__ tile_job,
-                                                       const FrameState *__restrict__ frames, int nxcd)
+                                                       const FrameState *__restrict__ frames, XcdMap xm)
 {
 	using S = FwdTile<W, H, LUT>;
 	__shared__ float sA[S::kInH * S::kPitch];
@@ -374,7 +374,7 @@ __global__ __launch_bounds__(2 * W) void k_dwt_forward(const DwtJob *__restrict_
 	float *s_lut = LUT ? s_lut_mem : nullptr;
 
 	const int tid = threadIdx.x;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
+	const int tile = xcd_swizzle((int)blockIdx.x, xm);
 	// The job index is uniform: readfirstlane keeps J in scalar registers.
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
@@ -684,7 +684,7 @@ __device__ __forceinline__ void store_u8_run(uint8_t *dst, int n_valid, const fl
 template <int TW, int TH, int THREADS, int WIN, bool PRESCALE>
 __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restrict__ jobs,
                                                          const uint16_t *__restrict__ tile_job,
-                                                         const FrameState *__restrict__ frames, int nxcd)
+                                                         const FrameState *__restrict__ frames, XcdMap xm)
 {
 	using S = InvTile<TW, TH, THREADS, WIN, PRESCALE>;
 	constexpr int kOut = S::kOut;
@@ -693,7 +693,7 @@ __global__ __launch_bounds__(THREADS) void k_dwt_inverse(const DwtJob *__restric
 	__shared__ int s_ymap[S::kInH];
 
 	const int tid = threadIdx.x;
-	const int tile = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd);
+	const int tile = xcd_swizzle((int)blockIdx.x, xm);
 	// The job index is uniform: readfirstlane keeps J in scalar registers.
 	const DwtJob J = jobs[__builtin_amdgcn_readfirstlane(tile_job[tile])];
 	const int local = tile - J.tile_base;
@@ -1145,13 +1145,13 @@ __device__ __forceinline__ void forward_stream_tile(const DwtJob &J, const void 
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream(const DwtJob *__restrict__ jobs,
                                                                           const uint16_t *__restrict__ tile_job,
                                                                           const FrameState *__restrict__ frames,
-                                                                          int tiles, int nxcd)
+                                                                          int tiles, XcdMap xm)
 {
 	constexpr int TW = kFwdVariantTileW[kFwdStreamVariant], TH = kFwdVariantTileH[kFwdStreamVariant];
 	const int lane = threadIdx.x & 63;
 	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
 	// everything derived from it stay in scalar registers).
-	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, xm) *
 	                                                    kStreamWaves +
 	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
@@ -1379,12 +1379,12 @@ __device__ __forceinline__ void forward_fused_tile(const DwtJob &J, const DwtJob
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_fused(const FusedJob *__restrict__ jobs,
                                                                          const uint16_t *__restrict__ tile_job,
                                                                          const FrameState *__restrict__ frames,
-                                                                         int tiles, int nxcd)
+                                                                         int tiles, XcdMap xm)
 {
 	const int lane = threadIdx.x & 63;
 	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
 	// everything derived from it stay in scalar registers).
-	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, xm) *
 	                                                    kStreamWaves +
 	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
@@ -1585,14 +1585,14 @@ __device__ __forceinline__ void inverse_stream_tile(const DwtJob &J, const Frame
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream(const DwtJob *__restrict__ jobs,
                                                                           const uint16_t *__restrict__ tile_job,
                                                                           const FrameState *__restrict__ frames,
-                                                                          int tiles, int nxcd)
+                                                                          int tiles, XcdMap xm)
 {
 	constexpr int TW = kInvVariantTileW[kInvStreamVariant], TH = kInvVariantTileH[kInvStreamVariant];
 	static_assert(TW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
 	const int lane = threadIdx.x & 63;
 	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
 	// everything derived from it stay in scalar registers).
-	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, xm) *
 	                                                    kStreamWaves +
 	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
@@ -1838,11 +1838,11 @@ template <int TH>
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_fused(const FusedJob *__restrict__ jobs,
                                                                          const uint16_t *__restrict__ tile_job,
                                                                          const FrameState *__restrict__ frames,
-                                                                         int tiles, int nxcd)
+                                                                         int tiles, XcdMap xm)
 {
 	const int lane = threadIdx.x & 63;
 	// Wave-uniform (readfirstlane tells the compiler so).
-	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, xm) *
 	                                                    kStreamWaves +
 	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
@@ -2073,13 +2073,13 @@ __device__ __forceinline__ void forward_stream_tile_h(const DwtJob &J, const voi
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_forward_stream_h(const DwtJob *__restrict__ jobs,
                                                                             const uint16_t *__restrict__ tile_job,
                                                                             const FrameState *__restrict__ frames,
-                                                                            int tiles, int nxcd)
+                                                                            int tiles, XcdMap xm)
 {
 	constexpr int TW = kFwdVariantTileW[kHalfStreamVariant], TH = kFwdVariantTileH[kHalfStreamVariant];
 	const int lane = threadIdx.x & 63;
 	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
 	// everything derived from it stay in scalar registers).
-	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, xm) *
 	                                                    kStreamWaves +
 	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
@@ -2229,14 +2229,14 @@ __device__ __forceinline__ void inverse_stream_tile_h(const DwtJob &J, const Fra
 __global__ __launch_bounds__(64 * kStreamWaves) void k_dwt_inverse_stream_h(const DwtJob *__restrict__ jobs,
                                                                             const uint16_t *__restrict__ tile_job,
                                                                             const FrameState *__restrict__ frames,
-                                                                            int tiles, int nxcd)
+                                                                            int tiles, XcdMap xm)
 {
 	constexpr int TW = kInvVariantTileW[kHalfStreamVariant], TH = kInvVariantTileH[kHalfStreamVariant];
 	static_assert(TW == 2 * 64 - 2 * kApron, "one interleaved column pair per lane");
 	const int lane = threadIdx.x & 63;
 	// Wave-uniform (readfirstlane tells the compiler so: the tile origin and
 	// everything derived from it stay in scalar registers).
-	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) *
+	const int tile = __builtin_amdgcn_readfirstlane(xcd_swizzle((int)blockIdx.x, xm) *
 	                                                    kStreamWaves +
 	                                                (int)(threadIdx.x >> 6));
 	if (tile >= tiles)
@@ -2254,32 +2254,35 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	if (tiles <= 0)
 		return;
+	// Streamed shapes: kStreamWaves tiles per workgroup; tiled shapes: one.
+	const dim3 sgrid((tiles + kStreamWaves - 1) / kStreamWaves), tgrid(tiles), sblock(64 * kStreamWaves);
 	if (variant == kHalfStreamVariant)
-		hipLaunchKernelGGL(k_dwt_forward_stream_h, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
-		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+		hipLaunchKernelGGL(k_dwt_forward_stream_h, sgrid, sblock, 0, stream, jobs, tile_job, frames, tiles,
+		                   xcd_map(sgrid));
 	else if (variant == kFwdStreamVariant)
-		hipLaunchKernelGGL(k_dwt_forward_stream, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
-		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+		hipLaunchKernelGGL(k_dwt_forward_stream, sgrid, sblock, 0, stream, jobs, tile_job, frames, tiles,
+		                   xcd_map(sgrid));
 	else if (variant == 1)
-		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[1], kFwdVariantTileH[1], true>), dim3(tiles),
-		                   dim3(2 * kFwdVariantTileW[1]), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[1], kFwdVariantTileH[1], true>), tgrid,
+		                   dim3(2 * kFwdVariantTileW[1]), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 2)
-		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[2], kFwdVariantTileH[2], false>), dim3(tiles),
-		                   dim3(2 * kFwdVariantTileW[2]), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[2], kFwdVariantTileH[2], false>), tgrid,
+		                   dim3(2 * kFwdVariantTileW[2]), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 3)
-		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[3], kFwdVariantTileH[3], false>), dim3(tiles),
-		                   dim3(2 * kFwdVariantTileW[3]), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[3], kFwdVariantTileH[3], false>), tgrid,
+		                   dim3(2 * kFwdVariantTileW[3]), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else
-		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[0], kFwdVariantTileH[0], true>), dim3(tiles),
-		                   dim3(2 * kFwdVariantTileW[0]), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[0], kFwdVariantTileH[0], true>), tgrid,
+		                   dim3(2 * kFwdVariantTileW[0]), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 }
 
 void launch_dwt_forward_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
                               hipStream_t stream)
 {
-	if (tiles > 0)
-		hipLaunchKernelGGL(k_dwt_forward_fused, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
-		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+	if (tiles <= 0)
+		return;
+	const dim3 grid((tiles + kStreamWaves - 1) / kStreamWaves), block(64 * kStreamWaves);
+	hipLaunchKernelGGL(k_dwt_forward_fused, grid, block, 0, stream, jobs, tile_job, frames, tiles, xcd_map(grid));
 }
 
 void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, int tiles, int tile_h,
@@ -2290,10 +2293,10 @@ void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, in
 	const dim3 grid((tiles + kStreamWaves - 1) / kStreamWaves), block(64 * kStreamWaves);
 	if (tile_h == kInvFusedTallH)
 		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedTallH>, grid, block, 0, stream, jobs, tile_job, frames,
-		                   tiles, xcd_count());
+		                   tiles, xcd_map(grid));
 	else
 		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedShortH>, grid, block, 0, stream, jobs, tile_job, frames,
-		                   tiles, xcd_count());
+		                   tiles, xcd_map(grid));
 }
 
 void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles, const FrameState *frames,
@@ -2301,24 +2304,25 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 {
 	if (tiles <= 0)
 		return;
+	const dim3 sgrid((tiles + kStreamWaves - 1) / kStreamWaves), tgrid(tiles), sblock(64 * kStreamWaves);
 	if (variant == kHalfStreamVariant)
-		hipLaunchKernelGGL(k_dwt_inverse_stream_h, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
-		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+		hipLaunchKernelGGL(k_dwt_inverse_stream_h, sgrid, sblock, 0, stream, jobs, tile_job, frames, tiles,
+		                   xcd_map(sgrid));
 	else if (variant == kInvStreamVariant)
-		hipLaunchKernelGGL(k_dwt_inverse_stream, dim3((tiles + kStreamWaves - 1) / kStreamWaves),
-		                   dim3(64 * kStreamWaves), 0, stream, jobs, tile_job, frames, tiles, xcd_count());
+		hipLaunchKernelGGL(k_dwt_inverse_stream, sgrid, sblock, 0, stream, jobs, tile_job, frames, tiles,
+		                   xcd_map(sgrid));
 	else if (variant == 1)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128, 16, false>), dim3(tiles),
-		                   dim3(128), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128, 16, false>), tgrid,
+		                   dim3(128), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 2)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 256, 16, false>), dim3(tiles),
-		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 256, 16, false>), tgrid,
+		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 3)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[3], kInvVariantTileH[3], 256, 24, true>), dim3(tiles),
-		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[3], kInvVariantTileH[3], 256, 24, true>), tgrid,
+		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[0], kInvVariantTileH[0], 256, 16, false>), dim3(tiles),
-		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_count());
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[0], kInvVariantTileH[0], 256, 16, false>), tgrid,
+		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 }
 
 int xcd_count()
@@ -2335,6 +2339,11 @@ int xcd_count()
 	if (dev < 64)
 		cache[dev] = n;
 	return n;
+}
+
+XcdMap xcd_map(const dim3 &grid)
+{
+	return make_xcd_map(int(grid.x), xcd_count());
 }
 
 void upload_constants()

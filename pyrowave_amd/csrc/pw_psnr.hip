@@ -146,7 +146,7 @@ __global__ __launch_bounds__(kPsnrThreads) void k_psnr_hvs_m(const uint8_t *__re
                                                              const uint8_t *__restrict__ b, uint32_t pitch_b, int w,
                                                              int h, int nbx, int nblocks,
                                                              const float *__restrict__ tables,
-                                                             unsigned long long *__restrict__ totals, int nxcd)
+                                                             unsigned long long *__restrict__ totals, XcdMap xm)
 {
 	// [factor][csf, mask, inv_mask][64]
 	__shared__ float s_tab[kPsnrFactors * 3 * 64];
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(kPsnrThreads) void k_psnr_hvs_m(const uint8_t *__re
 		s_tab[i] = tables[i];
 	__syncthreads();
 
-	const int g = xcd_swizzle((int)blockIdx.x, (int)gridDim.x, nxcd) * kPsnrThreads + tid;
+	const int g = xcd_swizzle((int)blockIdx.x, xm) * kPsnrThreads + tid;
 	float da[64], db[64];
 	float pop_a = 0.0f, pop_b = 0.0f;
 	const bool active = g < nblocks;
@@ -223,6 +223,6 @@ void launch_psnr_hvs_m(const uint8_t *a, uint32_t pitch_a, const uint8_t *b, uin
 	const int nblocks = nbx * nby;
 	const int groups = (nblocks + kPsnrThreads - 1) / kPsnrThreads;
 	hipLaunchKernelGGL(k_psnr_hvs_m, dim3(groups), dim3(kPsnrThreads), 0, stream, a, pitch_a, b, pitch_b, w, h, nbx,
-	                   nblocks, tables, reinterpret_cast<unsigned long long *>(totals), xcd_count());
+	                   nblocks, tables, reinterpret_cast<unsigned long long *>(totals), xcd_map(dim3(groups)));
 }
 } // namespace pw
