@@ -94,6 +94,7 @@ constexpr int kDppXor1 = 0xB1;
 constexpr int kDppXor2 = 0x4E;
 constexpr int kDppHalfMirror = 0x141;
 constexpr int kDppRowMirror = 0x140; // lane i <-> 15 - i within each 16-lane row
+constexpr int kDppRowRor8 = 0x128;   // lane i reads lane (i + 8) mod 16 of its 16-lane row
 
 template <int Ctrl>
 __device__ __forceinline__ int dpp_i(int v)
@@ -371,11 +372,11 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 	// Each wave runs the levels of its own eight 8x8 blocks (the analyze below
 	// reads level q of a block only for q <= its msb): the largest msb of the
 	// wave, equal within each 8-lane cluster.
-	int qmax = msb;
-	qmax = max(qmax, __shfl_xor(qmax, 8, 64));
-	qmax = max(qmax, __shfl_xor(qmax, 16, 64));
-	qmax = max(qmax, __shfl_xor(qmax, 32, 64));
-	qmax = __builtin_amdgcn_readfirstlane(qmax);
+	// row_ror 8 pairs the two clusters of each 16-lane row; the four row
+	// maxima are read out as scalars.
+	const int mrow = max(msb, dpp_i<kDppRowRor8>(msb));
+	const int qmax = max(max(__builtin_amdgcn_readlane(mrow, 0), __builtin_amdgcn_readlane(mrow, 16)),
+	                     max(__builtin_amdgcn_readlane(mrow, 32), __builtin_amdgcn_readlane(mrow, 48)));
 
 	float a[8];
 #pragma unroll
