@@ -1,6 +1,6 @@
 // pw_codec.hip -- per-32x32-block kernels of the MI355X PyroWave codec.
 //
-//   k_quant_analyze : quantizer + rate analysis fused; one 128-lane workgroup
+//   k_quant_analyze : quantizer + rate analysis fused; one wave (workgroup)
 //                     per coded 32x32 block (reference shaders/wavelet_quant.comp
 //                     and shaders/analyze_rate_control.comp).  The per-8x8
 //                     statistics stay in LDS; only 48 B per block leave.
@@ -15,10 +15,13 @@
 //                     coalesced stores (shaders/block_packing.comp).
 //   k_dequant       : block parse + reconstruction (shaders/wavelet_dequant.comp).
 //
-// Lane layout inside a 32x32 block (all kernels): lane = 8 * k8 + s, where
-// k8 = 4 * y8 + x8 is the 8x8 block in raster order and s the 4x2 subblock;
-// subblock s covers x = 4*(s>>2) .. +3, y = 2*(s&3) .. +1 and its pixel p sits
-// at (p >> 1, p & 1) (reference bitstream/bitstream.md:350-383).
+// Lane layout inside a 32x32 block (quantizer, packer, dequantizer: one
+// 64-lane wave per block): lane = 4 * k8 + j, where k8 = 4 * y8 + x8 is the
+// 8x8 block in raster order and the lane holds its 4x2 subblocks 2j and
+// 2j + 1 (a 4x4 region); subblock s covers x = 4*(s>>2) .. +3,
+// y = 2*(s&3) .. +1 and its pixel p sits at (p >> 1, p & 1) (reference
+// bitstream/bitstream.md:350-383).  Lane order is the bitstream's subblock
+// order, so every per-subblock stream position is a scan over the lanes.
 #include "pw_common.hpp"
 
 #include <hip/hip_fp16.h>
@@ -29,7 +32,9 @@ namespace pw
 {
 namespace
 {
-constexpr int kBlockThreads = 128;
+// Quantizer and packer: one wave per 32x32 block, lane = 4 * k8 + j holding
+// subblocks 2j and 2j+1 of 8x8 block k8.
+constexpr int kBlockLanes = 64;
 
 // Smallest floats >= 2^0.25 and 2^0.75: floor(60.5 + 2*log2(r)) for
 // r = m * 2^E, m in [1,2), is 60 + 2E + [m >= 2^0.25] + [m >= 2^0.75].
@@ -85,15 +90,15 @@ __device__ __forceinline__ float f16_bits_to_f32(uint32_t h)
 	return __half2float(__ushort_as_half((unsigned short)h));
 }
 
-// Cross-lane moves inside 8-lane clusters with DPP: quad_perm [1,0,3,2] and
-// [2,3,0,1] swap neighbours at distance 1 and 2, row_half_mirror pairs lane i
-// with lane 7-i.  After the two quad steps all four lanes of a quad hold the
-// same value, so the mirror step completes the butterfly
-// ((v0+v1)+(v2+v3)) + ((v4+v5)+(v6+v7)) in every lane.
+// Cross-lane moves with DPP: quad_perm [1,0,3,2] and [2,3,0,1] swap
+// neighbours at distance 1 and 2 (after both, all four lanes of a quad hold
+// the butterfly ((v0+v1)+(v2+v3))), row_half_mirror pairs lane i with lane
+// 7-i of each 8-lane half-row, row_mirror lane i with 15-i of each row.
 constexpr int kDppXor1 = 0xB1;
 constexpr int kDppXor2 = 0x4E;
 constexpr int kDppHalfMirror = 0x141;
 constexpr int kDppRowMirror = 0x140; // lane i <-> 15 - i within each 16-lane row
+constexpr int kDppRowRor4 = 0x124;   // lane i reads lane (i + 4) mod 16 of its 16-lane row
 constexpr int kDppRowRor8 = 0x128;   // lane i reads lane (i + 8) mod 16 of its 16-lane row
 
 template <int Ctrl>
@@ -108,22 +113,6 @@ __device__ __forceinline__ float dpp_f(float v)
 	return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, 0xF, 0xF, false));
 }
 
-__device__ __forceinline__ float cluster_sum_f(float v)
-{
-	v = v + dpp_f<kDppXor1>(v);
-	v = v + dpp_f<kDppXor2>(v);
-	v = v + dpp_f<kDppHalfMirror>(v);
-	return v;
-}
-
-__device__ __forceinline__ int cluster_sum_i(int v)
-{
-	v += dpp_i<kDppXor1>(v);
-	v += dpp_i<kDppXor2>(v);
-	v += dpp_i<kDppHalfMirror>(v);
-	return v;
-}
-
 // Sum over the 64 lanes of a wave (exact for integers): row butterfly with
 // DPP, then the four row sums through readlane.
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
@@ -135,26 +124,6 @@ __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v)
 	x += dpp_i<kDppRowMirror>(x);
 	return (uint32_t)(__builtin_amdgcn_readlane(x, 0) + __builtin_amdgcn_readlane(x, 16) +
 	                  __builtin_amdgcn_readlane(x, 32) + __builtin_amdgcn_readlane(x, 48));
-}
-
-// Maximum over an 8-lane cluster of non-negative, non-NaN floats: their bit
-// patterns order as unsigned integers, so the integer maximum (one DPP
-// instruction a step, no float canonicalization) returns the same float.
-__device__ __forceinline__ float cluster_max_nonneg_f(float v)
-{
-	uint32_t b = __float_as_uint(v);
-	b = max(b, (uint32_t)dpp_i<kDppXor1>((int)b));
-	b = max(b, (uint32_t)dpp_i<kDppXor2>((int)b));
-	b = max(b, (uint32_t)dpp_i<kDppHalfMirror>((int)b));
-	return __uint_as_float(b);
-}
-
-__device__ __forceinline__ uint32_t cluster_or(uint32_t v)
-{
-	v |= (uint32_t)dpp_i<kDppXor1>((int)v);
-	v |= (uint32_t)dpp_i<kDppXor2>((int)v);
-	v |= (uint32_t)dpp_i<kDppHalfMirror>((int)v);
-	return v;
 }
 
 // fp16 rounding (nearest even, subnormals kept) of 0 <= x <= 60000, returned
@@ -186,78 +155,107 @@ __device__ __forceinline__ int wave_incl_scan(int v)
 	return v;
 }
 
-// Exclusive scan over the 128 lanes of a workgroup (two waves).
-__device__ __forceinline__ int block_excl_scan128(int v, int *s_tmp, int &total)
-{
-	const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-	const int incl = wave_incl_scan(v);
-	if (lane == 63)
-		s_tmp[wave] = incl;
-	__syncthreads();
-	int base = wave ? s_tmp[0] : 0;
-	total = s_tmp[0] + s_tmp[1];
-	__syncthreads();
-	return base + incl - v;
-}
-
-// Quantizer state of one lane (one 4x2 subblock).
+// Quantizer state of one lane: subblocks 2j and 2j+1 of its 8x8 block (a
+// 4x4 region; lane = 4 * k8 + j).  Texel h * 8 + p is pixel p of subblock
+// 2j + h, at column p >> 1 and row 2h + (p & 1) of the region.
 struct LaneQuant
 {
-	float t[8];           // scaled texels
-	int n[8];             // magnitudes
-	int b4max;            // subblock max magnitude
+	float t[16];          // scaled texels
+	int n[16];            // magnitudes
+	int b4max[2];         // subblock max magnitudes
 	int msb;              // 8x8 msb, -1 when the 8x8 quantizes to zero
 	uint32_t scale_code;
 	float iq2;            // (1 / (resolution * scale))^2
 };
 
-// reference shaders/wavelet_quant.comp:172-205, 292-316.
-__device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, int k8, int s, LaneQuant &Q)
+// Sums and maxima over the four lanes of one 8x8 block (a quad), in the
+// order ((v0 + v1) + (v2 + v3)) in every lane.
+__device__ __forceinline__ float quad_sum_f(float v)
 {
-	const int x = bx * 32 + 8 * (k8 & 3) + 4 * (s >> 2);
-	const int y = by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
+	v = v + dpp_f<kDppXor1>(v);
+	v = v + dpp_f<kDppXor2>(v);
+	return v;
+}
+
+__device__ __forceinline__ int quad_sum_i(int v)
+{
+	v += dpp_i<kDppXor1>(v);
+	v += dpp_i<kDppXor2>(v);
+	return v;
+}
+
+__device__ __forceinline__ uint32_t quad_or(uint32_t v)
+{
+	v |= (uint32_t)dpp_i<kDppXor1>((int)v);
+	v |= (uint32_t)dpp_i<kDppXor2>((int)v);
+	return v;
+}
+
+// Maximum over a quad of non-negative, non-NaN floats: their bit patterns
+// order as unsigned integers, so the integer maximum (one DPP instruction a
+// step, no float canonicalization) returns the same float.
+__device__ __forceinline__ float quad_max_nonneg_f(float v)
+{
+	uint32_t b = __float_as_uint(v);
+	b = max(b, (uint32_t)dpp_i<kDppXor1>((int)b));
+	b = max(b, (uint32_t)dpp_i<kDppXor2>((int)b));
+	return __uint_as_float(b);
+}
+
+// reference shaders/wavelet_quant.comp:172-205, 292-316.
+__device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, int k8, int j, LaneQuant &Q)
+{
+	const int x = bx * 32 + 8 * (k8 & 3) + 4 * (j >> 1);
+	const int y = by * 32 + 8 * (k8 >> 2) + 4 * (j & 1);
 	// Band storage is padded to whole 32x32 blocks (pitch and rows multiples
-	// of 32, 256-byte aligned), so both rows of the 4x2 subblock load as one
+	// of 32, 256-byte aligned), so the four rows of the region load as one
 	// aligned vector each.  The padding is zero: encoder bands are cleared
 	// when the slot is made and every forward transform stores only inside
 	// its band (x < bw, y < bh), so samples outside the band read as 0.
-	float row[2][4];
+	float row[4][4];
 	const long i0 = (long)y * J.pitch + x;
 	if (J.kind == kStoreF16)
 	{
 		const __half *b16 = static_cast<const __half *>(J.data);
-		const uint2 r0 = *reinterpret_cast<const uint2 *>(b16 + i0);
-		const uint2 r1 = *reinterpret_cast<const uint2 *>(b16 + i0 + J.pitch);
-		const uint32_t w[4] = {r0.x, r0.y, r1.x, r1.y};
 #pragma unroll
-		for (int j = 0; j < 4; j++)
+		for (int r = 0; r < 4; r++)
 		{
-			row[j >> 1][2 * (j & 1)] = __half2float(__ushort_as_half((unsigned short)(w[j] & 0xffffu)));
-			row[j >> 1][2 * (j & 1) + 1] = __half2float(__ushort_as_half((unsigned short)(w[j] >> 16)));
+			const uint2 w = *reinterpret_cast<const uint2 *>(b16 + i0 + r * J.pitch);
+			row[r][0] = __half2float(__ushort_as_half((unsigned short)(w.x & 0xffffu)));
+			row[r][1] = __half2float(__ushort_as_half((unsigned short)(w.x >> 16)));
+			row[r][2] = __half2float(__ushort_as_half((unsigned short)(w.y & 0xffffu)));
+			row[r][3] = __half2float(__ushort_as_half((unsigned short)(w.y >> 16)));
 		}
 	}
 	else
 	{
 		const float *b32 = static_cast<const float *>(J.data);
-		const float4 r0 = *reinterpret_cast<const float4 *>(b32 + i0);
-		const float4 r1 = *reinterpret_cast<const float4 *>(b32 + i0 + J.pitch);
-		row[0][0] = r0.x; row[0][1] = r0.y; row[0][2] = r0.z; row[0][3] = r0.w;
-		row[1][0] = r1.x; row[1][1] = r1.y; row[1][2] = r1.z; row[1][3] = r1.w;
-	}
-	float ms = 0.0f;
 #pragma unroll
-	for (int p = 0; p < 8; p++)
-	{
-		const float v = row[p & 1][p >> 1];
-		// Magnitudes beyond 16383.75 would overflow the 14 planes a block can
-		// describe; they are clamped (reachable only far outside the nominal
-		// coefficient range).
-		float t = fminf(fmaxf(v * J.qres, -16383.75f), 16383.75f);
-		Q.t[p] = t;
-		ms = fmaxf(ms, fabsf(t));
+		for (int r = 0; r < 4; r++)
+		{
+			const float4 w = *reinterpret_cast<const float4 *>(b32 + i0 + r * J.pitch);
+			row[r][0] = w.x;
+			row[r][1] = w.y;
+			row[r][2] = w.z;
+			row[r][3] = w.w;
+		}
 	}
+	float ms[2] = {0.0f, 0.0f};
+#pragma unroll
+	for (int h = 0; h < 2; h++)
+#pragma unroll
+		for (int p = 0; p < 8; p++)
+		{
+			// Magnitudes beyond 16383.75 would overflow the 14 planes a block
+			// can describe; they are clamped (reachable only far outside the
+			// nominal coefficient range).
+			const float v = row[2 * h + (p & 1)][p >> 1];
+			const float t = fminf(fmaxf(v * J.qres, -16383.75f), 16383.75f);
+			Q.t[8 * h + p] = t;
+			ms[h] = fmaxf(ms[h], fabsf(t));
+		}
 	// ms >= 0 and never NaN (t is clamped with fminf / fmaxf).
-	float M = cluster_max_nonneg_f(ms);
+	float M = quad_max_nonneg_f(fmaxf(ms[0], ms[1]));
 
 	float quant_scale;
 	if (M < 1.0f)
@@ -275,16 +273,17 @@ __device__ __forceinline__ void quantize_lane(const BandJob &J, int bx, int by, 
 		quant_scale = 1.0f / ((float)Q.scale_code / 8.0f + 0.25f);
 	}
 #pragma unroll
-	for (int p = 0; p < 8; p++)
+	for (int p = 0; p < 16; p++)
 		Q.t[p] = Q.t[p] * quant_scale;
 	M = M * quant_scale;
-	ms = ms * quant_scale;
 	float inv_quant = 1.0f / (J.qres * quant_scale);
 	Q.iq2 = inv_quant * inv_quant;
 #pragma unroll
-	for (int p = 0; p < 8; p++)
+	for (int p = 0; p < 16; p++)
 		Q.n[p] = abs((int)Q.t[p]);
-	Q.b4max = (int)ms;
+	// max(|t|) * scale = max(|t * scale|): rounding is monotone.
+	Q.b4max[0] = (int)(ms[0] * quant_scale);
+	Q.b4max[1] = (int)(ms[1] * quant_scale);
 	int maxabs = (int)M;
 	Q.msb = find_msb((uint32_t)maxabs);
 }
@@ -294,14 +293,21 @@ __device__ __forceinline__ float dot4_sq(float a, float b, float c, float d)
 	return ((a * a + b * b) + c * c) + d * d;
 }
 
+// Squared magnitude of one subblock's eight values, in the reference's order.
+__device__ __forceinline__ float sub_sq(const float *v)
+{
+	return dot4_sq(v[0], v[1], v[2], v[3]) + dot4_sq(v[4], v[5], v[6], v[7]);
+}
+
 // Code word of an 8x8 block at quant 0 (reference shaders/wavelet_quant.comp:219-229).
-__device__ __forceinline__ uint32_t code_word_q0(const LaneQuant &Q, int s)
+__device__ __forceinline__ uint32_t code_word_q0(const LaneQuant &Q, int j)
 {
 	if (Q.msb < 0)
 		return 0;
 	int qp0 = Q.msb >= 3 ? Q.msb - 2 : 0;
-	uint32_t plane_code = (uint32_t)(find_msb((uint32_t)(Q.b4max >> qp0)) + 1);
-	uint32_t cw = cluster_or(plane_code << (2 * s));
+	const uint32_t code0 = (uint32_t)(find_msb((uint32_t)(Q.b4max[0] >> qp0)) + 1);
+	const uint32_t code1 = (uint32_t)(find_msb((uint32_t)(Q.b4max[1] >> qp0)) + 1);
+	uint32_t cw = quad_or((code0 | (code1 << 2)) << (4 * j));
 	return cw | ((uint32_t)qp0 << 16) | ((Q.scale_code & 0xfu) << 20);
 }
 
@@ -338,10 +344,10 @@ __device__ __forceinline__ BlockCoord locate_block(const BandJob *bands, const u
 	return C;
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *__restrict__ bands,
-                                                                 const uint32_t *__restrict__ blk_band,
-                                                                 const FrameState *__restrict__ frames,
-                                                                 XcdMap xm)
+__global__ __launch_bounds__(kBlockLanes) void k_quant_analyze(const BandJob *__restrict__ bands,
+                                                               const uint32_t *__restrict__ blk_band,
+                                                               const FrameState *__restrict__ frames,
+                                                               XcdMap xm)
 {
 	__shared__ int s_cost[16][16];
 	__shared__ float s_dist[16][16];
@@ -353,49 +359,62 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 
 	const BlockCoord C = locate_block(bands, blk_band, xm);
 	const BandJob &J = C.J;
-	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
+	const int tid = threadIdx.x, j = tid & 3, k8 = tid >> 2;
 	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
 	const bool in_range = x8 < J.bx8 && y8 < J.by8;
 
 	LaneQuant Q;
-	quantize_lane(J, C.bx, C.by, k8, s, Q);
+	quantize_lane(J, C.bx, C.by, k8, j, Q);
 	const int msb = in_range ? Q.msb : -1;
 
 	// Distortion of discarding the whole 8x8 block; the reference leaves out
 	// rdo_distortion_scale here (reference shaders/wavelet_quant.comp:254).
-	float dd = cluster_sum_f((dot4_sq(Q.t[0], Q.t[1], Q.t[2], Q.t[3]) + dot4_sq(Q.t[4], Q.t[5], Q.t[6], Q.t[7])) * Q.iq2);
-	if (s == 0)
+	// Per subblock, then over the eight in the order ((s0 + s1) + (s2 + s3))
+	// + ((s4 + s5) + (s6 + s7)): the lane adds its pair, the quad the rest.
+	const float dd = quad_sum_f(sub_sq(Q.t) * Q.iq2 + sub_sq(Q.t + 8) * Q.iq2);
+	if (j == 0)
 	{
 		s_msb[k8] = msb;
 		s_disc[k8] = msb >= 0 ? round_f16_nonneg(fminf(dd, 60000.0f)) : 0.0f;
 	}
-	// Each wave runs the levels of its own eight 8x8 blocks (the analyze below
-	// reads level q of a block only for q <= its msb): the largest msb of the
-	// wave, equal within each 8-lane cluster.
-	// row_ror 8 pairs the two clusters of each 16-lane row; the four row
-	// maxima are read out as scalars.
-	const int mrow = max(msb, dpp_i<kDppRowRor8>(msb));
+	// The wave runs the levels of its sixteen 8x8 blocks up to their largest
+	// msb (the analyze below reads level q of a block only for q <= its msb):
+	// row_ror 4 and 8 reduce each 16-lane row, the four row maxima are read
+	// out as scalars.
+	int mrow = max(msb, dpp_i<kDppRowRor4>(msb));
+	mrow = max(mrow, dpp_i<kDppRowRor8>(mrow));
 	const int qmax = max(max(__builtin_amdgcn_readlane(mrow, 0), __builtin_amdgcn_readlane(mrow, 16)),
 	                     max(__builtin_amdgcn_readlane(mrow, 32), __builtin_amdgcn_readlane(mrow, 48)));
 
-	float a[8];
+	float a[16];
 #pragma unroll
-	for (int p = 0; p < 8; p++)
+	for (int p = 0; p < 16; p++)
 		a[p] = fabsf(Q.t[p]);
-	u16x2 npair[4];
+	u16x2 npair[8];
 #pragma unroll
-	for (int i = 0; i < 4; i++)
+	for (int i = 0; i < 8; i++)
 		npair[i] = u16x2{(unsigned short)Q.n[2 * i], (unsigned short)Q.n[2 * i + 1]};
-	// Late plane count of the lane's subblock at quant level q, max(early - 1,
-	// 0) of reference shaders/wavelet_quant.comp:139-163.  With m = msb - q
-	// >= 3 the early count is (m - 2) quality planes + a sign plane + the
-	// planes of (b4max >> q) >> (m - 2) = b4max >> (msb - 2), which does not
-	// depend on q; otherwise it is 1 + the planes of b4max >> q when that is
-	// nonzero, else 0.  So both cases are a subtraction from a per-lane
+	// Late plane count of a subblock at quant level q, max(early - 1, 0) of
+	// reference shaders/wavelet_quant.comp:139-163.  With m = msb - q >= 3
+	// the early count is (m - 2) quality planes + a sign plane + the planes
+	// of (b4max >> q) >> (m - 2) = b4max >> (msb - 2), which does not depend
+	// on q; otherwise it is 1 + the planes of b4max >> q when that is
+	// nonzero, else 0.  So both cases are a subtraction from a per-subblock
 	// constant (checked for every msb <= 13, b4max < 2^(msb+1), q <= msb).
-	const int m4 = find_msb((uint32_t)Q.b4max);
-	const int fq = msb >= 2 ? find_msb((uint32_t)Q.b4max >> (msb - 2)) : 0;
-	auto late_planes = [&](int q) { return q <= msb - 3 ? msb - q + fq - 1 : max(m4 - q + 1, 0); };
+	int m4[2], fq[2];
+#pragma unroll
+	for (int h = 0; h < 2; h++)
+	{
+		m4[h] = find_msb((uint32_t)Q.b4max[h]);
+		fq[h] = msb >= 2 ? find_msb((uint32_t)Q.b4max[h] >> (msb - 2)) : 0;
+	}
+	auto late_planes = [&](int q) {
+		int sum = 0;
+#pragma unroll
+		for (int h = 0; h < 2; h++)
+			sum += q <= msb - 3 ? msb - q + fq[h] - 1 : max(m4[h] - q + 1, 0);
+		return sum;
+	};
 	// Level 0: cost only (its distortion is stored as 0, as the reference does).
 	{
 		int packed = 0;
@@ -403,12 +422,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 		{
 			int nsig = 0;
 #pragma unroll
-			for (int p = 0; p < 8; p++)
+			for (int p = 0; p < 16; p++)
 				nsig += Q.n[p] != 0 ? 1 : 0;
 			packed = late_planes(0) | (nsig << 16);
 		}
-		const int sums = cluster_sum_i(packed);
-		if (s == 0 && msb >= 0)
+		const int sums = quad_sum_i(packed);
+		if (j == 0 && msb >= 0)
 		{
 			s_cost[0][k8] = 8 * (sums & 0xffff) + (sums >> 16);
 			s_dist[0][k8] = 0.0f;
@@ -427,13 +446,13 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 			// r is 0 or at least 2^q, so min(r, 2^(q-1)) is 2^(q-1) exactly
 			// for the significant magnitudes and 0 for the others: r plus it
 			// is the reconstruction (0 when not significant), and its sum
-			// over the eight is 2^(q-1) times their significant count.
+			// over the sixteen is 2^(q-1) times their significant count.
 			const unsigned short keep = (unsigned short)(0xffffu << q), half_bit = (unsigned short)(1u << (q - 1));
 			const u16x2 keep2 = {keep, keep}, half2 = {half_bit, half_bit};
 			u16x2 msum = {0, 0};
-			float err[8];
+			float err[16];
 #pragma unroll
-			for (int i = 0; i < 4; i++)
+			for (int i = 0; i < 8; i++)
 			{
 				const u16x2 r = npair[i] & keep2;
 				const u16x2 m = __builtin_elementwise_min(r, half2);
@@ -443,13 +462,12 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 				err[2 * i + 1] = a[2 * i + 1] - (float)recon.y;
 			}
 			const int nsig = (int)(((uint32_t)msum.x + (uint32_t)msum.y) >> (q - 1));
-			se_lane = ((dot4_sq(err[0], err[1], err[2], err[3]) + dot4_sq(err[4], err[5], err[6], err[7])) *
-			           J.rdo_scale) * Q.iq2;
+			se_lane = (sub_sq(err) * J.rdo_scale) * Q.iq2 + (sub_sq(err + 8) * J.rdo_scale) * Q.iq2;
 			packed = late_planes(q) | (nsig << 16);
 		}
-		float se = cluster_sum_f(se_lane);
-		int sums = cluster_sum_i(packed);
-		if (s == 0 && q <= msb)
+		const float se = quad_sum_f(se_lane);
+		const int sums = quad_sum_i(packed);
+		if (j == 0 && q <= msb)
 		{
 			s_cost[q][k8] = 8 * (sums & 0xffff) + (sums >> 16);
 			s_dist[q][k8] = round_f16_nonneg(fminf(se, 60000.0f));
@@ -459,26 +477,26 @@ __global__ __launch_bounds__(kBlockThreads) void k_quant_analyze(const BandJob *
 
 	const FrameState &F = frames[J.frame];
 	{
-		// reference shaders/analyze_rate_control.comp:169-215.  Lanes 8q .. 8q+7
-		// sum quant level q over the 16 8x8 blocks, two blocks per lane; the
-		// cluster butterfly adds the pair sums in the tree order
-		// (((d0+d1)+(d2+d3))+((d4+d5)+(d6+d7))) + (same for d8..d15).
-		const int q = tid >> 3, j = tid & 7;
-		int cpair = 0;
-		float d[2];
+		// reference shaders/analyze_rate_control.comp:169-215.  Lanes 4q ..
+		// 4q+3 sum quant level q over the 16 8x8 blocks, four blocks per lane
+		// as ((d0 + d1) + (d2 + d3)); the quad then adds the lane sums, so the
+		// total is (((d0+d1)+(d2+d3))+((d4+d5)+(d6+d7))) + (same for d8..d15).
+		const int q = tid >> 2;
+		int cost4 = 0;
+		float d[4];
 #pragma unroll
-		for (int i = 0; i < 2; i++)
+		for (int i = 0; i < 4; i++)
 		{
-			const int k = 2 * j + i;
+			const int k = 4 * j + i;
 			// errors[min(q, num_planes)]: past the last plane the entry is the
 			// cost-free "discard" one; zero 8x8 blocks are (0, 0) throughout.
 			const bool live = q <= s_msb[k];
 			const int c = live ? s_cost[q][k] : 0;
 			d[i] = live ? s_dist[q][k] : s_disc[k];
-			cpair += c ? c + 24 : 0;
+			cost4 += c ? c + 24 : 0;
 		}
-		uint32_t total = (uint32_t)cluster_sum_i(cpair);
-		const float D = cluster_sum_f(d[0] + d[1]);
+		uint32_t total = (uint32_t)quad_sum_i(cost4);
+		const float D = quad_sum_f((d[0] + d[1]) + (d[2] + d[3]));
 		if (j == 0)
 		{
 			if (total)
@@ -1035,19 +1053,17 @@ __device__ __forceinline__ void lds_store_byte(uint32_t *buf, uint32_t byte_pos,
 	reinterpret_cast<uint8_t *>(buf)[byte_pos] = (uint8_t)value;
 }
 
-__global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restrict__ bands,
-                                                        const uint32_t *__restrict__ blk_band,
-                                                        FrameState *__restrict__ frames,
-                                                        XcdMap xm)
+__global__ __launch_bounds__(kBlockLanes) void k_pack(const BandJob *__restrict__ bands,
+                                                      const uint32_t *__restrict__ blk_band,
+                                                      FrameState *__restrict__ frames,
+                                                      XcdMap xm)
 {
 	__shared__ __align__(16) uint32_t s_buf[kMaxBlockWords];
-	__shared__ uint32_t s_wave[2][3]; // per wave: scan partial, coded 8x8 mask, coded bits
-	__shared__ uint32_t s_group_base;
 
 	const BlockCoord C = locate_block(bands, blk_band, xm);
 	const BandJob &J = C.J;
 	FrameState &F = frames[J.frame];
-	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
+	const int tid = threadIdx.x, j = tid & 3, k8 = tid >> 2;
 	// The block's 16 word counts load with its quant level (one round trip
 	// instead of two dependent ones).
 	const uint4 *const w16 = reinterpret_cast<const uint4 *>(F.words + (long)C.b * 16);
@@ -1066,131 +1082,130 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 		return;
 	}
 
-	// One 16-byte store per lane clears the block buffer.
-	static_assert(kMaxBlockWords == 4 * kBlockThreads, "one uint4 of the block buffer per lane");
+	// Two 16-byte stores per lane clear the block buffer.
+	static_assert(kMaxBlockWords == 8 * kBlockLanes, "two uint4 of the block buffer per lane");
 	reinterpret_cast<uint4 *>(s_buf)[tid] = make_uint4(0u, 0u, 0u, 0u);
-	if (tid < 64)
-	{
-		// Start of this block's 256-block group: sum of the earlier groups.
-		const int groups = C.b >> 8;
-		uint32_t acc = 0;
-		for (int i = tid; i < groups; i += 64)
-			acc += F.group_sum[i];
-		acc = wave_sum_u32(acc);
-		if (tid == 0)
-			s_group_base = acc;
-	}
+	reinterpret_cast<uint4 *>(s_buf)[tid + kBlockLanes] = make_uint4(0u, 0u, 0u, 0u);
+	// Start of this block's 256-block group: sum of the earlier groups.
+	uint32_t gacc = 0;
+	for (int i = tid; i < (C.b >> 8); i += kBlockLanes)
+		gacc += F.group_sum[i];
+	const uint32_t group_base = wave_sum_u32(gacc);
 
 	const int x8 = C.bx * 4 + (k8 & 3), y8 = C.by * 4 + (k8 >> 2);
 	const bool in_range = x8 < J.bx8 && y8 < J.by8;
 	LaneQuant Q;
-	quantize_lane(J, C.bx, C.by, k8, s, Q);
-	uint32_t cw0 = in_range ? code_word_q0(Q, s) : 0u;
-	uint32_t cw = quantize_code_word(cw0, quant);
+	quantize_lane(J, C.bx, C.by, k8, j, Q);
+	const uint32_t cw0 = in_range ? code_word_q0(Q, j) : 0u;
+	const uint32_t cw = quantize_code_word(cw0, quant);
 	const bool active = (cw & 0xffffu) != 0;
 
+	// Per subblock h (subblock 2j + h, code field at bit 2 (2j + h)): the
+	// planes it had at quant 0 and the ones kept.  Bit planes go MSB first:
+	// plane byte bit p is bit `bit` of magnitude p.  Magnitudes p and p+4
+	// share a register (low / high half), so one plane of all eight is four
+	// shift-and-mask steps and a fold.  The planes kept are bits
+	// in_planes - 1 down to in_planes - out_planes, so a magnitude is
+	// significant (its sign is sent) when it has a bit at or above
+	// in_planes - out_planes, which min(n >> drop, 1) tests on both halves
+	// at once.
 	const uint32_t in_q = (cw0 >> 16) & 0xf, out_q = (cw >> 16) & 0xf;
-	const uint32_t in_planes = ((cw0 >> (2 * s)) & 3u) + in_q;
-	const uint32_t out_planes = active ? ((cw >> (2 * s)) & 3u) + out_q : 0u;
-	// Bit planes, MSB first: plane byte bit p is bit `bit` of magnitude p.
-	// Magnitudes p and p+4 share a register (low / high half), so one plane
-	// of all eight is four shift-and-mask steps and a fold.  The planes kept
-	// are bits in_planes - 1 down to in_planes - out_planes, so a magnitude
-	// is significant (its sign is sent) when it has a bit at or above
-	// in_planes - out_planes; the plane bytes themselves are extracted once
-	// their LDS position is known, after the scan below.
-	u16x2 npair[4];
-	uint32_t pair[4];
+	uint32_t in_planes[2], out_planes[2], sig[2], pair[2][4];
 #pragma unroll
-	for (int i = 0; i < 4; i++)
+	for (int h = 0; h < 2; h++)
 	{
-		npair[i] = u16x2{(unsigned short)Q.n[i], (unsigned short)Q.n[i + 4]};
-		pair[i] = __builtin_bit_cast(uint32_t, npair[i]);
-	}
-	// Significance of both halves at once: min(n >> drop, 1) is 0 or 1 in
-	// each 16-bit lane; bit i of the low half is magnitude i, of the high
-	// half magnitude i + 4.
-	uint32_t sig = 0;
-	if (out_planes)
-	{
-		const unsigned short drop = (unsigned short)(in_planes - out_planes);
-		const u16x2 drop2 = {drop, drop}, one2 = {1, 1};
-		uint32_t t = 0;
+		const int sb = 2 * j + h;
+		in_planes[h] = ((cw0 >> (2 * sb)) & 3u) + in_q;
+		out_planes[h] = active ? ((cw >> (2 * sb)) & 3u) + out_q : 0u;
+		u16x2 np[4];
 #pragma unroll
 		for (int i = 0; i < 4; i++)
-			t |= __builtin_bit_cast(uint32_t, __builtin_elementwise_min(npair[i] >> drop2, one2)) << i;
-		sig = (t & 0xfu) | ((t >> 12) & 0xf0u);
-	}
-	const int nsig = __popc(sig);
-	const int cost8 = cluster_sum_i(8 * (int)out_planes + nsig);
-
-	// Each wave holds eight 8x8 blocks (lane 8k is block k's first lane, and
-	// every lane of a block has the same code word and cost): its coded-block
-	// mask from a ballot, its coded bits (cost + 24 per coded 8x8) as a wave
-	// sum, and its part of one scan of (planes << 16 | significant count)
-	// (both sums stay below 2^16: 128 lanes x 14 planes, 1024 coefficients)
-	// go to LDS together, so one barrier publishes all three.
-	const int lane = tid & 63, wave = tid >> 6;
-	const uint64_t act = __ballot(s == 0 && active);
-	uint32_t act8 = 0;
+		{
+			np[i] = u16x2{(unsigned short)Q.n[8 * h + i], (unsigned short)Q.n[8 * h + i + 4]};
+			pair[h][i] = __builtin_bit_cast(uint32_t, np[i]);
+		}
+		sig[h] = 0;
+		if (out_planes[h])
+		{
+			const unsigned short drop = (unsigned short)(in_planes[h] - out_planes[h]);
+			const u16x2 drop2 = {drop, drop}, one2 = {1, 1};
+			uint32_t t = 0;
 #pragma unroll
-	for (int k = 0; k < 8; k++)
-		act8 |= (uint32_t)((act >> (8 * k)) & 1u) << k;
-	const uint32_t wbits = wave_sum_u32(s == 0 && cost8 ? (uint32_t)cost8 + 24u : 0u);
-	const int scan_v = ((int)out_planes << 16) | nsig;
-	const int scan_incl = wave_incl_scan(scan_v);
-	if (lane == 63)
-	{
-		s_wave[wave][0] = (uint32_t)scan_incl;
-		s_wave[wave][1] = act8;
-		s_wave[wave][2] = wbits;
+			for (int i = 0; i < 4; i++)
+				t |= __builtin_bit_cast(uint32_t, __builtin_elementwise_min(np[i] >> drop2, one2)) << i;
+			sig[h] = (t & 0xfu) | ((t >> 12) & 0xf0u);
+		}
 	}
-	__syncthreads();
-	const uint32_t ballot = s_wave[0][1] | (s_wave[1][1] << 8);
+	const int nsig0 = __popc(sig[0]), nsig = nsig0 + __popc(sig[1]);
+	const int lane_planes = (int)(out_planes[0] + out_planes[1]);
+	const int cost8 = quad_sum_i(8 * lane_planes + nsig);
+
+	// The coded-block mask: every lane of an 8x8 block has the same code
+	// word, so the ballot holds a nibble 0xf or 0 per block; folding the
+	// nibbles gives one bit per block.  The coded bits (cost + 24 per coded
+	// 8x8) and one scan of (planes << 16 | significant count) over the lanes
+	// (both sums stay below 2^16: 64 lanes x 28 planes, 1024 coefficients)
+	// are wave-wide: the block is one wave.
+	uint64_t act = __ballot(active) & 0x1111111111111111ull;
+	act = (act | (act >> 3)) & 0x0303030303030303ull;
+	act = (act | (act >> 6)) & 0x000f000f000f000full;
+	act = (act | (act >> 12)) & 0x000000ff000000ffull;
+	const uint32_t ballot = (uint32_t)((act | (act >> 24)) & 0xffffu);
 	const uint32_t N = __popc(ballot);
-	uint32_t wcount = (s_wave[0][2] + s_wave[1][2] + 31) / 32;
+	const uint32_t bits = wave_sum_u32(j == 0 && cost8 ? (uint32_t)cost8 + 24u : 0u);
+	uint32_t wcount = (bits + 31) / 32;
 	if (wcount)
 		wcount += 2;
-	const int packed_excl = (wave ? (int)s_wave[0][0] : 0) + scan_incl - scan_v;
-	const int packed_total = (int)(s_wave[0][0] + s_wave[1][0]);
+	const int scan_v = (lane_planes << 16) | nsig;
+	const int scan_incl = wave_incl_scan(scan_v);
+	const int packed_excl = scan_incl - scan_v;
+	const int packed_total = __builtin_amdgcn_readlane(scan_incl, 63);
 	const int plane_excl = packed_excl >> 16, sig_excl = packed_excl & 0xffff;
 	const int planes_total = packed_total >> 16;
+	__syncthreads();
 
 	// Lanes own disjoint byte ranges of the block, so control words, scale
 	// bytes and plane bytes are plain byte stores; sign bits share words and
 	// go in with at most two atomic ORs per lane.
 	const uint32_t plane_base = 8 + 3 * N;
-	if (active && s == 0)
+	if (active && j == 0)
 	{
-		uint32_t j = __popc(ballot & ((1u << k8) - 1u));
-		lds_store_byte(s_buf, 8 + 2 * j, cw & 0xffu);
-		lds_store_byte(s_buf, 9 + 2 * j, (cw >> 8) & 0xffu);
-		lds_store_byte(s_buf, 8 + 2 * N + j, (cw >> 16) & 0xffu);
+		uint32_t jb = __popc(ballot & ((1u << k8) - 1u));
+		lds_store_byte(s_buf, 8 + 2 * jb, cw & 0xffu);
+		lds_store_byte(s_buf, 9 + 2 * jb, (cw >> 8) & 0xffu);
+		lds_store_byte(s_buf, 8 + 2 * N + jb, (cw >> 16) & 0xffu);
 	}
-	for (uint32_t pi = 0; pi < out_planes; pi++)
-	{
-		const int bit = (int)in_planes - 1 - (int)pi;
-		uint32_t c = 0;
+	uint32_t pos = plane_base + (uint32_t)plane_excl;
 #pragma unroll
-		for (int i = 0; i < 4; i++)
-			c |= ((pair[i] >> bit) & 0x00010001u) << i;
-		lds_store_byte(s_buf, plane_base + (uint32_t)plane_excl + pi, (c & 0xfu) | ((c >> 12) & 0xf0u));
-	}
-	if (sig)
+	for (int h = 0; h < 2; h++)
+		for (uint32_t pi = 0; pi < out_planes[h]; pi++)
+		{
+			const int bit = (int)in_planes[h] - 1 - (int)pi;
+			uint32_t c = 0;
+#pragma unroll
+			for (int i = 0; i < 4; i++)
+				c |= ((pair[h][i] >> bit) & 0x00010001u) << i;
+			lds_store_byte(s_buf, pos++, (c & 0xfu) | ((c >> 12) & 0xf0u));
+		}
+	if (nsig)
 	{
-		uint32_t bits = 0;
+		// The signs of the significant magnitudes, subblock 2j first, in
+		// magnitude order: at most 16 consecutive bits.
+		uint32_t sbits = 0;
 		int n = 0;
 #pragma unroll
-		for (int p = 0; p < 8; p++)
-		{
-			if ((sig >> p) & 1u)
+		for (int h = 0; h < 2; h++)
+#pragma unroll
+			for (int p = 0; p < 8; p++)
 			{
-				bits |= (Q.t[p] < 0.0f ? 1u : 0u) << n;
-				n++;
+				if ((sig[h] >> p) & 1u)
+				{
+					sbits |= (Q.t[8 * h + p] < 0.0f ? 1u : 0u) << n;
+					n++;
+				}
 			}
-		}
 		const uint32_t sbit = 8u * (plane_base + (uint32_t)planes_total) + (uint32_t)sig_excl;
-		const uint64_t wide = (uint64_t)bits << (sbit & 31);
+		const uint64_t wide = (uint64_t)sbits << (sbit & 31);
 		if ((uint32_t)wide)
 			atomicOr(&s_buf[sbit >> 5], (uint32_t)wide);
 		if ((uint32_t)(wide >> 32))
@@ -1208,10 +1223,10 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 	}
 	__syncthreads();
 
-	const uint32_t offset = s_group_base + F.excl[C.b];
+	const uint32_t offset = group_base + F.excl[C.b];
 	const bool fits = offset + wcount <= F.capacity_words && wcount <= (uint32_t)kMaxBlockWords;
 	if (fits)
-		for (uint32_t i = tid; i < wcount; i += kBlockThreads)
+		for (uint32_t i = tid; i < wcount; i += kBlockLanes)
 			F.bitstream[offset + i] = s_buf[i];
 	if (tid == 0)
 	{
@@ -1229,24 +1244,23 @@ __global__ __launch_bounds__(kBlockThreads) void k_pack(const BandJob *__restric
 // first copied to LDS with coalesced loads and parsed from there.  A malformed
 // block that reads past its own length reads the following payload bytes, as
 // the reference does; bytes past the payload read as zero.
-__global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__restrict__ bands,
-                                                           const uint32_t *__restrict__ blk_band,
-                                                           const FrameState *__restrict__ frames,
-                                                           XcdMap xm)
+__global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restrict__ bands,
+                                                         const uint32_t *__restrict__ blk_band,
+                                                         const FrameState *__restrict__ frames,
+                                                         XcdMap xm)
 {
 	__shared__ uint32_t s_pay[kMaxBlockWords];
 	__shared__ uint32_t s_poff[16];
 	__shared__ uint32_t s_ctrl[16];
 	__shared__ uint32_t s_qb[16];
 	__shared__ uint32_t s_sign_base;
-	__shared__ int s_tmp[2];
 
 	const BlockCoord C = locate_block(bands, blk_band, xm);
 	const BandJob &J = C.J;
 	const FrameState &F = frames[J.frame];
-	const int tid = threadIdx.x, s = tid & 7, k8 = tid >> 3;
-	const int x = C.bx * 32 + 8 * (k8 & 3) + 4 * (s >> 2);
-	const int y = C.by * 32 + 8 * (k8 >> 2) + 2 * (s & 3);
+	const int tid = threadIdx.x, j = tid & 3, k8 = tid >> 2;
+	const int x = C.bx * 32 + 8 * (k8 & 3) + 4 * (j >> 1);
+	const int y = C.by * 32 + 8 * (k8 >> 2) + 4 * (j & 1);
 
 	// The frame's decode inputs are read together, and a meta pair (offset,
 	// words) as one 8-byte load: each dependent lookup costs a round trip.
@@ -1269,13 +1283,17 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 	if (!missing && off >= payload_words)
 		missing = true;
 
-	float v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	// Texel h * 8 + p: pixel p of subblock 2j + h (column p >> 1, row 2h + (p & 1)).
+	float v[16];
+#pragma unroll
+	for (int p = 0; p < 16; p++)
+		v[p] = 0.0f;
 	if (!missing)
 	{
 		const uint32_t *payload = F.dec_payload + off;
-		// The first kBlockThreads words (one per lane; most blocks are
-		// shorter) load together with the header word that gives the
-		// length, so a typical block arrives in one round trip.
+		// The first kBlockLanes words (one per lane; most blocks are not much
+		// longer) load together with the header word that gives the length,
+		// so a typical block arrives in one or two round trips.
 		const uint32_t first = (uint32_t)tid < payload_words - off ? payload[tid] : 0u;
 		const uint32_t w0 = payload[0];
 		uint32_t nwords = (w0 >> 16) & 0xfffu;
@@ -1283,7 +1301,7 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		nwords = nwords < payload_words - off ? nwords : payload_words - off;
 		if ((uint32_t)tid < nwords)
 			s_pay[tid] = first;
-		for (uint32_t i = tid + kBlockThreads; i < nwords; i += kBlockThreads)
+		for (uint32_t i = tid + kBlockLanes; i < nwords; i += kBlockLanes)
 			s_pay[i] = payload[i];
 		__syncthreads();
 		// Byte `pos` of the coded block.  A well-formed block reads only its
@@ -1307,9 +1325,9 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			uint32_t ctrl = 0, qb = 0;
 			if ((ballot >> tid) & 1u)
 			{
-				uint32_t j = __popc(ballot & ((1u << tid) - 1u));
-				ctrl = byte_at(8 + 2 * j) | (byte_at(9 + 2 * j) << 8);
-				qb = byte_at(8 + 2 * N + j);
+				uint32_t jb = __popc(ballot & ((1u << tid) - 1u));
+				ctrl = byte_at(8 + 2 * jb) | (byte_at(9 + 2 * jb) << 8);
+				qb = byte_at(8 + 2 * N + jb);
 			}
 			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
 			msbs |= msbs >> 1;
@@ -1335,44 +1353,14 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 		{
 			const uint32_t ctrl = s_ctrl[k8], qb = s_qb[k8];
 			const uint32_t q_bits = qb & 0xfu;
-			const int bo = 2 * s;
+			const int bo = 4 * j; // code field of subblock 2j
 			const uint32_t lmask = (1u << bo) - 1u;
 			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
 			msbs |= msbs >> 1;
-			uint32_t byte_off = s_poff[k8] + (uint32_t)__popc(lsbs & lmask) + (uint32_t)__popc(msbs & lmask) + q_bits * s;
-			const int planes = (int)(q_bits + ((ctrl >> bo) & 3u));
-			int mag[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-			if (planes <= 16)
-			{
-				// Magnitudes p and p+4 accumulate in the low / high half of one
-				// register: a plane byte is spread once, then four shifts.
-				uint32_t pair[4] = {0, 0, 0, 0};
-				for (int pl = planes - 1; pl >= 0; pl--)
-				{
-					const uint32_t byte = byte_at(byte_off++);
-					const uint32_t c = (byte & 0xfu) | ((byte & 0xf0u) << 12);
-#pragma unroll
-					for (int i = 0; i < 4; i++)
-						pair[i] |= ((c >> i) & 0x00010001u) << pl;
-				}
-#pragma unroll
-				for (int i = 0; i < 4; i++)
-				{
-					mag[i] = (int)(pair[i] & 0xffffu);
-					mag[i + 4] = (int)(pair[i] >> 16);
-				}
-			}
-			else
-			{
-				// Up to 18 planes (only in malformed blocks).
-				for (int pl = planes - 1; pl >= 0; pl--)
-				{
-					uint32_t byte = byte_at(byte_off++);
-#pragma unroll
-					for (int p = 0; p < 8; p++)
-						mag[p] |= (int)((byte >> p) & 1u) << pl;
-				}
-			}
+			// Subblock 2j's planes start after those of the 8x8's earlier
+			// subblocks; subblock 2j + 1's follow them.
+			uint32_t byte_off = s_poff[k8] + (uint32_t)__popc(lsbs & lmask) + (uint32_t)__popc(msbs & lmask) +
+			                    q_bits * (uint32_t)(2 * j);
 			// reference shaders/wavelet_dequant.comp:168-175, 312-315.
 			int ef = (int)(q_code >> 3);
 			ef = ef > 24 ? 24 : ef;
@@ -1381,26 +1369,64 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			const float qstep = (1.0f / (8.0f * 1024.0f * 1024.0f)) * (float)((8 + m) * (1 << (20 + e)));
 			const float inv_scale = qstep * ((float)(qb >> 4) / 8.0f + 0.25f);
 #pragma unroll
-			for (int p = 0; p < 8; p++)
+			for (int h = 0; h < 2; h++)
 			{
-				float f = (float)mag[p];
-				if (f != 0.0f)
+				const int planes = (int)(q_bits + ((ctrl >> (bo + 2 * h)) & 3u));
+				int mag[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+				if (planes <= 16)
 				{
-					f += 0.5f;
-					sig_count++;
+					// Magnitudes p and p+4 accumulate in the low / high half of
+					// one register: a plane byte is spread once, then four
+					// shifts.
+					uint32_t pair[4] = {0, 0, 0, 0};
+					for (int pl = planes - 1; pl >= 0; pl--)
+					{
+						const uint32_t byte = byte_at(byte_off++);
+						const uint32_t c = (byte & 0xfu) | ((byte & 0xf0u) << 12);
+#pragma unroll
+						for (int i = 0; i < 4; i++)
+							pair[i] |= ((c >> i) & 0x00010001u) << pl;
+					}
+#pragma unroll
+					for (int i = 0; i < 4; i++)
+					{
+						mag[i] = (int)(pair[i] & 0xffffu);
+						mag[i + 4] = (int)(pair[i] >> 16);
+					}
 				}
-				v[p] = f * inv_scale;
+				else
+				{
+					// Up to 18 planes (only in malformed blocks).
+					for (int pl = planes - 1; pl >= 0; pl--)
+					{
+						uint32_t byte = byte_at(byte_off++);
+#pragma unroll
+						for (int p = 0; p < 8; p++)
+							mag[p] |= (int)((byte >> p) & 1u) << pl;
+					}
+				}
+#pragma unroll
+				for (int p = 0; p < 8; p++)
+				{
+					float f = (float)mag[p];
+					if (f != 0.0f)
+					{
+						f += 0.5f;
+						sig_count++;
+					}
+					v[8 * h + p] = f * inv_scale;
+				}
 			}
 		}
-		int sig_total;
-		const int sig_excl = block_excl_scan128(sig_count, s_tmp, sig_total);
-		// This lane's sign bits are consecutive (at most 8): two bytes hold
-		// them all.
-		const uint32_t sbit = s_sign_base + (uint32_t)sig_excl;
-		const uint32_t signs = (byte_at(sbit >> 3) | (byte_at((sbit >> 3) + 1) << 8)) >> (sbit & 7);
+		const int sig_incl = wave_incl_scan(sig_count);
+		// This lane's sign bits are consecutive (at most 16, subblock 2j's
+		// first): three bytes hold them all.
+		const uint32_t sbit = s_sign_base + (uint32_t)(sig_incl - sig_count);
+		const uint32_t sb = sbit >> 3;
+		const uint32_t signs = (byte_at(sb) | (byte_at(sb + 1) << 8) | (byte_at(sb + 2) << 16)) >> (sbit & 7);
 		int n = 0;
 #pragma unroll
-		for (int p = 0; p < 8; p++)
+		for (int p = 0; p < 16; p++)
 		{
 			if (v[p] != 0.0f)
 			{
@@ -1410,36 +1436,38 @@ __global__ __launch_bounds__(kBlockThreads) void k_dequant(const BandJob *__rest
 			}
 		}
 	}
-	// Both rows of the 4x2 subblock go out as one vector store each (band
-	// storage is padded to whole 32x32 blocks, so they are in bounds and
-	// aligned): sample p sits at row p & 1, column p >> 1.
+	// Each row of the 4x4 region goes out as one vector store (band storage
+	// is padded to whole 32x32 blocks, so they are in bounds and aligned):
+	// row r holds subblock r >> 1's pixels p with p & 1 == r & 1.
 	const long i0 = (long)y * J.pitch + x;
 	if (J.kind == kStoreF16)
 	{
 		__half *b16 = static_cast<__half *>(J.data);
-		uint2 r[2];
 		// Each value rounds to fp16 from its float (no merge with the
 		// product that made it into one mixed-precision rounding).
 #pragma unroll
-		for (int p = 0; p < 8; p++)
+		for (int p = 0; p < 16; p++)
 			asm volatile("" : "+v"(v[p]));
 #pragma unroll
-		for (int row = 0; row < 2; row++)
+		for (int r = 0; r < 4; r++)
 		{
-			const uint32_t h0 = __half_as_ushort(__float2half_rn(v[row]));
-			const uint32_t h1 = __half_as_ushort(__float2half_rn(v[row + 2]));
-			const uint32_t h2 = __half_as_ushort(__float2half_rn(v[row + 4]));
-			const uint32_t h3 = __half_as_ushort(__float2half_rn(v[row + 6]));
-			r[row] = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
+			const float *w = v + 8 * (r >> 1) + (r & 1);
+			const uint32_t h0 = __half_as_ushort(__float2half_rn(w[0]));
+			const uint32_t h1 = __half_as_ushort(__float2half_rn(w[2]));
+			const uint32_t h2 = __half_as_ushort(__float2half_rn(w[4]));
+			const uint32_t h3 = __half_as_ushort(__float2half_rn(w[6]));
+			*reinterpret_cast<uint2 *>(b16 + i0 + r * J.pitch) = make_uint2(h0 | (h1 << 16), h2 | (h3 << 16));
 		}
-		*reinterpret_cast<uint2 *>(b16 + i0) = r[0];
-		*reinterpret_cast<uint2 *>(b16 + i0 + J.pitch) = r[1];
 	}
 	else
 	{
 		float *b32 = static_cast<float *>(J.data);
-		*reinterpret_cast<float4 *>(b32 + i0) = make_float4(v[0], v[2], v[4], v[6]);
-		*reinterpret_cast<float4 *>(b32 + i0 + J.pitch) = make_float4(v[1], v[3], v[5], v[7]);
+#pragma unroll
+		for (int r = 0; r < 4; r++)
+		{
+			const float *w = v + 8 * (r >> 1) + (r & 1);
+			*reinterpret_cast<float4 *>(b32 + i0 + r * J.pitch) = make_float4(w[0], w[2], w[4], w[6]);
+		}
 	}
 }
 } // namespace
@@ -1472,7 +1500,7 @@ void launch_quant_analyze(const BandJob *bands, const uint32_t *blk_band, int nb
                           hipStream_t stream)
 {
 	if (nblocks > 0)
-		hipLaunchKernelGGL(k_quant_analyze, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
+		hipLaunchKernelGGL(k_quant_analyze, dim3(nblocks), dim3(kBlockLanes), 0, stream, bands, blk_band, frames,
 		                   xcd_map(dim3(nblocks)));
 }
 
@@ -1504,7 +1532,7 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
                  hipStream_t stream)
 {
 	if (nblocks > 0)
-		hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
+		hipLaunchKernelGGL(k_pack, dim3(nblocks), dim3(kBlockLanes), 0, stream, bands, blk_band, frames,
 		                   xcd_map(dim3(nblocks)));
 }
 
@@ -1512,7 +1540,7 @@ void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks,
                     hipStream_t stream)
 {
 	if (nblocks > 0)
-		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockThreads), 0, stream, bands, blk_band, frames,
+		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockLanes), 0, stream, bands, blk_band, frames,
 		                   xcd_map(dim3(nblocks)));
 }
 } // namespace pw

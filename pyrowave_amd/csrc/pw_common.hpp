@@ -47,15 +47,18 @@ constexpr int kInvStreamVariant = 4;
 constexpr int kHalfStreamVariant = 5;
 // Shape of a level that has not been autotuned: the streamed one when the
 // level has at least kStreamMinTiles of its tiles (eight waves per CU; it is
-// the fastest on the large levels of a batch), else the tiled shape 0 (a
+// the fastest on the large levels of a batch), else a tiled shape (a
 // streamed wave walks its rows one after another, which a level with few
-// tiles cannot hide: one 1080p frame took 176 instead of 129 us).  The
-// autotuner keeps the default unless another shape is 2% faster.
+// tiles cannot hide: one 1080p frame took 176 instead of 129 us): forward
+// 128x40 (shape 3), inverse 32x32 (shape 1), the autotuner's choice on the
+// small levels of lone 720p, 1080p and 4K frames (a lone 1080p FP16 frame
+// 118 -> 111 us against shape 0 everywhere).  The autotuner keeps the
+// default unless another shape is 2% faster.
 constexpr int kStreamMinTiles = 2048;
 static_assert(kFwdStreamVariant == kInvStreamVariant, "one streamed shape index");
-constexpr int default_dwt_variant(int stream_tiles)
+constexpr int default_dwt_variant(bool forward, int stream_tiles)
 {
-	return stream_tiles >= kStreamMinTiles ? kFwdStreamVariant : 0;
+	return stream_tiles >= kStreamMinTiles ? kFwdStreamVariant : forward ? 3 : 1;
 }
 constexpr int kFwdVariantTileW[kDwtVariants] = {128, 64, 128, 128, 120, 120};
 constexpr int kFwdVariantTileH[kDwtVariants] = {56, 56, 48, 40, 32, 32};

@@ -729,7 +729,7 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 			variant[l] = kHalfStreamVariant;
 			continue;
 		}
-		variant[l] = default_dwt_variant(tiles[l][kFwdStreamVariant]);
+		variant[l] = default_dwt_variant(encoder, tiles[l][kFwdStreamVariant]);
 		auto it = D.tuned_variant.find(tune_key(encoder, l, tiles[l][0]));
 		if (it != D.tuned_variant.end())
 			variant[l] = it->second;
@@ -899,7 +899,7 @@ void tune_level(Plan &P, bool encoder, int l, int reps, hipStream_t stream, Laun
 			best = v;
 			best_ms = v_ms[v];
 		}
-	const int dflt = default_dwt_variant(P.tiles[l][kFwdStreamVariant]);
+	const int dflt = default_dwt_variant(encoder, P.tiles[l][kFwdStreamVariant]);
 	const float default_ms = P.tiles[l][dflt] > 0 ? v_ms[dflt] : -1.0f;
 	if (best != dflt && default_ms >= 0.0f && best_ms > default_ms * (1.0f - kTuneMargin))
 		best = dflt;
