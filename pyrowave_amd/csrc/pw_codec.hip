@@ -1320,10 +1320,13 @@ __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restri
 		const uint32_t ballot = w0 & 0xffffu;
 		const uint32_t q_code = byte_at(4);
 		const uint32_t N = __popc(ballot);
-		if (tid < 16)
 		{
+			// Lanes 0-15 read 8x8 block `lane`'s control word and q / scale
+			// byte and scan the plane bytes before it (row_shr DPP steps: an
+			// inclusive scan within the 16-lane row; every lane takes part, the
+			// other rows see no coded blocks and write nothing).
 			uint32_t ctrl = 0, qb = 0;
-			if ((ballot >> tid) & 1u)
+			if (tid < 16 && ((ballot >> tid) & 1u))
 			{
 				uint32_t jb = __popc(ballot & ((1u << tid) - 1u));
 				ctrl = byte_at(8 + 2 * jb) | (byte_at(9 + 2 * jb) << 8);
@@ -1331,22 +1334,27 @@ __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restri
 			}
 			uint32_t lsbs = ctrl & 0x5555u, msbs = ctrl & 0xaaaau;
 			msbs |= msbs >> 1;
-			uint32_t cost = (uint32_t)__popc(lsbs) + (uint32_t)__popc(msbs) + (qb & 0xfu) * 8;
-			uint32_t incl = cost;
-#pragma unroll
-			for (int d = 1; d < 16; d <<= 1)
+			const int cost = __popc(lsbs) + __popc(msbs) + (int)(qb & 0xfu) * 8;
+			int incl = cost;
+			incl += __builtin_amdgcn_update_dpp(0, incl, 0x111, 0xF, 0xF, true);
+			incl += __builtin_amdgcn_update_dpp(0, incl, 0x112, 0xF, 0xF, true);
+			incl += __builtin_amdgcn_update_dpp(0, incl, 0x114, 0xF, 0xF, true);
+			incl += __builtin_amdgcn_update_dpp(0, incl, 0x118, 0xF, 0xF, true);
+			if (tid < 16)
 			{
-				uint32_t up = __shfl_up(incl, d, 16);
-				if (tid >= d)
-					incl += up;
+				s_poff[tid] = 8 + 3 * N + (uint32_t)(incl - cost);
+				s_ctrl[tid] = ctrl;
+				s_qb[tid] = qb;
 			}
-			s_poff[tid] = 8 + 3 * N + incl - cost;
-			s_ctrl[tid] = ctrl;
-			s_qb[tid] = qb;
 			if (tid == 15)
-				s_sign_base = 8u * (8 + 3 * N + incl);
+				s_sign_base = 8u * (8 + 3 * N + (uint32_t)incl);
 		}
 		__syncthreads();
+		// Whether the plane bytes (everything before the sign bits) lie inside
+		// the block's own words: true for every block an encoder writes, and
+		// then they are read from LDS without a bounds check.
+		const uint8_t *const pay8 = reinterpret_cast<const uint8_t *>(s_pay);
+		const bool planes_inside = s_sign_base <= 8u * nbytes;
 
 		int sig_count = 0;
 		if ((ballot >> k8) & 1u)
@@ -1379,14 +1387,18 @@ __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restri
 					// one register: a plane byte is spread once, then four
 					// shifts.
 					uint32_t pair[4] = {0, 0, 0, 0};
-					for (int pl = planes - 1; pl >= 0; pl--)
-					{
-						const uint32_t byte = byte_at(byte_off++);
+					auto spread = [&](uint32_t byte, int pl) {
 						const uint32_t c = (byte & 0xfu) | ((byte & 0xf0u) << 12);
 #pragma unroll
 						for (int i = 0; i < 4; i++)
 							pair[i] |= ((c >> i) & 0x00010001u) << pl;
-					}
+					};
+					if (planes_inside)
+						for (int pl = planes - 1; pl >= 0; pl--)
+							spread(pay8[byte_off++], pl);
+					else
+						for (int pl = planes - 1; pl >= 0; pl--)
+							spread(byte_at(byte_off++), pl);
 #pragma unroll
 					for (int i = 0; i < 4; i++)
 					{
@@ -1420,10 +1432,24 @@ __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restri
 		}
 		const int sig_incl = wave_incl_scan(sig_count);
 		// This lane's sign bits are consecutive (at most 16, subblock 2j's
-		// first): three bytes hold them all.
+		// first).  When all of the block's sign bits lie inside its own words
+		// (every block an encoder writes), the two LDS words around them hold
+		// them (bits read past the block's end are not used); else three
+		// bytes through the bounds-checked reader.
 		const uint32_t sbit = s_sign_base + (uint32_t)(sig_incl - sig_count);
-		const uint32_t sb = sbit >> 3;
-		const uint32_t signs = (byte_at(sb) | (byte_at(sb + 1) << 8) | (byte_at(sb + 2) << 16)) >> (sbit & 7);
+		const uint32_t sign_end = s_sign_base + (uint32_t)__builtin_amdgcn_readlane(sig_incl, 63);
+		uint32_t signs;
+		if (sign_end <= 8u * nbytes)
+		{
+			const uint32_t w = sbit >> 5;
+			const uint64_t two = (uint64_t)s_pay[w] | ((uint64_t)s_pay[min(w + 1, (uint32_t)kMaxBlockWords - 1u)] << 32);
+			signs = (uint32_t)(two >> (sbit & 31));
+		}
+		else
+		{
+			const uint32_t sb = sbit >> 3;
+			signs = (byte_at(sb) | (byte_at(sb + 1) << 8) | (byte_at(sb + 2) << 16)) >> (sbit & 7);
+		}
 		int n = 0;
 #pragma unroll
 		for (int p = 0; p < 16; p++)
