@@ -524,15 +524,21 @@ __global__ __launch_bounds__(kBlockLanes) void k_quant_analyze(const BandJob *__
 		s_v[q] = bi < 112u + (uint32_t)q ? bi : 112u + (uint32_t)q;
 	}
 	__syncthreads();
-	if (tid < 16)
 	{
+		// Bucket of level q: the maximum over k <= q of s_v[k] + (q - k)
+		// (inclusive_max_clustered16, reference shaders/analyze_rate_control.comp:85-99), that is q plus
+		// the prefix maximum of s_v[k] - k over lanes 0..q: a row-shift DPP
+		// max scan on s_v[k] - k + 16 (positive, so the zero the first lanes
+		// of the row read in never wins).
 		const int q = tid;
-		uint32_t Bq = s_v[q];
-		for (int k = 0; k < q; k++)
-		{
-			uint32_t cand = s_v[k] + (uint32_t)(q - k);
-			Bq = cand > Bq ? cand : Bq;
-		}
+		int m = tid < 16 ? (int)s_v[q] - q + 16 : 0;
+		m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x111, 0xF, 0xF, true));
+		m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x112, 0xF, 0xF, true));
+		m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x114, 0xF, 0xF, true));
+		m = max(m, __builtin_amdgcn_update_dpp(0, m, 0x118, 0xF, 0xF, true));
+		if (tid >= 16)
+			return;
+		const uint32_t Bq = (uint32_t)(m - 16 + q);
 		F.words[(long)C.b * 16 + q] = (uint16_t)s_words[q];
 		F.bucket[(long)C.b * 16 + q] = (uint8_t)Bq;
 		if (q != 0)
@@ -1190,20 +1196,20 @@ __global__ __launch_bounds__(kBlockLanes) void k_pack(const BandJob *__restrict_
 	if (nsig)
 	{
 		// The signs of the significant magnitudes, subblock 2j first, in
-		// magnitude order: at most 16 consecutive bits.
+		// magnitude order: at most 16 consecutive bits.  Without branches:
+		// the sign bit of every texel (a significant one is nonzero, so its
+		// sign bit is t < 0), kept where significant and moved down to the
+		// count of significant texels before it.
+		const uint32_t sig16 = sig[0] | (sig[1] << 8);
+		uint32_t neg = 0;
+#pragma unroll
+		for (int p = 0; p < 16; p++)
+			neg |= (__float_as_uint(Q.t[p]) >> 31) << p;
+		neg &= sig16;
 		uint32_t sbits = 0;
-		int n = 0;
 #pragma unroll
-		for (int h = 0; h < 2; h++)
-#pragma unroll
-			for (int p = 0; p < 8; p++)
-			{
-				if ((sig[h] >> p) & 1u)
-				{
-					sbits |= (Q.t[8 * h + p] < 0.0f ? 1u : 0u) << n;
-					n++;
-				}
-			}
+		for (int p = 0; p < 16; p++)
+			sbits |= ((neg >> p) & 1u) << __popc(sig16 & ((1u << p) - 1u));
 		const uint32_t sbit = 8u * (plane_base + (uint32_t)planes_total) + (uint32_t)sig_excl;
 		const uint64_t wide = (uint64_t)sbits << (sbit & 31);
 		if ((uint32_t)wide)
