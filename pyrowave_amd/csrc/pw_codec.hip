@@ -599,6 +599,48 @@ __global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, 
 	F.dec_payload_bytes = P.dec_payload_bytes;
 }
 
+// The finalize scan of reference shaders/analyze_rate_control_finalize.comp:
+// 14-38 on 512 threads holding four entries each (v: the thread's inclusive
+// in-thread prefix; w = v[3] its total).  The reference's Hillis-Steele loop
+// runs steps 1 .. 128 (quirk Q1), after which thread t holds its own prefix
+// plus the totals of threads t - 255 .. t - 1; the exact variant (steps up to
+// 256) adds every earlier thread's.  Both are differences of one exclusive
+// prefix P of the totals: P[t] - P[max(0, t - 255)], or P[t] (exact modulo
+// 2^32, like the sums themselves).  Threads 0-511 hold the entries (`scanner`);
+// every thread of the workgroup reaches the barriers.  On return s_scan[t]
+// holds v[3] for t < 512.
+__device__ __forceinline__ void window_scan512(uint32_t (&v)[4], bool exact, uint32_t *s_scan, bool scanner)
+{
+	__shared__ uint32_t s_wave[8];
+	const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+	const uint32_t w = v[3];
+	const uint32_t incl = (uint32_t)wave_incl_scan((int)w);
+	if (scanner && lane == 63)
+		s_wave[wave] = incl;
+	__syncthreads();
+	uint32_t excl = 0;
+	if (scanner)
+	{
+		uint32_t base = 0;
+#pragma unroll
+		for (int k = 0; k < 8; k++)
+			base += k < wave ? s_wave[k] : 0u;
+		excl = base + incl - w;
+		s_scan[t] = excl;
+	}
+	__syncthreads();
+	const uint32_t add = exact ? excl : excl - (scanner && t >= 255 ? s_scan[t - 255] : 0u);
+	__syncthreads();
+	if (scanner)
+	{
+#pragma unroll
+		for (int i = 0; i < 4; i++)
+			v[i] += add;
+		s_scan[t] = v[3];
+	}
+	__syncthreads();
+}
+
 // reference shaders/analyze_rate_control_finalize.comp:14-38 (512 threads x 4
 // entries) and the skip / required logic of shaders/resolve_rate_control.comp:38-56.
 __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ frames)
@@ -638,21 +680,7 @@ __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ fr
 	v[1] += v[0];
 	v[2] += v[1];
 	v[3] += v[2];
-	s_scan[t] = v[3];
-	__syncthreads();
-	// The reference stops at step < 256; the exact variant runs one more step.
-	const uint32_t limit = (F.flags & 1u) ? 512u : 256u;
-	for (uint32_t step = 1; step < limit; step *= 2)
-	{
-		__syncthreads();
-		uint32_t up = t >= (int)step ? s_scan[t - step] : 0u;
-		__syncthreads();
-#pragma unroll
-		for (int i = 0; i < 4; i++)
-			v[i] += up;
-		s_scan[t] = v[3];
-	}
-	__syncthreads();
+	window_scan512(v, (F.flags & 1u) != 0, s_scan, true);
 	const int required_base = (int)s_consumed - F.target_words;
 	uint32_t prev = t ? s_scan[t - 1] : 0u;
 #pragma unroll
@@ -876,24 +904,8 @@ __global__ __launch_bounds__(1024) void k_rc_resolve(FrameState *__restrict__ fr
 		v[1] += v[0];
 		v[2] += v[1];
 		v[3] += v[2];
-		s_scan[t] = v[3];
 	}
-	__syncthreads();
-	const uint32_t limit = (F.flags & 1u) ? 512u : 256u;
-	for (uint32_t step = 1; step < limit; step *= 2)
-	{
-		__syncthreads();
-		const uint32_t up = scanner && t >= (int)step ? s_scan[t - step] : 0u;
-		__syncthreads();
-		if (scanner)
-		{
-#pragma unroll
-			for (int i = 0; i < 4; i++)
-				v[i] += up;
-			s_scan[t] = v[3];
-		}
-	}
-	__syncthreads();
+	window_scan512(v, (F.flags & 1u) != 0, s_scan, scanner);
 	if (scanner)
 	{
 		const int required_base = (int)s_consumed - F.target_words;
