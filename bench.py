@@ -61,6 +61,8 @@ def parse():
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the 1080p, uneven-batch and serving legs (profiled runs: some of their transform "
                          "launches have the grids of the headline batch's levels)")
+    ap.add_argument("--lone-only", action="store_true",
+                    help="only the lone 1080p FP16 frame (eager and graph): profiled runs of the latency path")
     ap.add_argument("--serving-seconds", type=float, default=2.0,
                     help="length of each serving-loop run (64 sessions); 0 skips it")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -125,11 +127,11 @@ def fused_forward(batch):
 
 
 # Fused inverse tiles of one 3840x2160 4:2:0 frame: 120 plane samples wide,
-# 64 rows high in launches of at least 16384 such tiles, else 32 rows; the
-# library runs launches of fewer than 512 (32-row) tiles level by level.
+# 64 rows high in launches of at least 16384 such tiles, 16 rows in launches
+# of fewer than 4096 32-row tiles, else 32 rows.
 INV_FUSED_TILE_W = 120
-INV_FUSED_TALL_H, INV_FUSED_SHORT_H = 64, 32
-INV_FUSED_TALL_TILES, INV_FUSED_MIN_TILES = 16384, 512
+INV_FUSED_TALL_H, INV_FUSED_SHORT_H, INV_FUSED_LONE_H = 64, 32, 16
+INV_FUSED_TALL_TILES, INV_FUSED_LONE_TILES = 16384, 4096
 
 
 def inv_fused_tiles(batch, tile_h):
@@ -139,14 +141,16 @@ def inv_fused_tiles(batch, tile_h):
 
 
 def inv_fused_tile_h(batch):
-    return INV_FUSED_TALL_H if inv_fused_tiles(batch, INV_FUSED_TALL_H) >= INV_FUSED_TALL_TILES else INV_FUSED_SHORT_H
+    if inv_fused_tiles(batch, INV_FUSED_TALL_H) >= INV_FUSED_TALL_TILES:
+        return INV_FUSED_TALL_H
+    return INV_FUSED_LONE_H if inv_fused_tiles(batch, INV_FUSED_SHORT_H) < INV_FUSED_LONE_TILES else INV_FUSED_SHORT_H
 
 
 def fused_inverse(batch):
     """Whether a batch of 4K 4:2:0 frames takes the fused inverse launch of
     each component's first two levels."""
     mode = int(os.environ.get("PYROWAVE_FUSED", "1"), 0)
-    return mode >= 2 or (mode == 1 and inv_fused_tiles(batch, INV_FUSED_SHORT_H) >= INV_FUSED_MIN_TILES)
+    return mode >= 1
 
 
 def dwt_algorithmic_bytes(fp16):
@@ -310,11 +314,11 @@ def pmc_summary(fetch_csv, write_csv):
 
 # Transform tile shapes by index: (kernel, tile width, tile height, tiles per workgroup).
 TRANSFORM_SHAPES = {
-    "forward": [("k_dwt_forward<128,56,true>", 128, 56, 1), ("k_dwt_forward<64,56,true>", 64, 56, 1),
-                ("k_dwt_forward<128,48,false>", 128, 48, 1), ("k_dwt_forward<128,40,false>", 128, 40, 1),
+    "forward": [("k_dwt_forward<128,56,true>", 128, 56, 1), ("k_dwt_forward<32,32,false>", 32, 32, 1),
+                ("k_dwt_forward<32,16,false>", 32, 16, 1), ("k_dwt_forward<128,40,false>", 128, 40, 1),
                 ("k_dwt_forward_stream", 120, 32, 4)],
     "inverse": [("k_dwt_inverse<64,64,256,16,false>", 64, 64, 1), ("k_dwt_inverse<32,32,128,16,false>", 32, 32, 1),
-                ("k_dwt_inverse<64,56,256,16,false>", 64, 56, 1), ("k_dwt_inverse<64,48,256,24,true>", 64, 48, 1),
+                ("k_dwt_inverse<16,16,64,16,false>", 16, 16, 1), ("k_dwt_inverse<64,48,256,24,true>", 64, 48, 1),
                 ("k_dwt_inverse_stream", 120, 32, 4)],
 }
 
@@ -578,6 +582,40 @@ def main():
                     stages=stages, roof=roof, B=B, nstreams=nstreams, tuned=tuned)
 
     B = args.batch
+    # Launch-bound case: one 1920x1080 4:2:0 frame per call (crop of the
+    # resident sequence), FP16, eager launches vs a replayed hipGraph.
+    def single_1080p(graphs, steps=60, warmup=5):
+        d2 = pw.Device()
+        d2.set_graphs(graphs)
+        d2.set_timing(False)
+        e = pw.Encoder(d2, 1920, 1080, pw.CHROMA_420, precision=pw.PRECISION_FP16)
+        dd = pw.Decoder(d2, 1920, 1080, pw.CHROMA_420, precision=pw.PRECISION_FP16)
+        out = [torch.empty((1080, 1920), dtype=torch.uint8, device="cuda"),
+               torch.empty((540, 960), dtype=torch.uint8, device="cuda"),
+               torch.empty((540, 960), dtype=torch.uint8, device="cuda")]
+        tg = [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in out])]
+        frs = [pw.hip_frame([(f[0].data_ptr(), f[0].stride(0), 1), (f[1].data_ptr(), f[1].stride(0), 1),
+                             (f[2].data_ptr(), f[2].stride(0), 1)]) for f in dev_frames[:8]]
+        for k in range(warmup):
+            pw.encode_batch([e], [frs[k % 8]], [400000], stream=sptr)
+            pw.decode_encoded_batch([dd], [e], tg, stream=sptr)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for k in range(steps):
+            pw.encode_batch([e], [frs[k % 8]], [400000], stream=sptr)
+            pw.decode_encoded_batch([dd], [e], tg, stream=sptr)
+        torch.cuda.synchronize()
+        us = (time.perf_counter() - t) / steps * 1e6
+        e.close()
+        dd.close()
+        d2.close()
+        return round(us, 2)
+
+    if args.lone_only:
+        print(json.dumps({"single_1080p_fp16_us_per_frame": {"eager": single_1080p(False),
+                                                              "graph": single_1080p(True)}}), flush=True)
+        return
+
     fp16 = timed(pw.PRECISION_FP16, B, args.steps, args.warmup, args.streams)
     print(f"[bench] fp16 batched done", file=sys.stderr, flush=True)
     fp32 = timed(pw.PRECISION_FP32, 1, max(args.steps, 8), args.warmup, 1)
@@ -654,35 +692,6 @@ def main():
                 "traffic": measured_traffic(direction, shapes, args.batch, fused_first),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch), "avg_launch_us": round(us_per_launch, 2),
                 "launches_timed": launches, "launches_per_batch": per_batch}
-
-    # Launch-bound case: one 1920x1080 4:2:0 frame per call (crop of the
-    # resident sequence), FP16, eager launches vs a replayed hipGraph.
-    def single_1080p(graphs, steps=60, warmup=5):
-        d2 = pw.Device()
-        d2.set_graphs(graphs)
-        d2.set_timing(False)
-        e = pw.Encoder(d2, 1920, 1080, pw.CHROMA_420, precision=pw.PRECISION_FP16)
-        dd = pw.Decoder(d2, 1920, 1080, pw.CHROMA_420, precision=pw.PRECISION_FP16)
-        out = [torch.empty((1080, 1920), dtype=torch.uint8, device="cuda"),
-               torch.empty((540, 960), dtype=torch.uint8, device="cuda"),
-               torch.empty((540, 960), dtype=torch.uint8, device="cuda")]
-        tg = [pw.hip_target([(o.data_ptr(), o.stride(0)) for o in out])]
-        frs = [pw.hip_frame([(f[0].data_ptr(), f[0].stride(0), 1), (f[1].data_ptr(), f[1].stride(0), 1),
-                             (f[2].data_ptr(), f[2].stride(0), 1)]) for f in dev_frames[:8]]
-        for k in range(warmup):
-            pw.encode_batch([e], [frs[k % 8]], [400000], stream=sptr)
-            pw.decode_encoded_batch([dd], [e], tg, stream=sptr)
-        torch.cuda.synchronize()
-        t = time.perf_counter()
-        for k in range(steps):
-            pw.encode_batch([e], [frs[k % 8]], [400000], stream=sptr)
-            pw.decode_encoded_batch([dd], [e], tg, stream=sptr)
-        torch.cuda.synchronize()
-        us = (time.perf_counter() - t) / steps * 1e6
-        e.close()
-        dd.close()
-        d2.close()
-        return round(us, 2)
 
     single = f1080 = uneven = None
     if not args.headline_only:
@@ -807,8 +816,8 @@ def main():
         "ms_per_step": round(fp16["wall"] / args.steps * 1e3, 3),
         "fp16_one_stream_us_per_frame": round(us_fp16_one, 2),
         "autotuned_tile_shapes": {"forward_levels_0_4": fp16["tuned"][0], "inverse_levels_0_4": fp16["tuned"][1],
-                                  "shapes": "forward 0: 128x56, 1: 64x56, 2: 128x48, 3: 128x40; inverse 0: 64x64, "
-                                            "1: 32x32, 2: 64x56, 3: 64x48; both 4: 120x32 streamed"},
+                                  "shapes": "forward 0: 128x56, 1: 32x32, 2: 32x16, 3: 128x40; inverse 0: 64x64, "
+                                            "1: 32x32, 2: 16x16, 3: 64x48; both 4: 120x32 streamed"},
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

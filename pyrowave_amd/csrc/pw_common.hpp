@@ -49,21 +49,24 @@ constexpr int kHalfStreamVariant = 5;
 // level has at least kStreamMinTiles of its tiles (eight waves per CU; it is
 // the fastest on the large levels of a batch), else a tiled shape (a
 // streamed wave walks its rows one after another, which a level with few
-// tiles cannot hide: one 1080p frame took 176 instead of 129 us): forward
-// 128x40 (shape 3), inverse 32x32 (shape 1), the autotuner's choice on the
-// small levels of lone 720p, 1080p and 4K frames (a lone 1080p FP16 frame
-// 118 -> 111 us against shape 0 everywhere).  The autotuner keeps the
-// default unless another shape is 2% faster.
+// tiles cannot hide: one 1080p frame took 176 instead of 129 us): the small
+// tiles the autotuner keeps on the levels of lone 720p, 1080p and 4K frames,
+// forward 32x32 (shape 1) from 512 streamed tiles and 32x16 (shape 2) below,
+// inverse 32x32 (shape 1) from 128 and 16x16 (shape 2) below (a lone 1080p
+// FP16 frame 118 -> 101 us against shape 0 everywhere).  The autotuner keeps
+// the default unless another shape is 2% faster.
 constexpr int kStreamMinTiles = 2048;
 static_assert(kFwdStreamVariant == kInvStreamVariant, "one streamed shape index");
 constexpr int default_dwt_variant(bool forward, int stream_tiles)
 {
-	return stream_tiles >= kStreamMinTiles ? kFwdStreamVariant : forward ? 3 : 1;
+	return stream_tiles >= kStreamMinTiles ? kFwdStreamVariant
+	       : forward                       ? (stream_tiles >= 512 ? 1 : 2)
+	                                       : (stream_tiles >= 128 ? 1 : 2);
 }
-constexpr int kFwdVariantTileW[kDwtVariants] = {128, 64, 128, 128, 120, 120};
-constexpr int kFwdVariantTileH[kDwtVariants] = {56, 56, 48, 40, 32, 32};
-constexpr int kInvVariantTileW[kDwtVariants] = {64, 32, 64, 64, 120, 120};
-constexpr int kInvVariantTileH[kDwtVariants] = {64, 32, 56, 48, 32, 32};
+constexpr int kFwdVariantTileW[kDwtVariants] = {128, 32, 32, 128, 120, 120};
+constexpr int kFwdVariantTileH[kDwtVariants] = {56, 32, 16, 40, 32, 32};
+constexpr int kInvVariantTileW[kDwtVariants] = {64, 32, 16, 64, 120, 120};
+constexpr int kInvVariantTileH[kDwtVariants] = {64, 32, 16, 48, 32, 32};
 
 // Largest coded 32x32 block: 16 8x8 blocks x 8 subblocks x 14 planes, 1024
 // sign bits, 8 B header and 3 B per 8x8 block, rounded up to words.
@@ -214,14 +217,19 @@ constexpr int kFusedMinTiles = 16384;
 // tile is kInvFusedTileW plane samples wide and kInvFusedTallH high in
 // launches of at least kInvFusedTallTiles such tiles, kInvFusedShortH high
 // otherwise (measured, 4K 4:2:0 iDWT per frame: batches of 32, 15.1 against
-// 15.5 us; one frame, 39.4 against 41.7 us).  Launches of fewer than
-// kInvFusedMinTiles short tiles run level by level (a lone 1080p frame has
-// 832 and takes the same time either way).
+// 15.5 us; one frame, 39.4 against 41.7 us), kInvFusedLoneH high in launches
+// of fewer than kInvFusedLoneTiles short tiles, where twice the waves with
+// half a short tile's walk each outweigh the longer apron (lone frames,
+// encode + decode: 1080p FP16 101.4 -> 98.9 us, 4K FP32 173.5 -> 172.5 us).
+// With these tiles every launch size is faster fused than level by level
+// (lone 256x144 to 1280x720 frames 0.7 to 2.4 us), so the inverse always
+// fuses.
 constexpr int kInvFusedTileW = 120;
 constexpr int kInvFusedTallH = 64;
 constexpr int kInvFusedShortH = 32;
+constexpr int kInvFusedLoneH = 16;
 constexpr int kInvFusedTallTiles = 16384;
-constexpr int kInvFusedMinTiles = 512;
+constexpr int kInvFusedLoneTiles = 4096;
 
 // One transmitted band of one frame, for the per-32x32-block kernels.
 struct BandJob

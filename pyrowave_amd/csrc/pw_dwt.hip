@@ -2263,7 +2263,7 @@ void launch_dwt_forward(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 		hipLaunchKernelGGL(k_dwt_forward_stream, sgrid, sblock, 0, stream, jobs, tile_job, frames, tiles,
 		                   xcd_map(sgrid));
 	else if (variant == 1)
-		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[1], kFwdVariantTileH[1], true>), tgrid,
+		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[1], kFwdVariantTileH[1], false>), tgrid,
 		                   dim3(2 * kFwdVariantTileW[1]), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 2)
 		hipLaunchKernelGGL((k_dwt_forward<kFwdVariantTileW[2], kFwdVariantTileH[2], false>), tgrid,
@@ -2294,8 +2294,11 @@ void launch_dwt_inverse_fused(const FusedJob *jobs, const uint16_t *tile_job, in
 	if (tile_h == kInvFusedTallH)
 		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedTallH>, grid, block, 0, stream, jobs, tile_job, frames,
 		                   tiles, xcd_map(grid));
-	else
+	else if (tile_h == kInvFusedShortH)
 		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedShortH>, grid, block, 0, stream, jobs, tile_job, frames,
+		                   tiles, xcd_map(grid));
+	else
+		hipLaunchKernelGGL(k_dwt_inverse_fused<kInvFusedLoneH>, grid, block, 0, stream, jobs, tile_job, frames,
 		                   tiles, xcd_map(grid));
 }
 
@@ -2315,8 +2318,8 @@ void launch_dwt_inverse(const DwtJob *jobs, const uint16_t *tile_job, int tiles,
 		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[1], kInvVariantTileH[1], 128, 16, false>), tgrid,
 		                   dim3(128), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 2)
-		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 256, 16, false>), tgrid,
-		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
+		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[2], kInvVariantTileH[2], 64, 16, false>), tgrid,
+		                   dim3(64), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));
 	else if (variant == 3)
 		hipLaunchKernelGGL((k_dwt_inverse<kInvVariantTileW[3], kInvVariantTileH[3], 256, 24, true>), tgrid,
 		                   dim3(256), 0, stream, jobs, tile_job, frames, xcd_map(tgrid));

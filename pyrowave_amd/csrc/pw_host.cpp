@@ -538,16 +538,23 @@ bool Plan::build(const std::vector<Slot *> &slots_, bool encoder_, PlanStorage *
 	has_fused = !half_math;
 	// Inverse tile height: tall tiles where the launch has waves enough.
 	inv_fused_tile_h = kInvFusedShortH;
+	inv_fused_short_tiles = 0;
 	if (has_fused && !encoder)
 	{
-		long tall = 0;
+		long tall = 0, short_tiles = 0;
 		for (Slot *S : slots)
 			for (int c = 0; c < kComponents; c++)
 				if (fusable(S->geom, c))
-					tall += long((S->geom.plane_w(c) + kInvFusedTileW - 1) / kInvFusedTileW) *
-					        ((S->geom.plane_h(c) + kInvFusedTallH - 1) / kInvFusedTallH);
+				{
+					const long tx = (S->geom.plane_w(c) + kInvFusedTileW - 1) / kInvFusedTileW;
+					tall += tx * ((S->geom.plane_h(c) + kInvFusedTallH - 1) / kInvFusedTallH);
+					short_tiles += tx * ((S->geom.plane_h(c) + kInvFusedShortH - 1) / kInvFusedShortH);
+				}
+		inv_fused_short_tiles = int(std::min(short_tiles, long(INT32_MAX)));
 		if (tall >= kInvFusedTallTiles)
 			inv_fused_tile_h = kInvFusedTallH;
+		else if (short_tiles < kInvFusedLoneTiles)
+			inv_fused_tile_h = kInvFusedLoneH;
 	}
 
 	for (int f = 0; f < n; f++)
@@ -995,7 +1002,7 @@ void run_decode(Plan &P, hipStream_t stream, bool timed)
 	const int reps = timed ? D.tune_reps : 0;
 	D.begin_stage(stream, kStageIDWT, n, &ev);
 	if (P.has_fused && D.forced_variant < 0 && reps == 0 &&
-	    (D.fused >= 2 || (D.fused == 1 && P.fused_tiles >= kInvFusedMinTiles)))
+	    D.fused >= 1)
 	{
 		// The levels above each component's first two with their tuned
 		// shapes, then those two in one launch.

@@ -144,8 +144,9 @@ struct Device
 	int forced_variant = -1; // >= 0: every transform level uses this variant
 	bool fused_resolve = true; // rate control of a frame in one launch (k_rc_resolve) where it fits
 	// A component's first two transform levels in one launch, forward
-	// (k_dwt_forward_fused) and inverse (k_dwt_inverse_fused): 0 never, 1 for
-	// launches of at least kFusedMinTiles / kInvFusedMinTiles tiles, 2 always.
+	// (k_dwt_forward_fused) and inverse (k_dwt_inverse_fused): 0 never, 1 the
+	// forward for launches of at least kFusedMinTiles tiles and the inverse
+	// always, 2 both always.
 	int fused = 1;
 	int tune_reps = 0;       // > 0: the next eager encode / decode autotunes
 	// PSNR-HVS-M-H tables per image height (device copies).
@@ -256,7 +257,8 @@ struct Plan
 	FusedJob *d_fused_jobs = nullptr;
 	uint16_t *d_fused_tile_job = nullptr;
 	int fused_tiles = 0;
-	int inv_fused_tile_h = kInvFusedShortH; // decoder plans: kInvFusedTallH or kInvFusedShortH
+	int inv_fused_tile_h = kInvFusedShortH; // decoder plans: kInvFusedTallH, kInvFusedShortH or kInvFusedLoneH
+	int inv_fused_short_tiles = 0;          // decoder plans: fused tiles at kInvFusedShortH rows
 	DwtJob *d_rest_jobs[kLevels][kDwtVariants] = {};
 	uint16_t *d_rest_tile_job[kLevels][kDwtVariants] = {};
 	int rest_tiles[kLevels][kDwtVariants] = {};

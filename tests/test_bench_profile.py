@@ -54,4 +54,4 @@ def test_fused_inverse_tiles():
     # chroma planes: 32 x 34 + 2 x 16 x 17 (64 rows), 32 x 68 + 2 x 16 x 34 (32 rows).
     assert bench.inv_fused_tiles(1, 64) == 1632
     assert bench.inv_fused_tiles(1, 32) == 3264
-    assert bench.inv_fused_tile_h(32) == 64 and bench.inv_fused_tile_h(8) == 32
+    assert bench.inv_fused_tile_h(32) == 64 and bench.inv_fused_tile_h(8) == 32 and bench.inv_fused_tile_h(1) == 16
