@@ -1656,7 +1656,19 @@ PYROWAVE_PUBLIC_API pyrowave_result pyrowave_hip_encoder_read_rate_control(pyrow
 	if (buckets)
 		(void)hipMemcpy(buckets, S.fs.bucket, n32 * 16, hipMemcpyDeviceToHost);
 	if (totals)
-		(void)hipMemcpy(totals, S.fs.totals, kFlat * 4, hipMemcpyDeviceToHost);
+	{
+		// The quantizer's copies of the totals, summed.
+		const int ncopies = totals_copies(S.geom.count32);
+		std::vector<uint32_t> copies(size_t(ncopies) * kFlat);
+		(void)hipMemcpy(copies.data(), S.fs.totals, copies.size() * 4, hipMemcpyDeviceToHost);
+		for (int x = 0; x < kFlat; x++)
+		{
+			uint32_t s = 0;
+			for (int r = 0; r < ncopies; r++)
+				s += copies[size_t(r) * kFlat + x];
+			totals[x] = s;
+		}
+	}
 	if (prefix)
 		(void)hipMemcpy(prefix, S.fs.prefix, kFlat * 4, hipMemcpyDeviceToHost);
 	if (consumed)

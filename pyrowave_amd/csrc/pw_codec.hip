@@ -545,7 +545,8 @@ __global__ __launch_bounds__(kBlockLanes) void k_quant_analyze(const BandJob *__
 		{
 			uint32_t saving = s_words[q - 1] - s_words[q];
 			if (saving)
-				atomicAdd(&F.totals[Bq * kSubdiv + ((uint32_t)C.b >> F.shamt)], saving);
+				atomicAdd(&F.totals[(C.b & (F.totals_copies - 1)) * kFlat + Bq * kSubdiv + ((uint32_t)C.b >> F.shamt)],
+				          saving);
 		}
 		else
 			// Unquantized words, also into a contiguous per-block array that
@@ -556,32 +557,9 @@ __global__ __launch_bounds__(kBlockLanes) void k_quant_analyze(const BandJob *__
 	}
 }
 
-// Grid (clear blocks, frames of the parameter block).  Block (0, i) writes
-// frame i's per-call fields; every block clears its share of the frame's
-// rate-control accumulators (encoder frames only: a decoder's FrameState has
-// no rate-control arrays).  A kernel, not memsets: it is captured into graphs.
-__global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, int first, int count)
+// A frame's per-call fields, written by one thread.
+__device__ __forceinline__ void write_call_fields(FrameState &F, const CallParams &P)
 {
-	const int i = blockIdx.y;
-	if (i >= count)
-		return;
-	FrameState &F = frames[first + i];
-	if (F.totals)
-	{
-		const int n = F.n32 > kFlat ? F.n32 : kFlat;
-		for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-		{
-			if (k < kFlat)
-				F.totals[k] = 0;
-			if (k < F.n32)
-				F.partial_quant[k] = 0;
-			if (k < kRcCount)
-				F.rc[k] = 0;
-		}
-	}
-	if (blockIdx.x != 0 || threadIdx.x != 0)
-		return;
-	const CallParams &P = block.p[i];
 	for (int c = 0; c < 3; c++)
 	{
 		F.src[c] = P.src[c];
@@ -597,6 +575,35 @@ __global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, 
 	F.dec_payload = P.dec_payload;
 	F.dec_offset_stride = P.dec_offset_stride;
 	F.dec_payload_bytes = P.dec_payload_bytes;
+}
+
+// Grid (clear blocks, frames of the parameter block).  Block (0, i) writes
+// frame i's per-call fields; every block clears its share of the frame's
+// rate-control accumulators (encoder frames only: a decoder's FrameState has
+// no rate-control arrays).  A kernel, not memsets: it is captured into graphs.
+__global__ void k_set_params(FrameState *__restrict__ frames, ParamBlock block, int first, int count)
+{
+	const int i = blockIdx.y;
+	if (i >= count)
+		return;
+	FrameState &F = frames[first + i];
+	if (F.totals)
+	{
+		const int kTotals = F.totals_copies * kFlat;
+		const int n = F.n32 > kTotals ? F.n32 : kTotals;
+		for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
+		{
+			if (k < kTotals)
+				F.totals[k] = 0;
+			if (k < F.n32)
+				F.partial_quant[k] = 0;
+			if (k < kRcCount)
+				F.rc[k] = 0;
+		}
+	}
+	if (blockIdx.x != 0 || threadIdx.x != 0)
+		return;
+	write_call_fields(F, block.p[i]);
 }
 
 // The finalize scan of reference shaders/analyze_rate_control_finalize.comp:
@@ -641,6 +648,21 @@ __device__ __forceinline__ void window_scan512(uint32_t (&v)[4], bool exact, uin
 	__syncthreads();
 }
 
+// Totals 4t .. 4t + 3, summed over the copies.
+__device__ __forceinline__ uint4 sum_total_copies(const uint32_t *totals, int copies, int t)
+{
+	uint4 s = make_uint4(0u, 0u, 0u, 0u);
+	for (int r = 0; r < copies; r++)
+	{
+		const uint4 c = reinterpret_cast<const uint4 *>(totals + r * kFlat)[t];
+		s.x += c.x;
+		s.y += c.y;
+		s.z += c.z;
+		s.w += c.w;
+	}
+	return s;
+}
+
 // reference shaders/analyze_rate_control_finalize.comp:14-38 (512 threads x 4
 // entries) and the skip / required logic of shaders/resolve_rate_control.comp:38-56.
 __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ frames)
@@ -673,10 +695,9 @@ __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ fr
 		if (t == 0)
 			F.rc[kRcConsumed] = s_consumed;
 	}
-	uint32_t v[4];
-#pragma unroll
-	for (int i = 0; i < 4; i++)
-		v[i] = F.totals[4 * t + i];
+	const uint4 tv = sum_total_copies(F.totals, F.totals_copies, t);
+	const uint32_t total4[4] = {tv.x, tv.y, tv.z, tv.w};
+	uint32_t v[4] = {tv.x, tv.y, tv.z, tv.w};
 	v[1] += v[0];
 	v[2] += v[1];
 	v[3] += v[2];
@@ -702,7 +723,7 @@ __global__ __launch_bounds__(512) void k_rc_finalize(FrameState *__restrict__ fr
 		}
 		uint8_t cls = kRcSkip;
 		if (!empty && required > 0)
-			cls = (int)F.totals[x] <= required ? kRcFull : kRcPartial;
+			cls = (int)total4[i] <= required ? kRcFull : kRcPartial;
 		F.cls[x] = cls;
 		F.required[x] = required;
 		prev = v[i];
@@ -846,7 +867,7 @@ __global__ __launch_bounds__(1024) void k_rc_resolve(FrameState *__restrict__ fr
 	uint32_t v[4] = {0u, 0u, 0u, 0u};
 	if (scanner)
 	{
-		const uint4 tv = reinterpret_cast<const uint4 *>(F.totals)[t];
+		const uint4 tv = sum_total_copies(F.totals, F.totals_copies, t);
 		v[0] = tv.x;
 		v[1] = tv.y;
 		v[2] = tv.z;
@@ -1262,10 +1283,16 @@ __global__ __launch_bounds__(kBlockLanes) void k_pack(const BandJob *__restrict_
 // first copied to LDS with coalesced loads and parsed from there.  A malformed
 // block that reads past its own length reads the following payload bytes, as
 // the reference does; bytes past the payload read as zero.
+//
+// With nparams > 0 the call's fields of frames 0 .. nparams - 1 come in
+// `block` (the k_set_params launch of a small decode folded into this one):
+// the frame's decode inputs are read from there, and the wave of each
+// frame's block 0 writes the fields into its FrameState for the transform
+// launches that follow.
 __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restrict__ bands,
                                                          const uint32_t *__restrict__ blk_band,
-                                                         const FrameState *__restrict__ frames,
-                                                         XcdMap xm)
+                                                         FrameState *__restrict__ frames,
+                                                         XcdMap xm, ParamBlock block, int nparams)
 {
 	__shared__ uint32_t s_pay[kMaxBlockWords];
 	__shared__ uint32_t s_poff[16];
@@ -1282,9 +1309,14 @@ __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restri
 
 	// The frame's decode inputs are read together, and a meta pair (offset,
 	// words) as one 8-byte load: each dependent lookup costs a round trip.
-	const uint32_t *const offsets = F.dec_offsets;
-	const int offset_stride = F.dec_offset_stride;
-	const uint32_t payload_words = F.dec_payload_bytes >> 2;
+	const bool folded = J.frame < nparams; // wave-uniform
+	const CallParams &CP = block.p[folded ? J.frame : 0];
+	if (folded && C.b == 0 && tid == 0)
+		write_call_fields(frames[J.frame], CP);
+	const uint32_t *const offsets = folded ? CP.dec_offsets : F.dec_offsets;
+	const int offset_stride = folded ? CP.dec_offset_stride : F.dec_offset_stride;
+	const uint32_t payload_words = (folded ? CP.dec_payload_bytes : F.dec_payload_bytes) >> 2;
+	const uint32_t *const dec_payload = folded ? CP.dec_payload : F.dec_payload;
 	uint32_t off;
 	bool missing;
 	if (offset_stride == 2)
@@ -1308,7 +1340,7 @@ __global__ __launch_bounds__(kBlockLanes) void k_dequant(const BandJob *__restri
 		v[p] = 0.0f;
 	if (!missing)
 	{
-		const uint32_t *payload = F.dec_payload + off;
+		const uint32_t *payload = dec_payload + off;
 		// The first kBlockLanes words (one per lane; most blocks are not much
 		// longer) load together with the header word that gives the length,
 		// so a typical block arrives in one or two round trips.
@@ -1526,7 +1558,8 @@ void launch_set_params(FrameState *frames, const CallParams *params, int nframes
 	int blocks = 1;
 	if (max_n32 > 0)
 	{
-		const int n = max_n32 > kFlat ? max_n32 : kFlat;
+		const int totals = totals_copies(max_n32) * kFlat;
+		const int n = max_n32 > totals ? max_n32 : totals;
 		blocks = (n + 255) / 256;
 		blocks = blocks > 64 ? 64 : blocks;
 	}
@@ -1580,11 +1613,17 @@ void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, Fr
 		                   xcd_map(dim3(nblocks)));
 }
 
-void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
-                    hipStream_t stream)
+void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
+                    hipStream_t stream, const CallParams *params, int nparams)
 {
-	if (nblocks > 0)
-		hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockLanes), 0, stream, bands, blk_band, frames,
-		                   xcd_map(dim3(nblocks)));
+	if (nblocks <= 0)
+		return;
+	ParamBlock block;
+	if (nparams < 0 || nparams > kParamsPerLaunch)
+		nparams = 0;
+	for (int i = 0; i < nparams; i++)
+		block.p[i] = params[i];
+	hipLaunchKernelGGL(k_dequant, dim3(nblocks), dim3(kBlockLanes), 0, stream, bands, blk_band, frames,
+	                   xcd_map(dim3(nblocks)), block, nparams);
 }
 } // namespace pw

@@ -19,6 +19,25 @@ constexpr int kMinimumImageSize = 4 << kLevels;
 constexpr int kSubdiv = 16;
 constexpr int kBuckets = 128;
 constexpr int kFlat = kBuckets * kSubdiv;
+// The quantizer adds each block's savings into one of several copies of the
+// 2048 totals (block b into copy b % copies), and the finalize sums the
+// copies: a frame's blocks meet on few (bucket, subdivision) entries, and
+// device-scope atomics on one address serialize (a lone 4K frame's quantizer
+// took 39 us with one copy, 20 with sixteen).  One copy per
+// kBlocksPerTotalsCopy blocks, a power of two up to kTotalsCopies (a lone
+// 1080p frame, 3261 blocks, keeps one).  Integer sums in any order, so the
+// totals are the same.
+constexpr int kTotalsCopies = 16;
+constexpr int kBlocksPerTotalsCopy = 4096;
+constexpr int totals_copies(int n32)
+{
+	int c = 1;
+	while (c < kTotalsCopies && c * kBlocksPerTotalsCopy < n32)
+		c *= 2;
+	return c;
+}
+static_assert(totals_copies(3261) == 1 && totals_copies(12429) == 4 && totals_copies(1 << 20) == kTotalsCopies,
+              "copies by frame size");
 
 // DWT tiles, one workgroup each, with a 4-sample apron on every side.  Two
 // tile shapes per direction; the autotuner picks one per level (both give
@@ -46,8 +65,9 @@ constexpr int kFwdStreamVariant = 4;
 constexpr int kInvStreamVariant = 4;
 constexpr int kHalfStreamVariant = 5;
 // Shape of a level that has not been autotuned: the streamed one when the
-// level has at least kStreamMinTiles of its tiles (eight waves per CU; it is
-// the fastest on the large levels of a batch), else a tiled shape (a
+// level has at least kStreamMinTiles of its tiles (six waves per CU; it is
+// the fastest on the large levels of a batch and on level 1 of a lone 4K
+// 4:2:0 frame, 1632 tiles), else a tiled shape (a
 // streamed wave walks its rows one after another, which a level with few
 // tiles cannot hide: one 1080p frame took 176 instead of 129 us): the small
 // tiles the autotuner keeps on the levels of lone 720p, 1080p and 4K frames,
@@ -55,7 +75,7 @@ constexpr int kHalfStreamVariant = 5;
 // inverse 32x32 (shape 1) from 128 and 16x16 (shape 2) below (a lone 1080p
 // FP16 frame 118 -> 101 us against shape 0 everywhere).  The autotuner keeps
 // the default unless another shape is 2% faster.
-constexpr int kStreamMinTiles = 2048;
+constexpr int kStreamMinTiles = 1536;
 static_assert(kFwdStreamVariant == kInvStreamVariant, "one streamed shape index");
 constexpr int default_dwt_variant(bool forward, int stream_tiles)
 {
@@ -117,7 +137,7 @@ struct FrameState
 	uint32_t *group_sum;    // [ngroups] words per 256-block group
 	uint32_t *meta;         // [n32][2] offset (words), words
 	uint32_t *bitstream;    // coded blocks in block-index order
-	uint32_t *totals;       // [2048] savings per (bucket, subdivision)
+	uint32_t *totals;       // [totals_copies][2048] savings per (bucket, subdivision)
 	uint32_t *prefix;       // [2048] finalized scan
 	uint8_t *cls;           // [2048] RcClass
 	int32_t *required;      // [2048] savings still required at each entry
@@ -142,6 +162,7 @@ struct FrameState
 	int32_t per_sub;
 	int32_t shamt;
 	int32_t ngroups;
+	int32_t totals_copies; // totals_copies(n32)
 	uint32_t capacity_words;
 	int32_t width, height;
 	uint32_t flags;   // bit 0: exact prefix in rate-control finalize
@@ -388,8 +409,10 @@ void launch_rc_prepare(FrameState *frames, const uint32_t *grp_frame, int ngroup
                        const int32_t *frame_group_base, hipStream_t stream);
 void launch_pack(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
                  hipStream_t stream);
-void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, const FrameState *frames,
-                    hipStream_t stream);
+// With nparams in 1 .. kParamsPerLaunch, also writes those frames' per-call
+// fields (params[0 .. nparams - 1]; no k_set_params launch before it).
+void launch_dequant(const BandJob *bands, const uint32_t *blk_band, int nblocks, FrameState *frames,
+                    hipStream_t stream, const CallParams *params = nullptr, int nparams = 0);
 // Writes the per-call fields and, for encoder frames, clears the rate-control
 // accumulators (max_n32: the largest frame's 32x32 block count, 0 to skip).
 void launch_set_params(FrameState *frames, const CallParams *params, int nframes, int max_n32, hipStream_t stream);

@@ -299,7 +299,7 @@ float decode_quant(uint8_t code);
 
 // `timed`: record the device's per-stage timing events.
 void run_encode(Plan &plan, hipStream_t stream, bool timed = true);
-void run_decode(Plan &plan, hipStream_t stream, bool timed = true);
+void run_decode(Plan &plan, hipStream_t stream, bool timed = true, bool capture = false);
 // Replays the plan's captured launch sequence (capturing it on first use).
 bool run_graph(Plan &plan, hipStream_t stream);
 // Autotuner: while Device::tune_reps > 0, run_encode / run_decode time every
