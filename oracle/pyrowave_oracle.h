@@ -44,12 +44,15 @@
  *   distortion weight ............ published Mannos-Sakrison CSF at the band
  *                                  midpoints
  *   quant resolution, band term .. measured analysis gains of the FP64 9/7
+ *   quant resolution .............. noise-power normalization: 2^(9 + log2 of
+ *                                  the band's FP64 synthesis-basis L2 norm) at
+ *                                  its stage count in its own plane (band,
+ *                                  level and 4:2:0 chroma terms, base bits)
  *   (tests/test_oracle_pinning.py; tests/test_oracle_mutants.py plants a
  *   mistake in each of these and requires a check to fail)
- *   parity unpinned, constants only: the level and chroma terms of the quant
- *   resolution (+1 bit per level, -1 bit for chroma) and its caps
- *   (pyrowave_encoder.cpp:144-172) are the reference encoder's choices, fixed
- *   by no bitstream rule or mathematical property; they are transcribed.
+ *   transcribed constants (pyrowave_encoder.cpp:144-172), fixed by no
+ *   bitstream rule or mathematical property: one bit less for 4:4:4 chroma
+ *   and the caps 4096 / 512; the check above asserts the capped values.
  *   finalize (quirk Q1) .......... literal simulation of the reference's
  *                                  512-thread Hillis-Steele loop
  *   resolve ...................... literal simulation of the reference's

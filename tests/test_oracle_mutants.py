@@ -47,6 +47,10 @@ MUTANTS = [
      "bi = bucket_of_ratio((base_f - cost_f) / (dd > 0.0f ? dd : 1.0f));"),
     ("savings_subdivision", "uint32_t subdiv = block >> shamt;", "uint32_t subdiv = (block >> shamt) & 7u;"),
     ("band_term", "else if (band < 3)\n\t\tbits += 1;", "else if (band < 3)\n\t\tbits += 2;"),
+    ("level_term", "bits += level;", "bits += level + (level > 2);"),
+    ("chroma_term_dropped", "if (component != 0)\n\t\tbits--;", "if (component > 2)\n\t\tbits--;"),
+    ("base_bits", "int bits = precision0 ? 6 : 8;", "int bits = precision0 ? 6 : 7;"),
+    ("base_bits_precision0", "int bits = precision0 ? 6 : 8;", "int bits = precision0 ? 5 : 8;"),
 ]
 
 

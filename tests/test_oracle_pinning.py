@@ -17,7 +17,9 @@ Each check_* function recomputes one oracle output from an independent route:
 - the distortion weight (pyrowave_encoder.cpp:107-142) from the published
   Mannos-Sakrison contrast sensitivity function at the band midpoints;
 - the band term of the quant resolution (pyrowave_encoder.cpp:152-172) from
-  the measured analysis gains of the FP64 CDF 9/7 transform.
+  the measured analysis gains of the FP64 CDF 9/7 transform, and the whole
+  resolution (band, level and 4:2:0 chroma terms, base bits) from the L2 norms
+  of the FP64 synthesis bases (noise-power normalization).
 
 tests/test_oracle_mutants.py plants a mistake in each of these oracle
 functions and requires one of these checks to fail.
@@ -373,13 +375,62 @@ def check_quant_resolution_band_term():
             ref = bits[3] + math.log2(g[3])
             for b in range(3):
                 assert abs(bits[b] + math.log2(g[b]) - ref) < 0.35, (l, c, b, bits, g)
-    # Every level doubles the resolution of each band below the cap.
-    for c in range(3):
-        for b in range(1, 4):
-            r = [lib.or_quant_resolution(l, c, b) for l in range(5)]
-            for l in range(4):
-                if r[l + 1] < 4096:
-                    assert r[l + 1] == 2 * r[l]
+    return True
+
+
+def _synthesis_norms(N=256):
+    """log2 of the L2 norm of the synthesis basis of band b after s analysis
+    stages (s = 1..5): a unit impulse at the band's centre, run through one
+    FP64 inverse level and then s - 1 inverse levels of LL only."""
+    out = {}
+    for s in range(1, 6):
+        for b in range(4):
+            n = N >> s
+            bands = [np.zeros((n, n)) for _ in range(4)]
+            bands[b][n // 2, n // 2] = 1.0
+            x = O.idwt_level(bands)
+            while x.shape[0] < N:
+                m = x.shape[0]
+                z = np.zeros((m, m))
+                x = O.idwt_level([x, z, z, z])
+            out[s, b] = 0.5 * math.log2(float((x ** 2).sum()))
+    return out
+
+
+def check_quant_resolution_noise_power():
+    """The quant resolution is noise-power normalized (pyrowave_encoder.cpp:
+    152-172): a quantization step of 1/resolution in any band reconstructs with
+    the same L2 norm, 2^-9 (half of one 8-bit step) for the precision >= 1
+    policy and 2^-7 for precision 0.  So resolution = 2^(9 + log2 of the
+    synthesis basis norm), rounded to a power of two, with the norm measured
+    on the FP64 transform at the band's stage count in its own plane (read
+    from the layout: a 4:2:0 chroma band at level l has passed l stages, a
+    luma band l + 1).  This fixes the band term, the bit per level, the 4:2:0
+    chroma bit and the base bits at once.  Transcribed, not fixed by this
+    rule: one bit less for 4:4:4 chroma, and the caps 4096 / 512."""
+    norms = _synthesis_norms()
+    lib = O.lib()
+    for chroma in (0, 1):
+        L = O.layout(512, 512, chroma)
+        for c, l, b, B in _bands(L):
+            ph, pw = O.plane_shape(L, c)
+            s = int(round(math.log2(pw / B.bw)))
+            assert pw == B.bw << s
+            for precision0, base, cap in ((0, 9, 4096.0), (1, 7, 512.0)):
+                want = base + norms[s, b] - (1 if c and chroma == 1 else 0)
+                got = lib.or_quant_resolution_policy(l, c, b, precision0)
+                if got == cap:
+                    assert want > math.log2(cap) - 0.25, (chroma, c, l, b, precision0, got, want)
+                else:
+                    assert got < cap and abs(math.log2(got) - want) < 0.25, (chroma, c, l, b, precision0, got, want)
+    # The distortion weight takes the resolution before the cap: for a band
+    # at the cap, rdo / CSF^2 * 256 is still 4^(9 + norm).
+    L = O.layout(512, 512, 0)
+    B = L.band[0][4][0]
+    assert B.present
+    csf = mannos_sakrison(max(math.hypot(0.25, 0.25) * 0.34 * 96.0 / 16.0, 8.0))
+    unc = math.sqrt(float(lib.or_rdo_distortion_scale(C_L(L), 4, 0, 0)) * 256.0) / csf
+    assert abs(math.log2(unc) - 9 - norms[5, 0]) < 0.25
     return True
 
 
@@ -412,7 +463,8 @@ def check_rdo_weights():
 
 CHECKS = [check_distortion_through_dequant, check_dist32_is_sum_of_blocks, check_cost_through_pack,
           check_truncation_bound, check_scale_codes, check_bucket_of_ratio, check_bucket_scan, check_bucket_tables,
-          check_totals_and_consumed, check_quant_resolution_band_term, check_rdo_weights]
+          check_totals_and_consumed, check_quant_resolution_band_term, check_quant_resolution_noise_power,
+          check_rdo_weights]
 
 
 @pytest.mark.parametrize("check", CHECKS, ids=[c.__name__ for c in CHECKS])
