@@ -89,7 +89,9 @@ void release(Mem *m)
 			(void)hipFree(m->ptr);
 		if (m->ext)
 			(void)hipDestroyExternalMemory(m->ext);
-		if (m->fd >= 0)
+		// Closed here only while still open: a runtime that owns the
+		// imported descriptor closes it on destroy.
+		if (m->fd >= 0 && fcntl(m->fd, F_GETFD) != -1)
 			(void)close(m->fd);
 	}
 	if (m->device)

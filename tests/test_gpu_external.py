@@ -59,6 +59,46 @@ def _shared(device, nbytes):
     return a, b
 
 
+def _open_fds():
+    return set(int(x) for x in os.listdir("/proc/self/fd"))
+
+
+def test_import_descriptor_ownership(device):
+    """Who closes the duplicate descriptor an import hands to the runtime:
+    across import and release, the process's open descriptors come back to
+    the set before the import (nothing leaks, nothing else is closed), and a
+    descriptor opened while the import is alive (it may take the number the
+    runtime gave up) survives the release."""
+    try:
+        a = pw.ExternalMemory.allocate(device, 1 << 20)
+    except pw.PyroWaveError as e:
+        pytest.skip(f"exportable allocation not available: {e}")
+    try:
+        # A first import and release: descriptors the runtime opens once and
+        # keeps (a render node) are open before the measured one.
+        try:
+            pw.ExternalMemory.import_fd(device, a.fd, a.size).close()
+        except pw.PyroWaveError as e:
+            pytest.skip(f"external memory import not available: {e}")
+        torch.cuda.synchronize()
+        before = _open_fds()
+        b = pw.ExternalMemory.import_fd(device, a.fd, a.size)
+        during = _open_fds()
+        sentinel = os.open("/dev/null", os.O_RDONLY)
+        try:
+            b.close()
+            torch.cuda.synchronize()
+            os.fstat(sentinel)  # raises if the release closed it
+            after = _open_fds() - {sentinel}
+        finally:
+            os.close(sentinel)
+        assert after == before, (sorted(before), sorted(during), sorted(after))
+        os.fstat(a.fd)  # the caller's own descriptor stays open
+    finally:
+        a.close()
+        os.close(a.fd)
+
+
 def _planar_layout():
     pitch = [320, 192, 192]
     off = [0, 320 * H, 320 * H + 192 * (H // 2)]
